@@ -1,0 +1,256 @@
+"""Python binding of libdenise_sh.so (ctypes over the C-ABI in include/denise_sh.h).
+
+Plumbing for tests and benchmarks: every computation runs in the HIP library;
+this module only marshals numpy arrays.  Loading fails loudly if the library
+has not been built (python denise-sh_amd/build.py).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdenise_sh.so")
+
+# Every symbol include/denise_sh.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "dn_set_globals", "update_v_PML", "update_s_elastic_PML", "update_s_visc_PML", "seismo_ssg", "psource",
+    "dn_create", "dn_destroy", "dn_last_error", "dn_set_model", "dn_get_model", "dn_set_receivers",
+    "dn_set_shots", "dn_set_observed", "dn_set_batch", "dn_forward", "dn_gradient", "dn_get_gradient",
+    "dn_forward_fields", "dn_last_timing",
+]
+
+
+class Globals(C.Structure):
+    _fields_ = [("NX", C.c_int), ("NY", C.c_int), ("FDORDER", C.c_int), ("FW", C.c_int), ("FREE_SURF", C.c_int),
+                ("BOUNDARY", C.c_int), ("L", C.c_int), ("GRAD_FORM", C.c_int), ("QUELLTYPB", C.c_int),
+                ("NT", C.c_int), ("DT", C.c_float), ("DH", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("maxrelerror", C.c_int), ("fw", C.c_int),
+                ("free_surf", C.c_int), ("L", C.c_int), ("fl", C.c_float * 4), ("dt", C.c_float), ("dh", C.c_float),
+                ("damping", C.c_float), ("fpml", C.c_float), ("npower", C.c_float), ("k_max_pml", C.c_float),
+                ("dtinv", C.c_int), ("snap_precision", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libdenise_sh.so (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libdenise_sh.so not built: run python denise-sh_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        fp = C.POINTER(C.c_float)
+        ip = C.POINTER(C.c_int)
+        L.dn_create.restype = C.c_void_p
+        L.dn_create.argtypes = [C.POINTER(Params)]
+        L.dn_destroy.argtypes = [C.c_void_p]
+        L.dn_last_error.restype = C.c_char_p
+        L.dn_last_error.argtypes = [C.c_void_p]
+        L.dn_set_model.argtypes = [C.c_void_p, fp, fp, fp]
+        L.dn_get_model.argtypes = [C.c_void_p, fp, fp]
+        L.dn_set_receivers.argtypes = [C.c_void_p, C.c_int, ip, ip]
+        L.dn_set_shots.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, fp]
+        L.dn_set_observed.argtypes = [C.c_void_p, fp]
+        L.dn_set_batch.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.dn_forward.argtypes = [C.c_void_p, C.c_int, C.c_int, fp]
+        L.dn_gradient.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+        L.dn_get_gradient.argtypes = [C.c_void_p, fp, fp, fp, fp]
+        L.dn_forward_fields.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp, fp]
+        L.dn_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                     C.POINTER(C.c_longlong)]
+        L.dn_set_globals.argtypes = [C.POINTER(Globals)]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class DeniseError(RuntimeError):
+    pass
+
+
+class Context:
+    """Handle API wrapper: one dn_ctx on the current HIP device."""
+
+    def __init__(self, nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, fl=(), dt=1e-4, dh=0.25,
+                 damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, dtinv=1, snap_precision=0):
+        p = Params()
+        p.nx, p.ny, p.fdorder, p.maxrelerror, p.fw, p.free_surf, p.L = nx, ny, fdorder, maxrelerror, fw, free_surf, L
+        for k, v in enumerate(fl):
+            p.fl[k] = v
+        p.dt, p.dh, p.damping, p.fpml, p.npower, p.k_max_pml = dt, dh, damping, fpml, npower, k_max_pml
+        p.dtinv, p.snap_precision = dtinv, snap_precision
+        self.p = p
+        self.nx, self.ny = nx, ny
+        self._h = lib().dn_create(C.byref(p))
+        if not self._h:
+            raise DeniseError(lib().dn_last_error(None).decode())
+        self.ntr = 0
+        self.nshots = 0
+        self.nt_max = 0
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise DeniseError(lib().dn_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().dn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, vs, rho, qs=None):
+        vs = np.ascontiguousarray(vs, np.float32)
+        rho = np.ascontiguousarray(rho, np.float32)
+        q = None if qs is None else np.ascontiguousarray(qs, np.float32)
+        self._chk(lib().dn_set_model(self._h, _f(vs), _f(rho), _f(q)))
+
+    def get_model(self):
+        vs = np.zeros((self.ny, self.nx), np.float32)
+        rho = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(lib().dn_get_model(self._h, _f(vs), _f(rho)))
+        return vs, rho
+
+    def set_receivers(self, rec_i, rec_j):
+        ri = np.ascontiguousarray(rec_i, np.int32)
+        rj = np.ascontiguousarray(rec_j, np.int32)
+        self._chk(lib().dn_set_receivers(self._h, len(ri), _i(ri), _i(rj)))
+        self.ntr = len(ri)
+
+    def set_shots(self, src_i, src_j, signals, nt=None):
+        sig = np.ascontiguousarray(signals, np.float32)
+        ns, ntm = sig.shape
+        nt = np.full(ns, ntm, np.int32) if nt is None else np.ascontiguousarray(nt, np.int32)
+        si = np.ascontiguousarray(src_i, np.int32)
+        sj = np.ascontiguousarray(src_j, np.int32)
+        self._chk(lib().dn_set_shots(self._h, ns, ntm, _i(nt), _i(si), _i(sj), _f(sig)))
+        self.nshots, self.nt_max = ns, ntm
+
+    def set_observed(self, obs):
+        obs = np.ascontiguousarray(obs, np.float32)
+        assert obs.shape == (self.nshots, self.ntr, self.nt_max)
+        self._chk(lib().dn_set_observed(self._h, _f(obs)))
+
+    def set_batch(self, max_shots=0, rows_per_segment=0):
+        self._chk(lib().dn_set_batch(self._h, max_shots, rows_per_segment))
+
+    def forward(self, shot0=0, nshots=None, want_seis=True):
+        nshots = self.nshots - shot0 if nshots is None else nshots
+        seis = np.zeros((nshots, self.ntr, self.nt_max), np.float32) if want_seis else None
+        self._chk(lib().dn_forward(self._h, shot0, nshots, _f(seis)))
+        return seis
+
+    def gradient(self, shot0=0, nshots=None, accumulate=False):
+        nshots = self.nshots - shot0 if nshots is None else nshots
+        l2 = C.c_double()
+        en = C.c_double()
+        self._chk(lib().dn_gradient(self._h, shot0, nshots, 1 if accumulate else 0, C.byref(l2), C.byref(en)))
+        return l2.value, en.value
+
+    def get_gradient(self):
+        out = [np.zeros((self.ny, self.nx), np.float32) for _ in range(4)]
+        self._chk(lib().dn_get_gradient(self._h, *[_f(a) for a in out]))
+        return out  # grad_vs, grad_rho, raw_u, raw_rho
+
+    def forward_fields(self, shot, nsteps):
+        out = [np.zeros((self.ny, self.nx), np.float32) for _ in range(3)]
+        self._chk(lib().dn_forward_fields(self._h, shot, nsteps, *[_f(a) for a in out]))
+        return out  # vy, sxy, syz
+
+    def last_timing(self):
+        ms = C.c_double()
+        la = C.c_longlong()
+        ce = C.c_longlong()
+        self._chk(lib().dn_last_timing(self._h, C.byref(ms), C.byref(la), C.byref(ce)))
+        return ms.value, la.value, ce.value
+
+
+# ---------------------------------------------------------------------------
+# Numerical-Recipes style offset arrays for the reference-named drop-ins.
+# ---------------------------------------------------------------------------
+class NRMatrix:
+    """float matrix m[nrl..nrh][ncl..nch] backed by a contiguous numpy array."""
+
+    def __init__(self, nrl, nrh, ncl, nch, data=None):
+        self.nrl, self.nrh, self.ncl, self.nch = nrl, nrh, ncl, nch
+        shape = (nrh - nrl + 1, nch - ncl + 1)
+        self.a = np.zeros(shape, np.float32) if data is None else np.ascontiguousarray(data, np.float32)
+        assert self.a.shape == shape
+        nrow = shape[0]
+        self._rows = (C.c_void_p * nrow)()
+        base = self.a.ctypes.data
+        for r in range(nrow):
+            self._rows[r] = base + (r * shape[1] - ncl) * 4
+        self.ptr = C.c_void_p(C.addressof(self._rows) - nrl * C.sizeof(C.c_void_p))
+
+    def __getitem__(self, ji):
+        j, i = ji
+        return self.a[j - self.nrl, i - self.ncl]
+
+
+class NRVector:
+    """float vector v[nl..nh]."""
+
+    def __init__(self, nl, nh, data=None):
+        self.nl = nl
+        self.a = np.zeros(nh - nl + 1, np.float32) if data is None else np.ascontiguousarray(data, np.float32)
+        self.ptr = C.c_void_p(self.a.ctypes.data - nl * 4)
+
+
+class NRTensor3:
+    """f3tensor t[nrl..nrh][ncl..nch][ndl..ndh] (last index fastest)."""
+
+    def __init__(self, nrl, nrh, ncl, nch, ndl, ndh, data=None):
+        shape = (nrh - nrl + 1, nch - ncl + 1, ndh - ndl + 1)
+        self.a = np.zeros(shape, np.float32) if data is None else np.ascontiguousarray(data, np.float32)
+        assert self.a.shape == shape
+        self._rows = (C.c_void_p * shape[0])()
+        self._cols = (C.c_void_p * (shape[0] * shape[1]))()
+        base = self.a.ctypes.data
+        colbase = C.addressof(self._cols)
+        for r in range(shape[0]):
+            for cidx in range(shape[1]):
+                self._cols[r * shape[1] + cidx] = base + ((r * shape[1] + cidx) * shape[2] - ndl) * 4
+            self._rows[r] = colbase + (r * shape[1] - ncl) * C.sizeof(C.c_void_p)
+        self.ptr = C.c_void_p(C.addressof(self._rows) - nrl * C.sizeof(C.c_void_p))
+
+
+class NRIMatrix:
+    """int matrix m[nrl..nrh][ncl..nch]."""
+
+    def __init__(self, nrl, nrh, ncl, nch, data):
+        shape = (nrh - nrl + 1, nch - ncl + 1)
+        self.a = np.ascontiguousarray(data, np.int32)
+        assert self.a.shape == shape
+        self._rows = (C.c_void_p * shape[0])()
+        for r in range(shape[0]):
+            self._rows[r] = self.a.ctypes.data + (r * shape[1] - ncl) * 4
+        self.ptr = C.c_void_p(C.addressof(self._rows) - nrl * C.sizeof(C.c_void_p))
+
+
+def set_globals(**kw):
+    g = Globals()
+    defaults = dict(BOUNDARY=0, GRAD_FORM=2, QUELLTYPB=2, L=0, FREE_SURF=1, NT=1)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(g, k, v)
+    rc = lib().dn_set_globals(C.byref(g))
+    if rc != 0:
+        raise DeniseError(lib().dn_last_error(None).decode())

@@ -1,0 +1,708 @@
+// Host orchestration of the fast path (handle API of include/denise_sh.h):
+// resident model and shot data, batched time loops, residual, energy
+// preconditioning and chain rule.  The reference's shot loop
+// (denise.c:875-1702) becomes: shots are grouped into batches that fit in
+// HBM; each batch runs its forward loop (one fused launch per step for all
+// shots), its residuals, its adjoint loop, then the per-shot preconditioning
+// and accumulation in shot order.
+#include "dn_core.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace dn;
+
+static thread_local std::string g_create_error;
+
+int dn_fail(dn_ctx *c, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c)
+        c->err = buf;
+    else
+        g_create_error = buf;
+    return -1;
+}
+
+#define HIPCHK(ctx, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
+
+static void dfree(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <typename T> static void dfree_t(T *&p)
+{
+    void *q = (void *)p;
+    dfree(q);
+    p = nullptr;
+}
+
+template <typename T> static hipError_t dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+extern "C" const char *dn_last_error(const dn_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" dn_ctx *dn_create(const dn_params *p)
+{
+    if (!p) {
+        dn_fail(nullptr, "dn_create: params is NULL");
+        return nullptr;
+    }
+    if (p->nx < 2 || p->ny < 2) {
+        dn_fail(nullptr, "dn_create: grid %dx%d too small", p->nx, p->ny);
+        return nullptr;
+    }
+    if (p->L < 0 || p->L > kMaxL) {
+        dn_fail(nullptr, "dn_create: L=%d outside 0..%d", p->L, kMaxL);
+        return nullptr;
+    }
+    if (p->fw < 0 || 2 * p->fw > std::min(p->nx, p->ny) + p->fw) {
+        dn_fail(nullptr, "dn_create: FW=%d larger than the grid", p->fw);
+        return nullptr;
+    }
+    if (p->dtinv < 1) {
+        dn_fail(nullptr, "dn_create: DTINV must be >= 1");
+        return nullptr;
+    }
+    if (p->snap_precision < 0 || p->snap_precision > 2) {
+        dn_fail(nullptr, "dn_create: snap_precision must be 0, 1 or 2");
+        return nullptr;
+    }
+    dn_ctx *c = new dn_ctx();
+    c->p = *p;
+    if (fd_coefficients(p->fdorder, p->maxrelerror, c->hc) != 0) {
+        dn_fail(nullptr, "Error in selection of FD coefficients: FDORDER=%d MAXRELERROR=%d", p->fdorder,
+                p->maxrelerror);
+        delete c;
+        return nullptr;
+    }
+    c->K = p->fdorder / 2;
+    Geo &g = c->g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.xoff = kPad;
+    g.yoff = kPad;
+    g.pitch = ((p->nx + 2 * kPad) + 31) / 32 * 32;
+    g.rows = p->ny + 2 * kPad;
+    g.plane = (long long)g.pitch * g.rows;
+    g.fw = p->fw;
+    g.free_surf = p->free_surf ? 1 : 0;
+    g.L = p->L;
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        dn_fail(nullptr, "dn_create: hipStreamCreate failed (no HIP device?)");
+        delete c;
+        return nullptr;
+    }
+    CpmlIn ci{p->nx, p->ny, p->fw, p->dh, p->dt, p->damping, p->fpml, p->npower, p->k_max_pml};
+    CpmlOut co;
+    cpml_profiles(ci, co);
+    const size_t nv = (size_t)co.n;
+    std::vector<float> packed(12 * nv);
+    for (int v = 0; v < 12; ++v) std::copy(co.v[v].begin(), co.v[v].end(), packed.begin() + v * nv);
+    if (dalloc(&c->d_pml, packed.size()) != hipSuccess ||
+        hipMemcpy(c->d_pml, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        dn_fail(nullptr, "dn_create: CPML upload failed");
+        dn_destroy(c);
+        return nullptr;
+    }
+    c->pml_host = packed;
+    PmlVec &pv = c->pml;
+    const float *b = c->d_pml;
+    pv.Kx = b + 0 * nv; pv.ax = b + 1 * nv; pv.bx = b + 2 * nv;
+    pv.Kxh = b + 3 * nv; pv.axh = b + 4 * nv; pv.bxh = b + 5 * nv;
+    pv.Ky = b + 6 * nv; pv.ay = b + 7 * nv; pv.by = b + 8 * nv;
+    pv.Kyh = b + 9 * nv; pv.ayh = b + 10 * nv; pv.byh = b + 11 * nv;
+
+    if (p->L > 0)
+        relaxation_constants(p->L, p->fl, p->dt, c->eta, &c->sumu, c->bip, c->bjm, c->cip, c->cjm);
+
+    // model buffers (padded)
+    const size_t P = (size_t)g.plane;
+    bool ok = dalloc(&c->d_vs, P) == hipSuccess && dalloc(&c->d_rho, P) == hipSuccess &&
+              dalloc(&c->d_taus, P) == hipSuccess && dalloc(&c->d_uip, P) == hipSuccess &&
+              dalloc(&c->d_ujp, P) == hipSuccess && dalloc(&c->d_b, P) == hipSuccess &&
+              dalloc(&c->d_tausipjp, P) == hipSuccess && dalloc(&c->d_fipjp, P) == hipSuccess &&
+              dalloc(&c->d_f, P) == hipSuccess && dalloc(&c->d_dip, P * std::max(1, p->L)) == hipSuccess &&
+              dalloc(&c->d_d, P * std::max(1, p->L)) == hipSuccess &&
+              dalloc(&c->d_gu, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_grho, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_gvs_out, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_grho_out, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_maxbits, 4096) == hipSuccess;
+    if (!ok) {
+        dn_fail(nullptr, "dn_create: out of device memory for the model");
+        dn_destroy(c);
+        return nullptr;
+    }
+    float *zs[] = {c->d_vs, c->d_rho, c->d_taus, c->d_uip, c->d_ujp, c->d_b, c->d_tausipjp, c->d_fipjp, c->d_f};
+    for (float *z : zs) (void)launch_fill(z, P, 0.0f, c->stream);
+    (void)launch_fill(c->d_dip, P * std::max(1, p->L), 0.0f, c->stream);
+    (void)launch_fill(c->d_d, P * std::max(1, p->L), 0.0f, c->stream);
+    (void)launch_fill(c->d_gu, (size_t)p->nx * p->ny, 0.0f, c->stream);
+    (void)launch_fill(c->d_grho, (size_t)p->nx * p->ny, 0.0f, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        dn_fail(nullptr, "dn_create: device initialisation failed");
+        dn_destroy(c);
+        return nullptr;
+    }
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    return c;
+}
+
+static void free_batch(dn_ctx *c)
+{
+    for (int k = 0; k < 2; ++k) {
+        dfree_t(c->d_vy[k]);
+        dfree_t(c->d_sxy[k]);
+        dfree_t(c->d_syz[k]);
+        dfree_t(c->d_psx[k]);
+        dfree_t(c->d_psy[k]);
+    }
+    dfree_t(c->d_r); dfree_t(c->d_q); dfree_t(c->d_pvx); dfree_t(c->d_pvy);
+    dfree_t(c->d_seis); dfree_t(c->d_resid); dfree_t(c->d_snap); dfree_t(c->d_snap16); dfree_t(c->d_snap_scale);
+    dfree_t(c->d_stage); dfree_t(c->d_stage_max);
+    dfree_t(c->d_Ws); dfree_t(c->d_Wr); dfree_t(c->d_gu_shot); dfree_t(c->d_grho_shot); dfree_t(c->d_part);
+    c->batch_cap = 0;
+    c->batch_nt_max = 0;
+}
+
+extern "C" void dn_destroy(dn_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_batch(c);
+    dfree_t(c->d_pml);
+    dfree_t(c->d_vs); dfree_t(c->d_rho); dfree_t(c->d_taus); dfree_t(c->d_uip); dfree_t(c->d_ujp);
+    dfree_t(c->d_b); dfree_t(c->d_tausipjp); dfree_t(c->d_fipjp); dfree_t(c->d_f); dfree_t(c->d_dip);
+    dfree_t(c->d_d); dfree_t(c->d_gu); dfree_t(c->d_grho); dfree_t(c->d_gvs_out); dfree_t(c->d_grho_out);
+    dfree_t(c->d_maxbits);
+    dfree_t(c->d_rec_row); dfree_t(c->d_rec_tab);
+    dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
+    dfree_t(c->d_obs);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// host [ny][nx] -> padded device field (interior only; halo zero)
+static int upload_padded(dn_ctx *c, float *dst, const float *src)
+{
+    const Geo &g = c->g;
+    HIPCHK(c, hipMemcpy2DAsync(dst + gidx(g, 1, 1), (size_t)g.pitch * sizeof(float), src, (size_t)g.nx * sizeof(float),
+                               (size_t)g.nx * sizeof(float), (size_t)g.ny, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+static int download_padded(dn_ctx *c, float *dst, const float *src)
+{
+    const Geo &g = c->g;
+    HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)g.nx * sizeof(float), src + gidx(g, 1, 1), (size_t)g.pitch * sizeof(float),
+                               (size_t)g.nx * sizeof(float), (size_t)g.ny, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+int dn_prepare_model(dn_ctx *c)
+{
+    ModelPrepArgs a{};
+    a.g = c->g;
+    a.DT = c->p.dt;
+    a.L = c->p.L;
+    a.vs = c->d_vs;
+    a.rho = c->d_rho;
+    a.taus = c->d_taus;
+    a.uip = c->d_uip;
+    a.ujp = c->d_ujp;
+    a.tausipjp = c->d_tausipjp;
+    a.bfield = c->d_b;
+    a.fipjp = c->d_fipjp;
+    a.fcoef = c->d_f;
+    a.dip = c->d_dip;
+    a.dcoef = c->d_d;
+    a.sumu = c->sumu;
+    for (int l = 0; l < kMaxL; ++l) a.eta[l] = (l < c->p.L) ? c->eta[l] : 0.0f;
+    HIPCHK(c, launch_model_prep(a, c->stream));
+    return 0;
+}
+
+extern "C" int dn_set_model(dn_ctx *c, const float *vs, const float *rho, const float *qs)
+{
+    if (!c || !vs || !rho) return dn_fail(c, "dn_set_model: NULL argument");
+    const int nx = c->p.nx, ny = c->p.ny, L = c->p.L;
+    if (L > 0 && !qs) return dn_fail(c, "dn_set_model: qs required for L=%d", L);
+    std::vector<float> taus;
+    if (L > 0) {
+        taus.resize((size_t)nx * ny);
+        for (size_t k = 0; k < taus.size(); ++k) taus[k] = (float)(2.0 / (double)qs[k]);  // readmod.c:99
+    }
+    if (upload_padded(c, c->d_vs, vs) || upload_padded(c, c->d_rho, rho)) return -1;
+    if (L > 0 && upload_padded(c, c->d_taus, taus.data())) return -1;
+    if (dn_prepare_model(c)) return -1;
+    if (!c->have_C) {
+        // denise.c:778-816: float sums, i outer, j inner
+        float vs_avg = 0.0f, rho_avg = 0.0f;
+        int count = 1;
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j) {
+                vs_avg += vs[(size_t)(j - 1) * nx + (i - 1)];
+                rho_avg += rho[(size_t)(j - 1) * nx + (i - 1)];
+                ++count;
+            }
+        --count;
+        vs_avg /= (float)count;
+        rho_avg /= (float)count;
+        c->C_vs = vs_avg;
+        c->C_rho = rho_avg;
+        c->have_C = true;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dn_get_model(dn_ctx *c, float *vs, float *rho)
+{
+    if (!c) return -1;
+    if (vs && download_padded(c, vs, c->d_vs)) return -1;
+    if (rho && download_padded(c, rho, c->d_rho)) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dn_set_receivers(dn_ctx *c, int ntr, const int *ri, const int *rj)
+{
+    if (!c || ntr < 0 || (ntr > 0 && (!ri || !rj))) return dn_fail(c, "dn_set_receivers: bad arguments");
+    const int nx = c->p.nx, ny = c->p.ny;
+    std::vector<int> row(ny + 1, -1);
+    std::vector<int> tab;
+    for (int r = 0; r < ntr; ++r) {
+        if (ri[r] < 1 || ri[r] > nx || rj[r] < 1 || rj[r] > ny)
+            return dn_fail(c, "receiver %d at (%d,%d) outside the grid", r + 1, ri[r], rj[r]);
+        if (row[rj[r]] < 0) {
+            row[rj[r]] = (int)tab.size();
+            tab.insert(tab.end(), (size_t)nx + 1, -1);
+        }
+        int &slot = tab[(size_t)row[rj[r]] + ri[r]];
+        if (slot >= 0) return dn_fail(c, "receivers %d and %d share a grid point", slot + 1, r + 1);
+        slot = r;
+    }
+    if (tab.empty()) tab.push_back(-1);
+    dfree_t(c->d_rec_row);
+    dfree_t(c->d_rec_tab);
+    HIPCHK(c, dalloc(&c->d_rec_row, row.size()));
+    HIPCHK(c, dalloc(&c->d_rec_tab, tab.size()));
+    HIPCHK(c, hipMemcpy(c->d_rec_row, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_rec_tab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->ntr = ntr;
+    c->rec_i.assign(ri, ri + ntr);
+    c->rec_j.assign(rj, rj + ntr);
+    free_batch(c);  // seismogram buffers depend on ntr
+    return 0;
+}
+
+extern "C" int dn_set_shots(dn_ctx *c, int nshots, int nt_max, const int *nt, const int *si, const int *sj,
+                            const float *signals)
+{
+    if (!c || nshots < 1 || nt_max < 1 || !nt || !si || !sj || !signals)
+        return dn_fail(c, "dn_set_shots: bad arguments");
+    for (int s = 0; s < nshots; ++s) {
+        if (nt[s] < 1 || nt[s] > nt_max) return dn_fail(c, "shot %d: nt=%d outside 1..%d", s + 1, nt[s], nt_max);
+        if (si[s] < 1 || si[s] > c->p.nx || sj[s] < 1 || sj[s] > c->p.ny)
+            return dn_fail(c, "shot %d: source (%d,%d) outside the grid", s + 1, si[s], sj[s]);
+    }
+    dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
+    dfree_t(c->d_obs);
+    std::vector<int> off(nshots + 1);
+    for (int s = 0; s <= nshots; ++s) off[s] = s;
+    HIPCHK(c, dalloc(&c->d_shot_nt, nshots));
+    HIPCHK(c, dalloc(&c->d_src_off, nshots + 1));
+    HIPCHK(c, dalloc(&c->d_src_i, nshots));
+    HIPCHK(c, dalloc(&c->d_src_j, nshots));
+    HIPCHK(c, dalloc(&c->d_sig, (size_t)nshots * nt_max));
+    HIPCHK(c, hipMemcpy(c->d_shot_nt, nt, nshots * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_src_off, off.data(), (nshots + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_src_i, si, nshots * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_src_j, sj, nshots * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_sig, signals, (size_t)nshots * nt_max * sizeof(float), hipMemcpyHostToDevice));
+    c->nshots = nshots;
+    c->nt_max = nt_max;
+    c->shot_nt.assign(nt, nt + nshots);
+    c->src_i.assign(si, si + nshots);
+    c->src_j.assign(sj, sj + nshots);
+    free_batch(c);
+    return 0;
+}
+
+extern "C" int dn_set_observed(dn_ctx *c, const float *obs)
+{
+    if (!c || !obs) return dn_fail(c, "dn_set_observed: NULL argument");
+    if (c->nshots < 1 || c->ntr < 1) return dn_fail(c, "dn_set_observed: set shots and receivers first");
+    const size_t n = (size_t)c->nshots * c->ntr * c->nt_max;
+    if (!c->d_obs) HIPCHK(c, dalloc(&c->d_obs, n));
+    HIPCHK(c, hipMemcpy(c->d_obs, obs, n * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int dn_set_batch(dn_ctx *c, int max_shots, int rows_per_segment)
+{
+    if (!c) return -1;
+    c->batch_req = std::max(0, max_shots);
+    c->rows_req = std::max(0, rows_per_segment);
+    free_batch(c);
+    return 0;
+}
+
+int dn_nsnap(const dn_ctx *c) { return (c->nt_max + c->p.dtinv - 1) / c->p.dtinv; }
+
+// bytes of per-shot device state for a gradient batch
+static size_t per_shot_bytes(const dn_ctx *c, bool gradient)
+{
+    const Geo &g = c->g;
+    const size_t P = (size_t)g.plane, nxny = (size_t)g.nx * g.ny;
+    size_t b = 6 * P * 4 + 2 * (size_t)g.L * P * 4;                 // vy,sxy,syz x2, r, q
+    b += (size_t)(2 * 2 + 2) * (size_t)(2 * g.fw) * (g.nx + g.ny) * 4;  // psi
+    b += 2 * (size_t)c->ntr * c->nt_max * 4;                        // seis + resid
+    b += 4 * nxny * 4 + (size_t)c->ntr * 16;                        // Ws, Wr, gu, grho, partials
+    if (gradient) {
+        const size_t ns = (size_t)dn_nsnap(c);
+        b += (c->p.snap_precision == 0 ? 4 : 2) * 3 * nxny * ns;
+        if (c->p.snap_precision != 0) b += 3 * nxny * 4 + ns * 12;
+    }
+    return b;
+}
+
+int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
+{
+    if (c->nshots < 1) return dn_fail(c, "no shots set");
+    if (c->ntr < 1) return dn_fail(c, "no receivers set");
+    int cap = want;
+    if (c->batch_req > 0) cap = std::min(cap, c->batch_req);
+    const size_t per = per_shot_bytes(c, gradient || c->batch_has_snap);
+    if (c->batch_cap >= cap && c->batch_nt_max == c->nt_max && (!gradient || c->batch_has_snap)) return 0;
+    free_batch(c);
+    size_t freeb = 0, total = 0;
+    HIPCHK(c, hipMemGetInfo(&freeb, &total));
+    const size_t budget = (size_t)((double)freeb * 0.85);
+    const int fit = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
+    cap = std::max(1, std::min(std::min(cap, fit), 4096));
+    if (per > budget) return dn_fail(c, "one shot needs %zu MB but only %zu MB of HBM are free", per >> 20, freeb >> 20);
+    const Geo &g = c->g;
+    const size_t P = (size_t)g.plane, nxny = (size_t)g.nx * g.ny;
+    const size_t S = (size_t)cap;
+    const size_t fw2 = (size_t)(2 * g.fw);
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c, dalloc(&c->d_vy[k], S * P));
+        HIPCHK(c, dalloc(&c->d_sxy[k], S * P));
+        HIPCHK(c, dalloc(&c->d_syz[k], S * P));
+        HIPCHK(c, dalloc(&c->d_psx[k], S * fw2 * g.ny));
+        HIPCHK(c, dalloc(&c->d_psy[k], S * fw2 * g.nx));
+    }
+    HIPCHK(c, dalloc(&c->d_r, S * P * std::max(1, g.L)));
+    HIPCHK(c, dalloc(&c->d_q, S * P * std::max(1, g.L)));
+    HIPCHK(c, dalloc(&c->d_pvx, S * fw2 * g.ny));
+    HIPCHK(c, dalloc(&c->d_pvy, S * fw2 * g.nx));
+    HIPCHK(c, dalloc(&c->d_seis, S * c->ntr * c->nt_max));
+    HIPCHK(c, dalloc(&c->d_resid, S * c->ntr * c->nt_max));
+    HIPCHK(c, dalloc(&c->d_Ws, S * nxny));
+    HIPCHK(c, dalloc(&c->d_Wr, S * nxny));
+    HIPCHK(c, dalloc(&c->d_gu_shot, S * nxny));
+    HIPCHK(c, dalloc(&c->d_grho_shot, S * nxny));
+    HIPCHK(c, dalloc(&c->d_part, S * c->ntr * 2));
+    c->batch_has_snap = gradient;
+    if (gradient) {
+        const size_t ns = (size_t)dn_nsnap(c);
+        if (c->p.snap_precision == 0) {
+            HIPCHK(c, dalloc(&c->d_snap, S * ns * 3 * nxny));
+        } else {
+            HIPCHK(c, dalloc(&c->d_snap16, S * ns * 3 * nxny));
+            HIPCHK(c, dalloc(&c->d_snap_scale, S * ns * 3));
+            HIPCHK(c, dalloc(&c->d_stage, S * 3 * nxny));
+            HIPCHK(c, dalloc(&c->d_stage_max, S * 3));
+        }
+    }
+    c->batch_cap = cap;
+    c->batch_nt_max = c->nt_max;
+    return 0;
+}
+
+// rows per segment: enough workgroups to fill 256 CUs several times over
+static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
+{
+    if (c->rows_req > 0) return std::min(c->rows_req, c->g.ny);
+    const int target = 4096;
+    const int cand[] = {1 << 20, 512, 256, 128, 96, 64, 48, 32, 24, 16};
+    int best = 16;
+    for (int R : cand) {
+        const int Rr = std::min(R, c->g.ny);
+        const long long blocks = (long long)nshots * nstrips * ((c->g.ny + Rr - 1) / Rr);
+        best = Rr;
+        if (blocks >= target) break;
+    }
+    return std::max(std::min(best, c->g.ny), std::min(4 * c->K, c->g.ny));
+}
+
+void dn_fill_step_common(dn_ctx *c, StepArgs &a, int shot0, int n)
+{
+    const Geo &g = c->g;
+    a = StepArgs{};
+    a.g = g;
+    a.nshots = n;
+    a.ow = kWave + 1 - 2 * c->K;
+    a.nstrips = (g.nx + a.ow - 1) / a.ow;
+    a.R = choose_rows(c, n, a.nstrips);
+    a.nsegs = (g.ny + a.R - 1) / a.R;
+    a.nblocks = n * a.nstrips * a.nsegs;
+    a.nblocks8 = (a.nblocks + 7) / 8;
+    a.DT = c->p.dt;
+    a.DH = c->p.dh;
+    a.dhi = (c->p.L > 0) ? (float)(1.0 / (double)c->p.dh) : c->p.dt / c->p.dh;
+    a.dthalbe = (float)((double)c->p.dt / 2.0);
+    for (int i = 0; i < 7; ++i) a.hc[i] = c->hc[i];
+    a.hcy6 = c->hc[5];  // update_v_PML.c:469 (Q6)
+    a.pml = c->pml;
+    a.bfield = c->d_b;
+    a.uip = c->d_uip;
+    a.ujp = c->d_ujp;
+    a.fipjp = c->d_fipjp;
+    a.fcoef = c->d_f;
+    a.dip = c->d_dip;
+    a.dcoef = c->d_d;
+    for (int l = 0; l < kMaxL; ++l) {
+        a.bip[l] = c->bip[l];
+        a.bjm[l] = c->bjm[l];
+        a.cip[l] = c->cip[l];
+        a.cjm[l] = c->cjm[l];
+    }
+    a.r = c->d_r;
+    a.q = c->d_q;
+    a.pvx = c->d_pvx;
+    a.pvy = c->d_pvy;
+    a.shot_nt = c->d_shot_nt + shot0;
+    a.src_off = c->d_src_off + shot0;  // offsets are absolute indices into src arrays
+    a.src_i = c->d_src_i;
+    a.src_j = c->d_src_j;
+    a.src_sig = c->d_sig;
+    a.nt_max = c->nt_max;
+    a.rec_row = c->d_rec_row;
+    a.rec_tab = c->d_rec_tab;
+    a.ntr = c->ntr;
+    a.dtinv = c->p.dtinv;
+}
+
+int dn_zero_state(dn_ctx *c, int n)
+{
+    const Geo &g = c->g;
+    const size_t P = (size_t)g.plane, S = (size_t)n, fw2 = (size_t)(2 * g.fw);
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c, launch_fill(c->d_vy[k], S * P, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_sxy[k], S * P, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_syz[k], S * P, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_psx[k], S * fw2 * g.ny, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_psy[k], S * fw2 * g.nx, 0.0f, c->stream));
+    }
+    if (g.L > 0) {
+        HIPCHK(c, launch_fill(c->d_r, S * P * g.L, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_q, S * P * g.L, 0.0f, c->stream));
+    }
+    HIPCHK(c, launch_fill(c->d_pvx, S * fw2 * g.ny, 0.0f, c->stream));
+    HIPCHK(c, launch_fill(c->d_pvy, S * fw2 * g.nx, 0.0f, c->stream));
+    return 0;
+}
+
+// One pass (forward or adjoint) over shots [shot0, shot0+n) resident in batch
+// slots 0..n-1.  store: forward snapshots + Ws; seis: forward seismograms.
+int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool record)
+{
+    int ntmax = 0;
+    for (int s = 0; s < n; ++s) ntmax = std::max(ntmax, c->shot_nt[shot0 + s]);
+    if (dn_zero_state(c, n)) return -1;
+    StepArgs a;
+    dn_fill_step_common(c, a, shot0, n);
+    const size_t nxny = (size_t)c->g.nx * c->g.ny;
+    const int nsnap = dn_nsnap(c);
+    const int kind = c->p.snap_precision;
+    if (!adjoint) {
+        a.seis = record ? c->d_seis : nullptr;
+        if (record) HIPCHK(c, launch_fill(c->d_seis, (size_t)n * c->ntr * c->nt_max, 0.0f, c->stream));
+        a.store_mode = store ? (kind == 0 ? 1 : 2) : 0;
+        a.wacc = c->d_Ws;
+        if (store) HIPCHK(c, launch_fill(c->d_Ws, (size_t)n * nxny, 0.0f, c->stream));
+    } else {
+        a.resid = c->d_resid;
+        a.wacc = c->d_Wr;
+        a.grad_u = c->d_gu_shot;
+        a.grad_rho = c->d_grho_shot;
+        HIPCHK(c, launch_fill(c->d_Wr, (size_t)n * nxny, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_gu_shot, (size_t)n * nxny, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_grho_shot, (size_t)n * nxny, 0.0f, c->stream));
+    }
+    a.snap = c->d_snap;
+    a.snap16 = c->d_snap16;
+    a.snap_scale = c->d_snap_scale;
+    a.snap_kind = kind;
+    a.snap_shot_stride = (long long)nsnap * 3 * (long long)nxny;
+    a.stage = c->d_stage;
+    a.stage_max = c->d_stage_max;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int cur = 0;
+    for (int nt = 1; nt <= ntmax; ++nt) {
+        a.nt = nt;
+        a.vy_in = c->d_vy[cur];
+        a.sxy_in = c->d_sxy[cur];
+        a.syz_in = c->d_syz[cur];
+        a.vy_out = c->d_vy[cur ^ 1];
+        a.sxy_out = c->d_sxy[cur ^ 1];
+        a.syz_out = c->d_syz[cur ^ 1];
+        a.psx_in = c->d_psx[cur];
+        a.psx_out = c->d_psx[cur ^ 1];
+        a.psy_in = c->d_psy[cur];
+        a.psy_out = c->d_psy[cur ^ 1];
+        const bool pack = !adjoint && store && kind != 0 && ((nt - 1) % c->p.dtinv) == 0;
+        if (pack) HIPCHK(c, launch_fill_u32(c->d_stage_max, (size_t)n * 3, 0u, c->stream));
+        HIPCHK(c, launch_step(a, c->K, c->p.L > 0, adjoint, c->stream));
+        if (pack)
+            HIPCHK(c, launch_snap_pack(n, c->g.nx, c->g.ny, c->d_stage, c->d_stage_max, c->d_snap16, c->d_snap_scale,
+                                       a.snap_shot_stride, nsnap, (nt - 1) / c->p.dtinv, kind, c->stream));
+        cur ^= 1;
+        c->launches++;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->ms_steps += ms;
+    for (int s = 0; s < n; ++s) c->cell_updates += (long long)c->g.nx * c->g.ny * c->shot_nt[shot0 + s];
+    c->cur_final = cur;
+    // NaN probe of the centre cell (denise.c:1183-1185)
+    std::vector<float> centre(n);
+    const long long off = gidx(c->g, c->g.ny / 2, c->g.nx / 2);
+    for (int s = 0; s < n; ++s)
+        HIPCHK(c, hipMemcpy(&centre[s], c->d_vy[cur] + (size_t)s * c->g.plane + off, sizeof(float),
+                            hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; ++s)
+        if (std::isnan(centre[s])) return dn_fail(c, "Simulation is unstable ! (shot %d)", shot0 + s + 1);
+    return 0;
+}
+
+extern "C" int dn_forward(dn_ctx *c, int shot0, int nshots, float *seis)
+{
+    if (!c) return -1;
+    if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_forward: bad shot range");
+    c->ms_steps = 0.0;
+    c->launches = 0;
+    c->cell_updates = 0;
+    for (int b0 = shot0; b0 < shot0 + nshots;) {
+        if (dn_ensure_batch(c, shot0 + nshots - b0, false)) return -1;
+        const int n = std::min(c->batch_cap, shot0 + nshots - b0);
+        if (dn_run_pass(c, b0, n, false, false, true)) return -1;
+        if (seis)
+            HIPCHK(c, hipMemcpy(seis + (size_t)(b0 - shot0) * c->ntr * c->nt_max, c->d_seis,
+                                (size_t)n * c->ntr * c->nt_max * sizeof(float), hipMemcpyDeviceToHost));
+        b0 += n;
+    }
+    return 0;
+}
+
+extern "C" int dn_gradient(dn_ctx *c, int shot0, int nshots, int accumulate, double *l2, double *energy)
+{
+    if (!c) return -1;
+    if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_gradient: bad shot range");
+    if (!c->d_obs) return dn_fail(c, "dn_gradient: observed data not set");
+    if (!c->have_C) return dn_fail(c, "dn_gradient: model not set");
+    const size_t nxny = (size_t)c->g.nx * c->g.ny;
+    if (!accumulate) {
+        HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_grho, nxny, 0.0f, c->stream));
+    }
+    c->ms_steps = 0.0;
+    c->launches = 0;
+    c->cell_updates = 0;
+    double L2 = 0.0, EN = 0.0;
+    std::vector<double> part;
+    for (int b0 = shot0; b0 < shot0 + nshots;) {
+        if (dn_ensure_batch(c, shot0 + nshots - b0, true)) return -1;
+        const int n = std::min(c->batch_cap, shot0 + nshots - b0);
+        if (dn_run_pass(c, b0, n, false, true, true)) return -1;
+        HIPCHK(c, launch_residual(n, c->ntr, c->nt_max, c->d_shot_nt + b0, c->d_seis,
+                                  c->d_obs + (size_t)b0 * c->ntr * c->nt_max, c->d_resid, c->d_part, c->p.dt,
+                                  c->stream));
+        part.resize((size_t)n * c->ntr * 2);
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t t = 0; t < (size_t)n * c->ntr; ++t) {
+            L2 += part[2 * t];
+            EN += part[2 * t + 1];
+        }
+        if (dn_run_pass(c, b0, n, true, false, false)) return -1;
+        HIPCHK(c, launch_eprecond_accumulate(n, c->g.nx, c->g.ny, c->d_Ws, c->d_Wr, c->d_gu_shot, c->d_grho_shot,
+                                             c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits, c->stream));
+        b0 += n;
+    }
+    HIPCHK(c, launch_chain_rule(c->g, c->g.nx, c->g.ny, c->d_gu, c->d_grho, c->d_vs, c->d_rho, c->p.dt,
+                                c->d_gvs_out, c->d_grho_out, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (l2) *l2 = L2;
+    if (energy) *energy = EN;
+    return 0;
+}
+
+extern "C" int dn_get_gradient(dn_ctx *c, float *gvs, float *grho, float *raw_u, float *raw_rho)
+{
+    if (!c) return -1;
+    const size_t n = (size_t)c->g.nx * c->g.ny * sizeof(float);
+    if (gvs) HIPCHK(c, hipMemcpy(gvs, c->d_gvs_out, n, hipMemcpyDeviceToHost));
+    if (grho) HIPCHK(c, hipMemcpy(grho, c->d_grho_out, n, hipMemcpyDeviceToHost));
+    if (raw_u) HIPCHK(c, hipMemcpy(raw_u, c->d_gu, n, hipMemcpyDeviceToHost));
+    if (raw_rho) HIPCHK(c, hipMemcpy(raw_rho, c->d_grho, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int dn_forward_fields(dn_ctx *c, int shot, int nsteps, float *vy, float *sxy, float *syz)
+{
+    if (!c || shot < 0 || shot >= c->nshots || nsteps < 1) return dn_fail(c, "dn_forward_fields: bad arguments");
+    if (dn_ensure_batch(c, 1, false)) return -1;
+    const int saved = c->shot_nt[shot];
+    c->shot_nt[shot] = std::min(nsteps, saved);
+    HIPCHK(c, hipMemcpy(c->d_shot_nt + shot, &c->shot_nt[shot], sizeof(int), hipMemcpyHostToDevice));
+    const int rc = dn_run_pass(c, shot, 1, false, false, false);
+    c->shot_nt[shot] = saved;
+    HIPCHK(c, hipMemcpy(c->d_shot_nt + shot, &saved, sizeof(int), hipMemcpyHostToDevice));
+    if (rc) return rc;
+    const int cur = c->cur_final;
+    if (vy && download_padded(c, vy, c->d_vy[cur])) return -1;
+    if (sxy && download_padded(c, sxy, c->d_sxy[cur])) return -1;
+    if (syz && download_padded(c, syz, c->d_syz[cur])) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int dn_last_timing(dn_ctx *c, double *ms, long long *launches, long long *cells)
+{
+    if (!c) return -1;
+    if (ms) *ms = c->ms_steps;
+    if (launches) *launches = c->launches;
+    if (cells) *cells = c->cell_updates;
+    return 0;
+}

@@ -1,0 +1,64 @@
+// Context of the handle API (include/denise_sh.h) and helpers shared by the
+// host-side translation units of libdenise_sh.so.
+#pragma once
+#include "../../include/denise_sh.h"
+#include "dn_internal.h"
+#include "dn_setup.h"
+
+#include <string>
+#include <vector>
+
+struct dn_ctx {
+    dn_params p{};
+    int K = 1;
+    dn::Geo g{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    float hc[7] = {0};
+    std::vector<float> pml_host;   // 12 x (2FW+1), same packing as d_pml
+    float *d_pml = nullptr;
+    dn::PmlVec pml{};
+    // relaxation constants
+    float eta[dn::kMaxL] = {0}, sumu = 0.0f;
+    float bip[dn::kMaxL] = {0}, bjm[dn::kMaxL] = {0}, cip[dn::kMaxL] = {0}, cjm[dn::kMaxL] = {0};
+    // model (padded)
+    float *d_vs = nullptr, *d_rho = nullptr, *d_taus = nullptr, *d_uip = nullptr, *d_ujp = nullptr, *d_b = nullptr;
+    float *d_tausipjp = nullptr, *d_fipjp = nullptr, *d_f = nullptr, *d_dip = nullptr, *d_d = nullptr;
+    float C_vs = 0.0f, C_rho = 0.0f;
+    bool have_C = false;
+    // gradient sums [ny][nx] and outputs after the chain rule
+    float *d_gu = nullptr, *d_grho = nullptr, *d_gvs_out = nullptr, *d_grho_out = nullptr;
+    unsigned int *d_maxbits = nullptr;
+    // receivers
+    int ntr = 0;
+    std::vector<int> rec_i, rec_j;
+    int *d_rec_row = nullptr, *d_rec_tab = nullptr;
+    // shots
+    int nshots = 0, nt_max = 0;
+    std::vector<int> shot_nt, src_i, src_j;
+    int *d_shot_nt = nullptr, *d_src_off = nullptr, *d_src_i = nullptr, *d_src_j = nullptr;
+    float *d_sig = nullptr, *d_obs = nullptr;
+    // batch workspace
+    int batch_req = 0, rows_req = 0, batch_cap = 0, batch_nt_max = 0;
+    bool batch_has_snap = false;
+    float *d_vy[2] = {nullptr, nullptr}, *d_sxy[2] = {nullptr, nullptr}, *d_syz[2] = {nullptr, nullptr};
+    float *d_psx[2] = {nullptr, nullptr}, *d_psy[2] = {nullptr, nullptr};
+    float *d_r = nullptr, *d_q = nullptr, *d_pvx = nullptr, *d_pvy = nullptr;
+    float *d_seis = nullptr, *d_resid = nullptr, *d_snap = nullptr;
+    uint16_t *d_snap16 = nullptr;
+    float *d_snap_scale = nullptr, *d_stage = nullptr;
+    unsigned int *d_stage_max = nullptr;
+    float *d_Ws = nullptr, *d_Wr = nullptr, *d_gu_shot = nullptr, *d_grho_shot = nullptr;
+    double *d_part = nullptr;
+    int cur_final = 0;
+    // timing of the last pass set
+    double ms_steps = 0.0;
+    long long launches = 0, cell_updates = 0;
+};
+
+int dn_fail(dn_ctx *c, const char *fmt, ...);
+int dn_prepare_model(dn_ctx *c);
+int dn_ensure_batch(dn_ctx *c, int want, bool gradient);
+int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool record);
+int dn_nsnap(const dn_ctx *c);

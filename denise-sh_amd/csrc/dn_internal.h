@@ -1,0 +1,146 @@
+// Internal declarations shared by the HIP kernels and the host side of
+// libdenise_sh.so.  Not part of the public C-ABI (include/denise_sh.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace dn {
+
+constexpr int kWave = 64;      // CDNA wavefront width: one wave per strip
+constexpr int kMaxL = 4;       // relaxation mechanisms supported
+constexpr int kPad = 8;        // halo columns/rows kept around the interior (>= FDORDER/2)
+
+// Padded, x-fastest layout of one 2-D field.  Element (j,i) of the
+// reference's index space (1-based interior) lives at
+// (j-1+yoff)*pitch + (i-1+xoff).  The halo is a permanent zero frame for
+// wavefields (zero_fdveps.c) and holds the periodic copy for model fields
+// (matcopy.c).
+struct Geo {
+    int nx, ny;
+    int pitch;          // floats per padded row (multiple of 32: 128-byte rows)
+    int xoff, yoff;     // position of i = 1 / j = 1
+    int rows;           // padded rows
+    long long plane;    // pitch * rows
+    int fw, free_surf, L;
+};
+
+__host__ __device__ inline long long gidx(const Geo &g, int j, int i)
+{
+    return (long long)(j - 1 + g.yoff) * g.pitch + (long long)(i - 1 + g.xoff);
+}
+
+// CPML profile vectors, 1-based [1..2FW] as in PML_pro.c, packed on device.
+struct PmlVec {
+    const float *Kx, *ax, *bx, *Kxh, *axh, *bxh, *Ky, *ay, *by, *Kyh, *ayh, *byh;
+};
+
+// Arguments of the fused time-step kernel (one launch advances every active
+// shot of a batch by one time step).
+struct StepArgs {
+    Geo g;
+    int nshots;                    // shots in this launch
+    int ow, nstrips, R, nsegs;     // strip width (owned columns), strips, rows per segment, segments
+    int nblocks, nblocks8;         // real blocks and blocks per XCD slice
+    float DT, DH, dhi, dthalbe;
+    float hc[7];
+    float hcy6;                    // coefficient used for the 6th y-term of update_v (Q6: hc[5])
+    PmlVec pml;
+    // static model fields (shared by all shots), padded layout
+    const float *__restrict__ bfield;   // 1/rho (0 where rho < 1e-20)
+    const float *__restrict__ uip;      // elastic
+    const float *__restrict__ ujp;
+    const float *__restrict__ fipjp;    // visco
+    const float *__restrict__ fcoef;
+    const float *__restrict__ dip;      // [L][plane]
+    const float *__restrict__ dcoef;    // [L][plane]
+    float bip[kMaxL], bjm[kMaxL], cip[kMaxL], cjm[kMaxL];
+    // per-shot dynamic state; shot stride = plane (L*plane for r, q)
+    const float *vy_in, *sxy_in, *syz_in;
+    float *vy_out, *sxy_out, *syz_out;
+    float *r, *q;
+    const float *psx_in;  float *psx_out;   // psi_sxy_x [s][ny][2fw] (ping-pong)
+    const float *psy_in;  float *psy_out;   // psi_syy_y [s][2fw][nx] (ping-pong)
+    float *pvx, *pvy;                       // psi_vyx [s][ny][2fw], psi_vyy [s][2fw][nx]
+    // shot tables
+    const int *shot_nt;                     // [nshots] time steps of each shot
+    const int *src_off, *src_i, *src_j;     // sources of shot s: src_off[s]..src_off[s+1]-1
+    const float *src_sig;                   // [nsrc_total][nt_max]
+    int nt_max;
+    // receivers (shared by all shots)
+    const int *rec_row;                     // [ny+1] -> offset into rec_tab, or -1
+    const int *rec_tab;                     // rows of [nx+1]: receiver index or -1
+    int ntr;
+    float *seis;                            // forward: [s][ntr][nt_max]
+    const float *resid;                     // adjoint: [s][ntr][nt_max] time-reversed residual
+    // gradient storage
+    int dtinv;
+    int store_mode;                         // forward: 0 no snapshots, 1 fp32 direct, 2 fp32 staging
+    float *snap;                            // fp32 [s][nsnap][3][ny][nx] (store_mode 1 / adjoint fp32)
+    long long snap_shot_stride;             // elements per shot
+    const uint16_t *snap16;                 // adjoint read of bf16/fp16 snapshots [s][nsnap][3][ny][nx]
+    const float *snap_scale;                // [s][nsnap][3] decode scale
+    int snap_kind;                          // 0 fp32, 1 bf16, 2 fp16
+    float *stage;                           // store_mode 2: [s][3][ny][nx]
+    unsigned int *stage_max;                // store_mode 2: [s][3] max |x| as float bits
+    float *wacc;                            // Ws (forward) or Wr (adjoint) [s][ny][nx]
+    float *grad_u, *grad_rho;               // adjoint [s][ny][nx]
+    int nt;                                 // current step, 1-based
+};
+
+// launchers (dn_kernels.hip)
+hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipStream_t st);
+
+struct ModelPrepArgs {
+    Geo g;
+    float DT;
+    int L;
+    float *vs, *rho, *taus;                 // padded, halo filled by prep
+    float *uip, *ujp, *tausipjp, *bfield;
+    float *fipjp, *fcoef, *dip, *dcoef;
+    float sumu;                             // prepare_update_s.c:30-33 (host)
+    float eta[kMaxL];                       // readmod.c:33 (host)
+};
+hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st);
+
+// per-shot residual (calc_res / calc_energy): one thread per trace
+hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn,
+                           const float *obs, float *resid, double *l2_energy /*[2]*/, float DT, hipStream_t st);
+
+// energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)
+hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
+                                      const float *gu_shot, const float *grho_shot, float *gu, float *grho,
+                                      float C_vs, float C_rho, unsigned int *maxbits, hipStream_t st);
+
+// chain rule (denise.c:1756-1811)
+hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, const float *grho,
+                             const float *vs_pad, const float *rho_pad, float DT, float *grad_vs,
+                             float *grad_rho, hipStream_t st);
+
+// bf16/fp16 snapshot packing (store_mode 2)
+hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, const unsigned int *stage_max,
+                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap, int slot,
+                            int kind, hipStream_t st);
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t st);
+hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st);
+
+// split (one field update per launch) kernels used by the reference-named drop-ins
+struct SplitArgs {
+    Geo g;
+    float DT, DH;
+    float hc[7];
+    float hcy6;
+    int K;
+    PmlVec pml;
+    int sw;                                  // update_v: 0 forward, 1 adjoint
+    float *vy, *vyp1, *sxy, *syz, *uy, *uyx;
+    const float *rho, *uip, *ujp, *fipjp, *fcoef, *dip, *dcoef;
+    float *r, *q;
+    float bip[kMaxL], bjm[kMaxL], cip[kMaxL], cjm[kMaxL];
+    float *psi_a, *psi_b;                   // v: psi_syy_y [2fw][nx], psi_sxy_x [ny][2fw]; s: psi_vyy, psi_vyx
+};
+hipError_t launch_split_v(const SplitArgs &a, hipStream_t st);
+hipError_t launch_split_s(const SplitArgs &a, bool visco, hipStream_t st);
+
+}  // namespace dn

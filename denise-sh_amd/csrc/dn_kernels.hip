@@ -1,0 +1,805 @@
+// Hand-written CDNA4 (gfx950) kernels of the SH time-stepping and adjoint
+// gradient path.  Every floating-point expression keeps the reference's
+// operand order and rounding (FP32 operators, double where the reference
+// promotes through a double literal); the file is compiled with
+// -ffp-contract=off so no multiply-add is fused, which makes the results
+// reproduce the reference's CPU FP32 arithmetic.
+#include "dn_internal.h"
+#include <hip/hip_fp16.h>
+
+#pragma clang fp contract(off)
+
+namespace dn {
+
+// Load of a padded field with the zero frame made explicit: cells further
+// than K from the interior are never stored (they are outside the padded
+// array for strips/segments at the edges) and read as 0.
+template <int K>
+__device__ __forceinline__ float ldz(const float *__restrict__ p, const Geo &g, int j, int i)
+{
+    if (j < 1 - K || j > g.ny + K || i < 1 - K || i > g.nx + K) return 0.0f;
+    return p[gidx(g, j, i)];
+}
+
+hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st);
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint16_t f2bf16(float x)
+{
+    // round-to-nearest-even truncation of the fp32 pattern (NaN not expected)
+    unsigned int u = __float_as_uint(x);
+    unsigned int lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf162f(uint16_t h) { return __uint_as_float(((unsigned int)h) << 16); }
+__device__ __forceinline__ uint16_t f2h(float x) { return __half_as_ushort(__float2half_rn(x)); }
+__device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+
+// ---------------------------------------------------------------------------
+// Fused time step.
+//
+// One 64-lane wavefront (= one workgroup) owns `ow = 65 - 2K` consecutive
+// columns of one shot and walks down a segment of R rows.  Its lanes cover
+// ow + 2K - 1 = 64 columns: the velocity update (update_v_PML.c) is computed
+// for all 64 (the 2K-1 outer ones redundantly, as their owners in the
+// neighbouring strips also do) so that the stress update
+// (update_s_{elastic,visc}_PML.c) of the owned columns finds every x
+// neighbour of the new vy inside the wave.  In y the wave keeps register
+// windows: 2K rows of old syz (velocity y-derivative) and 2K rows of new vy
+// (stress y-derivative); the velocity runs K rows ahead of the stress.  The
+// x-neighbours come through LDS.  Old and new vy, sxy, syz and the velocity
+// CPML memory variables are separate buffers (ping-pong), so the redundant
+// reads of neighbouring strips never race with their owners' writes; the
+// stress CPML and the relaxation memory variables are touched by their owner
+// only and are updated in place.
+//
+// Also fused: source / residual injection (update_v_PML.c:563-583), receiver
+// sampling (seismo_ssg.c), snapshot store and Ws (denise.c:1259-1327),
+// correlation and Wr (denise.c:1580-1625).
+//
+// Block ids are remapped so that each of the 8 XCDs receives a contiguous
+// range of (tile, shot) pairs with the shot index fastest: the shots of one
+// tile share the static model fields through that XCD's L2.
+// ---------------------------------------------------------------------------
+template <int K, bool VISCO, bool ADJ>
+__global__ __launch_bounds__(64) void k_step(const StepArgs a)
+{
+    const int w = blockIdx.x;
+    const int lin = (w & 7) * a.nblocks8 + (w >> 3);
+    if (lin >= a.nblocks) return;
+    const int s = lin % a.nshots;
+    const int tile = lin / a.nshots;
+    const int strip = tile % a.nstrips;
+    const int seg = tile / a.nstrips;
+    const Geo &g = a.g;
+    const int NTs = a.shot_nt[s];
+    const int nt = a.nt;
+    if (nt > NTs) return;  // shot finished (uneven record lengths)
+
+    const int lane = threadIdx.x;
+    const int c0 = 1 + strip * a.ow;          // first owned column
+    const int c = c0 - (K - 1) + lane;        // column of this lane
+    const bool in_x = (c >= 1) && (c <= g.nx);
+    const bool own_x = (lane >= K - 1) && (lane < K - 1 + a.ow) && in_x;
+    const int j0 = 1 + seg * a.R;
+    const int j1 = min(j0 + a.R - 1, g.ny);
+    const long long so = (long long)s * g.plane;
+    const long long nxny = (long long)g.nx * g.ny;
+    const int fw = g.fw, fw2 = 2 * g.fw;
+
+    bool fstore = false, corr = false;
+    int slot = 0;
+    if (!ADJ) {
+        if (a.store_mode != 0 && ((nt - 1) % a.dtinv) == 0) {
+            fstore = true;
+            slot = (nt - 1) / a.dtinv;
+        }
+    } else {
+        const int sf = NTs - nt + 1;  // forward step paired with this adjoint step
+        if (((sf - 1) % a.dtinv) == 0) {
+            corr = true;
+            slot = (sf - 1) / a.dtinv;
+        }
+    }
+    const int sb = ADJ ? 0 : a.src_off[s];
+    const int se = ADJ ? 0 : a.src_off[s + 1];
+
+    const float *__restrict__ vy_in = a.vy_in + so;
+    const float *__restrict__ sxy_in = a.sxy_in + so;
+    const float *__restrict__ syz_in = a.syz_in + so;
+    float *__restrict__ vy_out = a.vy_out + so;
+    float *__restrict__ sxy_out = a.sxy_out + so;
+    float *__restrict__ syz_out = a.syz_out + so;
+
+    __shared__ float lsx[kWave + 2 * K];
+    __shared__ float lvy[kWave];
+
+    float zw[2 * K];      // old syz, slot t <-> row jv-K+t (after the head load)
+    float vw[2 * K];      // new vy, slot t <-> row jv-2K+1+t (after the push)
+    float sq[K + 1];      // old sxy at the lane's column, slot t <-> row jv-K+t
+    const int jstart = j0 - K + 1;
+    const int jend = j1 + K;
+#pragma unroll
+    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = ldz<K>(syz_in, g, jstart - K + t, c);
+#pragma unroll
+    for (int t = 0; t < 2 * K; ++t) vw[t] = 0.0f;
+#pragma unroll
+    for (int t = 0; t <= K; ++t) sq[t] = 0.0f;
+
+    for (int jv = jstart; jv <= jend; ++jv) {
+        zw[2 * K - 1] = ldz<K>(syz_in, g, jv + K - 1, c);
+        const float sc = ldz<K>(sxy_in, g, jv, c);
+        lsx[K + lane] = sc;
+        if (lane < K) lsx[lane] = ldz<K>(sxy_in, g, jv, c0 - 2 * K + 1 + lane);
+        if (lane < K - 1) lsx[kWave + K + lane] = ldz<K>(sxy_in, g, jv, c0 + 65 - K + lane);
+        __syncthreads();
+
+        float vnew = 0.0f;
+        if (jv >= 1 && jv <= g.ny && in_x) {
+            const bool own_v = own_x && jv >= j0 && jv <= j1;
+            float sxy_x = a.hc[1] * (lsx[K + lane] - lsx[K + lane - 1]);
+            float syy_y = a.hc[1] * (zw[K] - zw[K - 1]);
+#pragma unroll
+            for (int m = 2; m <= K; ++m) {
+                sxy_x = sxy_x + a.hc[m] * (lsx[K + lane + m - 1] - lsx[K + lane - m]);
+                const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
+                syy_y = syy_y + hy * (zw[K + m - 1] - zw[K - m]);
+            }
+            if (fw > 0) {
+                if (c <= fw) {
+                    const long long pi = ((long long)s * g.ny + (jv - 1)) * fw2 + (c - 1);
+                    const float p = a.pml.bx[c] * a.psx_in[pi] + a.pml.ax[c] * sxy_x;
+                    if (own_v) a.psx_out[pi] = p;
+                    sxy_x = sxy_x / a.pml.Kx[c] + p;
+                }
+                if (c >= g.nx - fw + 1) {
+                    const int h1 = c - g.nx + fw2;
+                    const long long pi = ((long long)s * g.ny + (jv - 1)) * fw2 + (h1 - 1);
+                    const float p = a.pml.bx[h1] * a.psx_in[pi] + a.pml.ax[h1] * sxy_x;
+                    if (own_v) a.psx_out[pi] = p;
+                    sxy_x = sxy_x / a.pml.Kx[h1] + p;
+                }
+                if (!g.free_surf && jv <= fw) {
+                    const long long pi = ((long long)s * fw2 + (jv - 1)) * g.nx + (c - 1);
+                    const float p = a.pml.by[jv] * a.psy_in[pi] + a.pml.ay[jv] * syy_y;
+                    if (own_v) a.psy_out[pi] = p;
+                    syy_y = syy_y / a.pml.Ky[jv] + p;
+                }
+                if (jv >= g.ny - fw + 1) {
+                    const int h1 = jv - g.ny + fw2;
+                    const long long pi = ((long long)s * fw2 + (h1 - 1)) * g.nx + (c - 1);
+                    const float p = a.pml.by[h1] * a.psy_in[pi] + a.pml.ay[h1] * syy_y;
+                    if (own_v) a.psy_out[pi] = p;
+                    syy_y = syy_y / a.pml.Ky[h1] + p;
+                }
+            }
+            const long long gi = gidx(g, jv, c);
+            const float vold = vy_in[gi];
+            const float bb = a.bfield[gi];
+            const float tsum = sxy_x + syy_y;
+            const long long ci = (long long)(jv - 1) * g.nx + (c - 1);
+            if (!ADJ && fstore && own_v) {
+                const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
+                if (a.store_mode == 1)
+                    a.snap[(long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci] = acc;
+                else
+                    a.stage[(long long)s * 3 * nxny + ci] = acc;
+            }
+            if (ADJ && corr && own_v) {
+                // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
+                float f0;
+                const long long sidx = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
+                if (a.snap_kind == 0)
+                    f0 = a.snap[sidx];
+                else {
+                    const float sc0 = a.snap_scale[((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3 + 0];
+                    f0 = (a.snap_kind == 1 ? bf162f(a.snap16[sidx]) : h2f(a.snap16[sidx])) * sc0;
+                }
+                float *gr = a.grad_rho + (long long)s * nxny + ci;
+                *gr = *gr + (vold * f0);
+            }
+            vnew = vold + ((bb * a.DT) * tsum) / a.DH;
+            if (!ADJ) {
+                for (int l = sb; l < se; ++l)
+                    if (a.src_j[l] == jv && a.src_i[l] == c) vnew = vnew + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
+            } else {
+                const int off = a.rec_row[jv];
+                if (off >= 0) {
+                    const int r = a.rec_tab[off + c];
+                    if (r >= 0) vnew = vnew + a.resid[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)];
+                }
+            }
+            if (own_v) {
+                vy_out[gi] = vnew;
+                if (!ADJ && a.seis != nullptr) {
+                    const int off = a.rec_row[jv];
+                    if (off >= 0) {
+                        const int r = a.rec_tab[off + c];
+                        if (r >= 0) a.seis[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)] = vnew;
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int t = 0; t < 2 * K - 1; ++t) vw[t] = vw[t + 1];
+        vw[2 * K - 1] = vnew;
+#pragma unroll
+        for (int t = 0; t < K; ++t) sq[t] = sq[t + 1];
+        sq[K] = sc;
+
+        const int js = jv - K;  // stress row of this iteration
+        if (js >= j0) {
+            lvy[lane] = vw[K - 1];
+            __syncthreads();
+            if (own_x) {
+                float vyx = a.hc[1] * (lvy[lane + 1] - lvy[lane]);
+                float vyy = a.hc[1] * (vw[K] - vw[K - 1]);
+#pragma unroll
+                for (int m = 2; m <= K; ++m) {
+                    vyx = vyx + a.hc[m] * (lvy[lane + m] - lvy[lane - m + 1]);
+                    vyy = vyy + a.hc[m] * (vw[K - 1 + m] - vw[K - m]);
+                }
+                vyx = vyx * a.dhi;
+                vyy = vyy * a.dhi;
+                if (fw > 0) {
+                    if (c <= fw) {
+                        const long long pi = ((long long)s * g.ny + (js - 1)) * fw2 + (c - 1);
+                        const float p = a.pml.bxh[c] * a.pvx[pi] + a.pml.axh[c] * vyx;
+                        a.pvx[pi] = p;
+                        vyx = vyx / a.pml.Kxh[c] + p;
+                    }
+                    if (c >= g.nx - fw + 1) {
+                        const int h1 = c - g.nx + fw2;
+                        const long long pi = ((long long)s * g.ny + (js - 1)) * fw2 + (h1 - 1);
+                        const float p = a.pml.bxh[h1] * a.pvx[pi] + a.pml.axh[h1] * vyx;
+                        a.pvx[pi] = p;
+                        vyx = vyx / a.pml.Kxh[h1] + p;
+                    }
+                    if (!g.free_surf && js <= fw) {
+                        // Q2: the viscoelastic kernel uses the full-grid y profile on top
+                        const long long pi = ((long long)s * fw2 + (js - 1)) * g.nx + (c - 1);
+                        const float bY = VISCO ? a.pml.by[js] : a.pml.byh[js];
+                        const float aY = VISCO ? a.pml.ay[js] : a.pml.ayh[js];
+                        const float KY = VISCO ? a.pml.Ky[js] : a.pml.Kyh[js];
+                        const float p = bY * a.pvy[pi] + aY * vyy;
+                        a.pvy[pi] = p;
+                        vyy = vyy / KY + p;
+                    }
+                    if (js >= g.ny - fw + 1) {
+                        const int h1 = js - g.ny + fw2;
+                        const long long pi = ((long long)s * fw2 + (h1 - 1)) * g.nx + (c - 1);
+                        const float p = a.pml.byh[h1] * a.pvy[pi] + a.pml.ayh[h1] * vyy;
+                        a.pvy[pi] = p;
+                        vyy = vyy / a.pml.Kyh[h1] + p;
+                    }
+                }
+                const long long gs = gidx(g, js, c);
+                const float sxo = sq[0];
+                const float syo = zw[0];
+                float sxn, syn;
+                if (!VISCO) {
+                    sxn = sxo + a.uip[gs] * vyx;
+                    syn = syo + a.ujp[gs] * vyy;
+                } else {
+                    float rr[kMaxL], qq[kMaxL];
+                    float sumr = 0.0f, sumq = 0.0f;
+                    float *rp = a.r + (long long)s * g.L * g.plane + gs;
+                    float *qp = a.q + (long long)s * g.L * g.plane + gs;
+#pragma unroll
+                    for (int l = 0; l < kMaxL; ++l)
+                        if (l < g.L) {
+                            rr[l] = rp[(long long)l * g.plane];
+                            qq[l] = qp[(long long)l * g.plane];
+                            sumr += rr[l];
+                            sumq += qq[l];
+                        }
+                    sxn = sxo + ((a.fipjp[gs] * vyx) + (a.dthalbe * sumr));
+                    syn = syo + ((a.fcoef[gs] * vyy) + (a.dthalbe * sumq));
+                    sumr = 0.0f;
+                    sumq = 0.0f;
+#pragma unroll
+                    for (int l = 0; l < kMaxL; ++l)
+                        if (l < g.L) {
+                            rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (a.dip[(long long)l * g.plane + gs] * vyx));
+                            qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (a.dcoef[(long long)l * g.plane + gs] * vyy));
+                            rp[(long long)l * g.plane] = rr[l];
+                            qp[(long long)l * g.plane] = qq[l];
+                            sumr += rr[l];
+                            sumq += qq[l];
+                        }
+                    sxn = sxn + (a.dthalbe * sumr);
+                    syn = syn + (a.dthalbe * sumq);
+                }
+                sxy_out[gs] = sxn;
+                syz_out[gs] = syn;
+                const long long ci = (long long)(js - 1) * g.nx + (c - 1);
+                const float vc = vw[K - 1];
+                if (!ADJ && fstore) {
+                    if (a.store_mode == 1) {
+                        float *b0 = a.snap + (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
+                        b0[nxny] = vyy;        // uy  (forward_prop_y)
+                        b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
+                    } else {
+                        float *b0 = a.stage + (long long)s * 3 * nxny + ci;
+                        b0[nxny] = vyy;
+                        b0[2 * nxny] = vyx;
+                    }
+                    float *wp = a.wacc + (long long)s * nxny + ci;
+                    *wp = *wp + vc * vc;       // eprecond.c:11
+                }
+                if (ADJ && corr) {
+                    float fy, fu;
+                    const long long sidx = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
+                    if (a.snap_kind == 0) {
+                        fy = a.snap[sidx + nxny];
+                        fu = a.snap[sidx + 2 * nxny];
+                    } else {
+                        const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
+                        if (a.snap_kind == 1) {
+                            fy = bf162f(a.snap16[sidx + nxny]) * scp[1];
+                            fu = bf162f(a.snap16[sidx + 2 * nxny]) * scp[2];
+                        } else {
+                            fy = h2f(a.snap16[sidx + nxny]) * scp[1];
+                            fu = h2f(a.snap16[sidx + 2 * nxny]) * scp[2];
+                        }
+                    }
+                    float *gu = a.grad_u + (long long)s * nxny + ci;
+                    *gu = *gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
+                    float *wp = a.wacc + (long long)s * nxny + ci;
+                    *wp = *wp + vc * vc;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2 * K - 1; ++t) zw[t] = zw[t + 1];
+    }
+
+    // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
+    if (!ADJ && fstore && a.store_mode == 2) {
+        // recomputed from the staged values this wave wrote (owned cells only)
+        float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+        if (own_x) {
+            const float *st = a.stage + (long long)s * 3 * nxny;
+            for (int j = j0; j <= j1; ++j) {
+                const long long ci = (long long)(j - 1) * g.nx + (c - 1);
+                m0 = fmaxf(m0, fabsf(st[ci]));
+                m1 = fmaxf(m1, fabsf(st[nxny + ci]));
+                m2 = fmaxf(m2, fabsf(st[2 * nxny + ci]));
+            }
+        }
+        m0 = wave_max(m0);
+        m1 = wave_max(m1);
+        m2 = wave_max(m2);
+        if (lane == 0) {
+            atomicMax(&a.stage_max[s * 3 + 0], __float_as_uint(m0));
+            atomicMax(&a.stage_max[s * 3 + 1], __float_as_uint(m1));
+            atomicMax(&a.stage_max[s * 3 + 2], __float_as_uint(m2));
+        }
+    }
+}
+
+template <int K>
+static hipError_t launch_step_k(const StepArgs &a, bool visco, bool adjoint, hipStream_t st)
+{
+    dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave);
+    if (!visco && !adjoint) hipLaunchKernelGGL((k_step<K, false, false>), grid, block, 0, st, a);
+    if (!visco && adjoint) hipLaunchKernelGGL((k_step<K, false, true>), grid, block, 0, st, a);
+    if (visco && !adjoint) hipLaunchKernelGGL((k_step<K, true, false>), grid, block, 0, st, a);
+    if (visco && adjoint) hipLaunchKernelGGL((k_step<K, true, true>), grid, block, 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipStream_t st)
+{
+    switch (K) {
+    case 1: return launch_step_k<1>(a, visco, adjoint, st);
+    case 2: return launch_step_k<2>(a, visco, adjoint, st);
+    case 3: return launch_step_k<3>(a, visco, adjoint, st);
+    case 4: return launch_step_k<4>(a, visco, adjoint, st);
+    case 5: return launch_step_k<5>(a, visco, adjoint, st);
+    case 6: return launch_step_k<6>(a, visco, adjoint, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Model preparation (denise.c:726-743): periodic halo (matcopy.c with one
+// rank), av_mue.c (INVMAT1 1), av_tau.c, 1/rho of update_v_PML.c:181-182 and
+// prepare_update_s.c:41-62.  Double where the reference promotes.
+// ---------------------------------------------------------------------------
+__global__ void k_model_wrap(Geo g, float *vs, float *rho, float *taus)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nx+1
+    const int j = blockIdx.y;                              // 0..ny+1
+    if (i > g.nx + 1 || j > g.ny + 1) return;
+    const bool edge = (i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1);
+    if (!edge) return;
+    // rows first, then columns (matcopy.c:40-123): the final value of a halo
+    // cell is the interior cell reached by wrapping each index once
+    const int jj = (j == 0) ? g.ny : (j == g.ny + 1 ? 1 : j);
+    const int ii = (i == 0) ? g.nx : (i == g.nx + 1 ? 1 : i);
+    const long long d = gidx(g, j, i), sidx = gidx(g, jj, ii);
+    vs[d] = vs[sidx];
+    rho[d] = rho[sidx];
+    if (taus) taus[d] = taus[sidx];
+}
+
+__global__ void k_model_coef(ModelPrepArgs a)
+{
+    const Geo &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long long o = gidx(g, j, i);
+    const long long oi = gidx(g, j, i + 1), oj = gidx(g, j + 1, i), oij = gidx(g, j + 1, i + 1);
+    // av_mue.c:33-46
+    const float u1 = a.rho[o] * a.vs[o] * a.vs[o];
+    const float u2 = a.rho[oi] * a.vs[oi] * a.vs[oi];
+    const float u3 = a.rho[oj] * a.vs[oj] * a.vs[oj];
+    float uip = (float)(2.0 / ((1.0 / (double)u1) + (1.0 / (double)u2)));
+    float ujp = (float)(2.0 / ((1.0 / (double)u1) + (1.0 / (double)u3)));
+    if ((u1 == 0.0f) || (u2 == 0.0f)) uip = 0.0f;
+    if ((u1 == 0.0f) || (u3 == 0.0f)) ujp = 0.0f;
+    a.uip[o] = uip;
+    a.ujp[o] = ujp;
+    // update_v_PML.c:181-182
+    const float rv = a.rho[o];
+    float b = (float)(1.0 / (double)rv);
+    if ((double)rv < 1e-20) b = 0.0f;
+    a.bfield[o] = b;
+    if (a.L > 0) {
+        // av_tau.c:12
+        const float tsum = a.taus[o] + a.taus[oi] + a.taus[oj] + a.taus[oij];
+        const float tip = (float)(0.25 * (double)tsum);
+        a.tausipjp[o] = tip;
+        const float ptaus = a.taus[o];
+        // prepare_update_s.c:45-57
+        const float mujp = (float)((double)ujp / (1.0 + (double)(a.sumu * ptaus)));
+        const float muip = (float)((double)uip / (1.0 + (double)(a.sumu * tip)));
+        a.fipjp[o] = (float)((double)(muip * a.DT) * (1.0 + (double)((float)a.L * tip)));
+        a.fcoef[o] = (float)((double)(mujp * a.DT) * (1.0 + (double)((float)a.L * ptaus)));
+        for (int l = 0; l < a.L; ++l) {
+            a.dip[(long long)l * g.plane + o] = muip * a.eta[l] * tip;
+            a.dcoef[(long long)l * g.plane + o] = mujp * a.eta[l] * ptaus;
+        }
+    }
+}
+
+hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st)
+{
+    dim3 b1(128), g1((a.g.nx + 2 + 127) / 128, a.g.ny + 2);
+    hipLaunchKernelGGL(k_model_wrap, g1, b1, 0, st, a.g, a.vs, a.rho, a.L > 0 ? a.taus : (float *)nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 b2(128), g2((a.g.nx + 127) / 128, a.g.ny);
+    hipLaunchKernelGGL(k_model_coef, g2, b2, 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Residual and misfit (calc_res.c:150-257 LNORM 2 / GRAD_FORM 2, Q7, and
+// calc_energy.c:53-61).  One thread per (shot, trace); per-trace FP64 partial
+// sums are written in place and reduced on the host in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn, const float *obs,
+                           float *resid, double *part, float DT)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nshots * ntr) return;
+    const int s = t / ntr;
+    const int ns = shot_nt[s];
+    const float *sy = syn + (long long)t * nt_max;
+    const float *ob = obs + (long long)t * nt_max;
+    float *df = resid + (long long)t * nt_max;
+    double l2 = 0.0, en = 0.0;
+    float intseis = 0.0f;
+    for (int j = 1; j <= ns; ++j) {
+        const float o = (j == 1) ? sy[0] : ob[j - 1];   // calc_res.c:152
+        const float d = sy[j - 1] - o;
+        df[ns - j] = d;                                 // time reversal, calc_res.c:224
+        l2 += (double)d * (double)d;
+        intseis = intseis + o;                          // calc_energy.c:59-60
+        en += (double)(intseis * intseis * DT * DT);
+    }
+    for (int j = ns; j < nt_max; ++j) df[j] = 0.0f;
+    part[2 * (long long)t] = l2;
+    part[2 * (long long)t + 1] = en;
+}
+
+hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn, const float *obs,
+                           float *resid, double *part, float DT, hipStream_t st)
+{
+    const int n = nshots * ntr;
+    hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, ntr, nt_max, shot_nt, syn, obs,
+                       resid, part, DT);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// eprecond1.c:15-70 and denise.c:1665-1694: We = sqrt(Ws*Wr) per shot,
+// We += 0.005*max(We), scale the shot gradients and add them to the sums in
+// shot order (deterministic).
+// ---------------------------------------------------------------------------
+__global__ void k_we_max(int nxny, const float *Ws, const float *Wr, unsigned int *maxbits)
+{
+    const int s = blockIdx.y;
+    float m = 0.0f;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x) {
+        const long long o = (long long)s * nxny + t;
+        const float we = (float)sqrt((double)(Ws[o] * Wr[o]));
+        m = fmaxf(m, we);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[s], __float_as_uint(m));
+}
+
+__global__ void k_eprecond_acc(int nshots, int nxny, const float *Ws, const float *Wr, const float *gu_shot,
+                               const float *grho_shot, float *gu, float *grho, float C_vs, float C_rho,
+                               const unsigned int *maxbits)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxny) return;
+    const float EPSILON_WE = 0.005f;
+    float au = gu[t], ar = grho[t];
+    for (int s = 0; s < nshots; ++s) {
+        const long long o = (long long)s * nxny + t;
+        float we = (float)sqrt((double)(Ws[o] * Wr[o]));
+        we = we + (EPSILON_WE * __uint_as_float(maxbits[s]));
+        const float su = gu_shot[o] / (we * C_vs * C_vs);
+        const float sr = grho_shot[o] / (we * C_rho * C_rho);
+        ar = ar + sr;
+        au = au + su;
+    }
+    gu[t] = au;
+    grho[t] = ar;
+}
+
+hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
+                                      const float *gu_shot, const float *grho_shot, float *gu, float *grho,
+                                      float C_vs, float C_rho, unsigned int *maxbits, hipStream_t st)
+{
+    const int nxny = nx * ny;
+    hipError_t e = launch_fill_u32(maxbits, (size_t)nshots, 0u, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, Ws, Wr, maxbits);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_eprecond_acc, dim3((nxny + 255) / 256), dim3(256), 0, st, nshots, nxny, Ws, Wr, gu_shot,
+                       grho_shot, gu, grho, C_vs, C_rho, maxbits);
+    return hipGetLastError();
+}
+
+// denise.c:1756-1811 with INVMAT 0, INVMAT1 1
+__global__ void k_chain_rule(Geo g, const float *gu, const float *grho, const float *vs, const float *rho, float DT,
+                             float *gvs, float *grh)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int j = t / g.nx + 1, i = t % g.nx + 1;
+    const long long o = gidx(g, j, i);
+    const float pu = vs[o], pr = rho[o];
+    const float gmu = -DT * gu[t];
+    const float grs = -DT * grho[t];
+    gvs[t] = (float)(2.0 * (double)pr * (double)pu * (double)gmu);
+    grh[t] = (pu * pu * gmu) + grs;
+}
+
+hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, const float *grho, const float *vs_pad,
+                             const float *rho_pad, float DT, float *grad_vs, float *grad_rho, hipStream_t st)
+{
+    (void)nx;
+    (void)ny;
+    const int n = g.nx * g.ny;
+    hipLaunchKernelGGL(k_chain_rule, dim3((n + 255) / 256), dim3(256), 0, st, g, gu, grho, vs_pad, rho_pad, DT,
+                       grad_vs, grad_rho);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision snapshot codec: x16 = round(x * T/max), decode scale
+// max/T per snapshot and field (T = 1 for bf16, 2^15 for fp16).
+// ---------------------------------------------------------------------------
+__global__ void k_snap_pack(int nxny, const float *stage, const unsigned int *stage_max, uint16_t *snap16,
+                            float *snap_scale, long long shot_stride16, int nsnap, int slot, int kind)
+{
+    const int s = blockIdx.z, f = blockIdx.y;
+    const float m = __uint_as_float(stage_max[s * 3 + f]);
+    const float T = (kind == 1) ? 1.0f : 32768.0f;
+    const float enc = (m > 0.0f) ? (T / m) : 1.0f;
+    const float dec = (m > 0.0f) ? (m / T) : 1.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) snap_scale[((long long)s * nsnap + slot) * 3 + f] = dec;
+    const float *src = stage + ((long long)s * 3 + f) * nxny;
+    uint16_t *dst = snap16 + (long long)s * shot_stride16 + ((long long)slot * 3 + f) * nxny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x) {
+        const float x = src[t] * enc;
+        dst[t] = (kind == 1) ? f2bf16(x) : f2h(x);
+    }
+}
+
+hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, const unsigned int *stage_max,
+                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap, int slot,
+                            int kind, hipStream_t st)
+{
+    const int nxny = nx * ny;
+    hipLaunchKernelGGL(k_snap_pack, dim3(std::min(1024, (nxny + 255) / 256), 3, nshots), dim3(256), 0, st, nxny,
+                       stage, stage_max, snap16, snap_scale, shot_stride16, nsnap, slot, kind);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(float *p, size_t n, float v)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+__global__ void k_fill_u32(unsigned int *p, size_t n, unsigned int v)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Split kernels behind the reference-named drop-ins: one thread per interior
+// cell, in-place update exactly as the reference loop bodies.
+// ---------------------------------------------------------------------------
+__global__ void k_split_v(SplitArgs a)
+{
+    const Geo &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
+    float sxy_x = a.hc[1] * (a.sxy[gidx(g, j, i)] - a.sxy[gidx(g, j, i - 1)]);
+    float syy_y = a.hc[1] * (a.syz[gidx(g, j, i)] - a.syz[gidx(g, j - 1, i)]);
+    for (int m = 2; m <= K; ++m) {
+        sxy_x = sxy_x + a.hc[m] * (a.sxy[gidx(g, j, i + m - 1)] - a.sxy[gidx(g, j, i - m)]);
+        const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
+        syy_y = syy_y + hy * (a.syz[gidx(g, j + m - 1, i)] - a.syz[gidx(g, j - m, i)]);
+    }
+    if (fw > 0) {
+        float *psx = a.psi_b, *psy = a.psi_a;
+        if (i <= fw) {
+            float &p = psx[(long long)(j - 1) * fw2 + (i - 1)];
+            p = a.pml.bx[i] * p + a.pml.ax[i] * sxy_x;
+            sxy_x = sxy_x / a.pml.Kx[i] + p;
+        }
+        if (i >= g.nx - fw + 1) {
+            const int h1 = i - g.nx + fw2;
+            float &p = psx[(long long)(j - 1) * fw2 + (h1 - 1)];
+            p = a.pml.bx[h1] * p + a.pml.ax[h1] * sxy_x;
+            sxy_x = sxy_x / a.pml.Kx[h1] + p;
+        }
+        if (!g.free_surf && j <= fw) {
+            float &p = psy[(long long)(j - 1) * g.nx + (i - 1)];
+            p = a.pml.by[j] * p + a.pml.ay[j] * syy_y;
+            syy_y = syy_y / a.pml.Ky[j] + p;
+        }
+        if (j >= g.ny - fw + 1) {
+            const int h1 = j - g.ny + fw2;
+            float &p = psy[(long long)(h1 - 1) * g.nx + (i - 1)];
+            p = a.pml.by[h1] * p + a.pml.ay[h1] * syy_y;
+            syy_y = syy_y / a.pml.Ky[h1] + p;
+        }
+    }
+    const long long o = gidx(g, j, i);
+    const float rv = a.rho[o];
+    float b = (float)(1.0 / (double)rv);
+    if ((double)rv < 1e-20) b = 0.0f;
+    if (a.sw == 0) a.vyp1[o] = (b * (sxy_x + syy_y)) / a.DH;
+    if (a.sw == 1) a.vyp1[o] = a.vy[o];
+    a.vy[o] = a.vy[o] + ((b * a.DT) * (sxy_x + syy_y)) / a.DH;
+}
+
+template <bool VISCO>
+__global__ void k_split_s(SplitArgs a)
+{
+    const Geo &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > g.nx || j > g.ny) return;
+    const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
+    const float dhi = VISCO ? (float)(1.0 / (double)a.DH) : a.DT / a.DH;
+    const float dthalbe = (float)((double)a.DT / 2.0);
+    float vyx = a.hc[1] * (a.vy[gidx(g, j, i + 1)] - a.vy[gidx(g, j, i)]);
+    float vyy = a.hc[1] * (a.vy[gidx(g, j + 1, i)] - a.vy[gidx(g, j, i)]);
+    for (int m = 2; m <= K; ++m) {
+        vyx = vyx + a.hc[m] * (a.vy[gidx(g, j, i + m)] - a.vy[gidx(g, j, i - m + 1)]);
+        vyy = vyy + a.hc[m] * (a.vy[gidx(g, j + m, i)] - a.vy[gidx(g, j - m + 1, i)]);
+    }
+    vyx = vyx * dhi;
+    vyy = vyy * dhi;
+    if (fw > 0) {
+        float *pvy = a.psi_a, *pvx = a.psi_b;
+        if (i <= fw) {
+            float &p = pvx[(long long)(j - 1) * fw2 + (i - 1)];
+            p = a.pml.bxh[i] * p + a.pml.axh[i] * vyx;
+            vyx = vyx / a.pml.Kxh[i] + p;
+        }
+        if (i >= g.nx - fw + 1) {
+            const int h1 = i - g.nx + fw2;
+            float &p = pvx[(long long)(j - 1) * fw2 + (h1 - 1)];
+            p = a.pml.bxh[h1] * p + a.pml.axh[h1] * vyx;
+            vyx = vyx / a.pml.Kxh[h1] + p;
+        }
+        if (!g.free_surf && j <= fw) {
+            float &p = pvy[(long long)(j - 1) * g.nx + (i - 1)];
+            const float bY = VISCO ? a.pml.by[j] : a.pml.byh[j];
+            const float aY = VISCO ? a.pml.ay[j] : a.pml.ayh[j];
+            const float KY = VISCO ? a.pml.Ky[j] : a.pml.Kyh[j];
+            p = bY * p + aY * vyy;
+            vyy = vyy / KY + p;
+        }
+        if (j >= g.ny - fw + 1) {
+            const int h1 = j - g.ny + fw2;
+            float &p = pvy[(long long)(h1 - 1) * g.nx + (i - 1)];
+            p = a.pml.byh[h1] * p + a.pml.ayh[h1] * vyy;
+            vyy = vyy / a.pml.Kyh[h1] + p;
+        }
+    }
+    const long long o = gidx(g, j, i);
+    if (!VISCO) {
+        a.uy[o] = vyy;
+        a.uyx[o] = vyx;
+        a.sxy[o] = a.sxy[o] + a.uip[o] * vyx;
+        a.syz[o] = a.syz[o] + a.ujp[o] * vyy;
+    } else {
+        float sumr = 0.0f, sumq = 0.0f;
+        for (int l = 0; l < g.L; ++l) {
+            sumr += a.r[(long long)l * g.plane + o];
+            sumq += a.q[(long long)l * g.plane + o];
+        }
+        a.sxy[o] = a.sxy[o] + ((a.fipjp[o] * vyx) + (dthalbe * sumr));
+        a.syz[o] = a.syz[o] + ((a.fcoef[o] * vyy) + (dthalbe * sumq));
+        sumr = 0.0f;
+        sumq = 0.0f;
+        for (int l = 0; l < g.L; ++l) {
+            float &rl = a.r[(long long)l * g.plane + o];
+            float &ql = a.q[(long long)l * g.plane + o];
+            rl = a.bip[l] * (rl * a.cip[l] - (a.dip[(long long)l * g.plane + o] * vyx));
+            ql = a.bjm[l] * (ql * a.cjm[l] - (a.dcoef[(long long)l * g.plane + o] * vyy));
+            sumr += rl;
+            sumq += ql;
+        }
+        a.uy[o] = vyy;
+        a.uyx[o] = vyx;
+        a.sxy[o] = a.sxy[o] + (dthalbe * sumr);
+        a.syz[o] = a.syz[o] + (dthalbe * sumq);
+    }
+}
+
+hipError_t launch_split_v(const SplitArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_split_v, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_split_s(const SplitArgs &a, bool visco, hipStream_t st)
+{
+    if (visco)
+        hipLaunchKernelGGL(k_split_s<true>, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_split_s<false>, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace dn

@@ -1,0 +1,172 @@
+/*
+ * denise_sh.h -- C-ABI of the MI355X SH time-stepping and adjoint-gradient
+ * library (libdenise_sh.so).
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the reference's names and signatures
+ *     (fd.h of DENISE-SH).  They take the reference's Numerical-Recipes
+ *     offset-indexed float** arrays in host memory, run the hand-written HIP
+ *     kernels on the device and leave the results in the same host arrays,
+ *     so a caller written against the reference links against this library
+ *     unchanged.  The reference's implicit inputs (globals DT, DH, FDORDER,
+ *     FW, FREE_SURF, L, ...) are handed over once with dn_set_globals().
+ *
+ *  2. A handle API (dn_*) for the fast path: the model, acquisition and
+ *     per-shot data stay resident in HBM, many shots advance together in one
+ *     launch per time step, and only scalars and requested results cross the
+ *     boundary.
+ *
+ * Conventions: grid indices are 1-based like the reference (i = x, j = y,
+ * y pointing down).  Host 2-D arrays in the handle API are row-major
+ * [ny][nx] float32 unless stated otherwise.  Functions returning int return
+ * 0 on success and a negative code on error; dn_last_error() gives the text.
+ */
+#ifndef DENISE_SH_H
+#define DENISE_SH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Layer 1: reference drop-ins
+ * ---------------------------------------------------------------------- */
+
+/* Snapshot of the reference globals the kernels read implicitly
+ * (update_v_PML.c:23-29, update_s_visc_PML.c:24-28, globvar.h).  POS,
+ * NPROCX and NPROCY are fixed to the single-domain values (0, 1, 1). */
+typedef struct dn_globals {
+    int NX, NY;          /* local = global grid size                       */
+    int FDORDER;         /* 2..12                                          */
+    int FW;              /* CPML width                                     */
+    int FREE_SURF;       /* 1: no top CPML (implicit free surface)         */
+    int BOUNDARY;        /* must be 0 (periodic wavefield not provided)    */
+    int L;               /* relaxation mechanisms                          */
+    int GRAD_FORM;       /* must be 2                                      */
+    int QUELLTYPB;       /* adjoint source type (2: force in y)            */
+    int NT;              /* number of time steps (psource)                 */
+    float DT, DH;
+} dn_globals;
+
+/* Sets the implicit inputs of the drop-ins below; call once after the
+ * parameter file is read (where the reference calls exchange_par()). */
+int dn_set_globals(const dn_globals *g);
+
+/* update_v_PML.c:12-16 (fd.h:406-411).  vym1, utty, absorb_coeff are
+ * unused, as in the reference. */
+void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float **vy, float **vyp1, float **vym1,
+                  float **utty, float **syz, float **sxy, float **rho, float **srcpos_loc, float **signals1,
+                  int nsrc, float **absorb_coeff, float *hc, int infoout, int sw, float *K_x, float *a_x,
+                  float *b_x, float *K_x_half, float *a_x_half, float *b_x_half, float *K_y, float *a_y,
+                  float *b_y, float *K_y_half, float *a_y_half, float *b_y_half, float **psi_syy_y,
+                  float **psi_sxy_x);
+
+/* update_s_elastic_PML.c:12-16 (fd.h:376-381).  u, absorb_coeff and rho
+ * are unused, as in the reference. */
+void update_s_elastic_PML(int nx1, int nx2, int ny1, int ny2, float **vy, float **uy, float **uyx, float **syz,
+                          float **sxy, float **u, float **uip, float **ujp, float **absorb_coeff, float **rho,
+                          float *hc, int infoout, float *K_x, float *a_x, float *b_x, float *K_x_half,
+                          float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y, float *K_y_half,
+                          float *a_y_half, float *b_y_half, float **psi_vyy, float **psi_vyx);
+
+/* update_s_visc_PML.c:12-17 (fd.h:386-392).  r, q, dip, d are the
+ * reference's f3tensors [j][i][1..L]; ujp, uip, rho, p, g, e are unused, as
+ * in the reference. */
+void update_s_visc_PML(int nx1, int nx2, int ny1, int ny2, float **vy, float **uy, float **uyx, float **syz,
+                       float **sxy, float **ujp, float **uip, float **rho, float *hc, int infoout, float ***r,
+                       float ***p, float ***q, float **fipjp, float **f, float **g, float *bip, float *bjm,
+                       float *cip, float *cjm, float ***d, float ***e, float ***dip, float *K_x, float *a_x,
+                       float *b_x, float *K_x_half, float *a_x_half, float *b_x_half, float *K_y, float *a_y,
+                       float *b_y, float *K_y_half, float *a_y_half, float *b_y_half, float **psi_vyy,
+                       float **psi_vyx);
+
+/* seismo_ssg.c:9 (fd.h:296-297): sectionvy[itr][lsamp] = vy[recpos[2][itr]][recpos[1][itr]]. */
+void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvy, float **vy, float **syy, float **u,
+                float *hc);
+
+/* psource.c:9-10 (no prototype in fd.h): explosive source, sxx,syy +=
+ * centred time derivative of the signal (one-sided at nt = 1 and NT). */
+void psource(int nt, float **sxx, float **syy, float **srcpos_loc, float **signals, int nsrc);
+
+/* ------------------------------------------------------------------------
+ * Layer 2: handle API (fast path).  Not in the reference: it replaces the
+ * per-step call sequence of denise.c:1179-1346 / 1493-1651 and the shot
+ * loop denise.c:875-1702 with resident, batched execution.
+ * ---------------------------------------------------------------------- */
+
+typedef struct dn_ctx dn_ctx;
+
+typedef struct dn_params {
+    int nx, ny;          /* NX, NY                                         */
+    int fdorder;         /* FDORDER 2..12                                  */
+    int maxrelerror;     /* MAXRELERROR 0..4                               */
+    int fw;              /* FW                                             */
+    int free_surf;       /* FREE_SURF                                      */
+    int L;               /* relaxation mechanisms (0 = elastic), <= 4      */
+    float fl[4];         /* FL[1..L]                                       */
+    float dt, dh;        /* DT, DH                                         */
+    float damping, fpml; /* DAMPING, FPML                                  */
+    float npower, k_max_pml;
+    int dtinv;           /* DTINV: snapshot every dtinv-th step            */
+    int snap_precision;  /* 0: fp32, 1: bf16, 2: fp16 (per-snapshot scale) */
+} dn_params;
+
+/* Context on the current HIP device; NULL on failure (dn_last_error(NULL)). */
+dn_ctx *dn_create(const dn_params *p);
+void dn_destroy(dn_ctx *ctx);
+/* Error text of the last failing call on ctx (or of dn_create when NULL). */
+const char *dn_last_error(const dn_ctx *ctx);
+
+/* Model: vs, rho and (visco) qs, host [ny][nx].  Runs matcopy (periodic
+ * halo), av_mue, av_tau and prepare_update_s on the device
+ * (denise.c:726-743).  The first call fixes C_vs and C_rho (means of the
+ * starting model, denise.c:778-816). */
+int dn_set_model(dn_ctx *ctx, const float *vs, const float *rho, const float *qs);
+int dn_get_model(dn_ctx *ctx, float *vs, float *rho);
+
+/* Receivers: grid indices (1-based), duplicates must already be removed
+ * (receiver.c:50-63 does this on the host). */
+int dn_set_receivers(dn_ctx *ctx, int ntr, const int *rec_i, const int *rec_j);
+
+/* Shots: nshots shots with one point force each (QUELLTYP 3) at grid
+ * (src_i, src_j); signals host [nshots][nt_max] (already filtered); nt[s]
+ * the number of time steps of shot s (<= nt_max). */
+int dn_set_shots(dn_ctx *ctx, int nshots, int nt_max, const int *nt, const int *src_i, const int *src_j,
+                 const float *signals);
+
+/* Observed data host [nshots][ntr][nt_max] (samples 1..nt[s] used). */
+int dn_set_observed(dn_ctx *ctx, const float *obs);
+
+/* Maximum number of shots advanced together (0 = automatic from free HBM). */
+int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
+
+/* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis
+ * (host [nshots][ntr][nt_max]) when non-NULL. */
+int dn_forward(dn_ctx *ctx, int shot0, int nshots, float *seis);
+
+/* Misfit gradient over shots [shot0, shot0+nshots) (denise.c:875-1694 and
+ * the chain rule 1743-1813): forward with snapshot storage, residual
+ * (calc_res LNORM 2), adjoint with correlation, energy preconditioning.
+ * Accumulates into the context's gradient when accumulate != 0, else
+ * starts from zero.  l2/energy receive the FP64 misfit sums over these
+ * shots. */
+int dn_gradient(dn_ctx *ctx, int shot0, int nshots, int accumulate, double *l2, double *energy);
+
+/* Gradient after the chain rule (waveconv_u -> vs, waveconv_rho -> rho),
+ * host [ny][nx].  raw_u/raw_rho (optional) receive the preconditioned
+ * shot sums before the chain rule. */
+int dn_get_gradient(dn_ctx *ctx, float *grad_vs, float *grad_rho, float *raw_u, float *raw_rho);
+
+/* Debug/parity access: wavefields of one shot after forward modelling of
+ * nsteps steps (vy, sxy, syz host [ny][nx]). */
+int dn_forward_fields(dn_ctx *ctx, int shot, int nsteps, float *vy, float *sxy, float *syz);
+
+/* Timing of the last dn_forward/dn_gradient: device milliseconds of the
+ * time-step kernels and number of kernel launches. */
+int dn_last_timing(dn_ctx *ctx, double *ms_steps, long long *launches, long long *cell_updates);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DENISE_SH_H */

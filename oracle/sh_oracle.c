@@ -1,0 +1,1228 @@
+/*
+ * sh_oracle.c -- CPU restatement of DENISE-SH's SH time-stepping and
+ * adjoint-gradient path.  TEST INFRASTRUCTURE ONLY: this file is the checker
+ * for the HIP product path.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load the libraries built from it.
+ *
+ * The file is compiled twice (see oracle/Makefile):
+ *   -DORC_REAL=float  -DORC_SFX=_f32   the reference's own arithmetic: float
+ *                                      storage, float operators, and the
+ *                                      reference's double-literal promotions
+ *                                      reproduced by keeping its literals;
+ *   -DORC_REAL=double -DORC_SFX=_f64   the same algorithm in FP64 (the
+ *                                      "FP64 oracle" every result is compared
+ *                                      against for its error statement).
+ * Both builds use -ffp-contract=off so no multiply-add is fused, as in the
+ * reference's x86 builds (icc -xAVX / gcc x86-64 have no FMA contraction).
+ *
+ * Arrays follow the reference's index space: wavefields and model fields are
+ * NR matrices [-nd+1..NY+nd][-nd+1..NX+nd] with nd = FDORDER/2+1, flattened
+ * row-major (x fastest) with width W = NX+2nd.  Memory variables r, q and the
+ * coefficient fields dip, d are stored [l][j][i] (one padded plane per
+ * mechanism) instead of the reference's f3tensor [j][i][l]; only the storage
+ * order differs.  CPML vectors are 1-based [1..2FW].
+ *
+ * Pinning of each function (what checks it beyond itself) is listed in the
+ * header block before it.  Functions without an independent pin say
+ * "parity unpinned".
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#ifndef ORC_REAL
+#define ORC_REAL float
+#endif
+#ifndef ORC_SFX
+#define ORC_SFX _f32
+#endif
+
+typedef ORC_REAL real;
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(name, ORC_SFX)
+
+#ifndef PI
+#define PI (3.141592653589793)
+#endif
+/* fd.h:18 */
+#define iround(x) ((int)(floor)(x + 0.5))
+
+/* ------------------------------------------------------------------------
+ * Configuration shared by all entry points.  Scalar parameters arrive as
+ * doubles holding float32 values (the reference reads them as float).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny;          /* interior grid                                   */
+    int fdorder;         /* 2..12                                           */
+    int maxrelerror;     /* 0..4 (Holberg table row)                        */
+    int fw;              /* CPML width in grid points                       */
+    int free_surf;       /* 1: implicit free surface on top                 */
+    int L;               /* relaxation mechanisms, 0 = elastic              */
+    int nt;              /* time steps                                      */
+    int dtinv;           /* snapshot stride                                 */
+    double dt, dh;
+    double damping, fpml, npower, k_max_pml;
+    double fl[8];        /* relaxation frequencies FL[1..L] at fl[0..L-1]   */
+} orc_cfg;
+
+#define ND(c) ((c)->fdorder / 2 + 1)
+#define WID(c) ((c)->nx + 2 * ND(c))
+#define HEI(c) ((c)->ny + 2 * ND(c))
+/* element (j,i) of a padded field, j,i in the reference's index space */
+#define AT(a, c, j, i) ((a)[((size_t)((j) + ND(c) - 1)) * WID(c) + (size_t)((i) + ND(c) - 1)])
+
+size_t F(orc_padded_size)(const orc_cfg *c) { return (size_t)WID(c) * (size_t)HEI(c); }
+
+/* ------------------------------------------------------------------------
+ * holbergcoeff.c:19-90.  Pinned by tests: Taylor rows equal the closed-form
+ * staggered-grid coefficients; Holberg rows reproduce the published table
+ * values typed independently in the test.
+ * ---------------------------------------------------------------------- */
+static const float HCALL[5][6][7] = {
+    {{23.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+     {8.0, 9.0 / 8.0, -1.0 / 24.0, 0.0, 0.0, 0.0, 0.0},
+     {6.0, 75.0 / 64.0, -25.0 / 384.0, 3.0 / 640.0, 0.0, 0.0, 0.0},
+     {5.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0, 0.0, 0.0},
+     {5.0, 19845.0 / 16384.0, -735.0 / 8192.0, 567.0 / 40960.0, -405.0 / 229376.0, 35.0 / 294912.0, 0.0},
+     {4.0, 160083.0 / 131072.0, -12705.0 / 131072.0, 22869.0 / 1310720.0, -5445.0 / 1835008.0, 847.0 / 2359296.0, -63.0 / 2883584.0}},
+    {{49.7, 1.0010, 0.0, 0.0, 0.0, 0.0, 0.0},
+     {8.32, 1.1382, -0.046414, 0.0, 0.0, 0.0, 0.0},
+     {4.77, 1.1965, -0.078804, 0.0081781, 0.0, 0.0, 0.0},
+     {3.69, 1.2257, -0.099537, 0.018063, -0.0026274, 0.0, 0.0},
+     {3.19, 1.2415, -0.11231, 0.026191, -0.0064682, 0.001191, 0.0},
+     {2.91, 1.2508, -0.12034, 0.032131, -0.010142, 0.0029857, -0.00066667}},
+    {{22.2, 1.0050, 0.0, 0.0, 0.0, 0.0, 0.0},
+     {5.65, 1.1534, -0.052806, 0.0, 0.0, 0.0, 0.0},
+     {3.74, 1.2111, -0.088313, 0.011768, 0.0, 0.0, 0.0},
+     {3.11, 1.2367, -0.10815, 0.023113, -0.0046905, 0.0, 0.0},
+     {2.80, 1.2496, -0.11921, 0.031130, -0.0093272, 0.0025161, 0.0},
+     {2.62, 1.2568, -0.12573, 0.036423, -0.013132, 0.0047484, -0.0015979}},
+    {{15.8, 1.0100, 0.0, 0.0, 0.0, 0.0, 0.0},
+     {4.80, 1.1640, -0.057991, 0.0, 0.0, 0.0, 0.0},
+     {3.39, 1.2192, -0.094070, 0.014608, 0.0, 0.0, 0.0},
+     {2.90, 1.2422, -0.11269, 0.026140, -0.0064054, 0.0, 0.0},
+     {2.65, 1.2534, -0.12257, 0.033755, -0.011081, 0.0036784, 0.0},
+     {2.51, 1.2596, -0.12825, 0.038550, -0.014763, 0.0058619, -0.0024538}},
+    {{9.16, 1.0300, 0.0, 0.0, 0.0, 0.0, 0.0},
+     {3.47, 1.1876, -0.072518, 0.0, 0.0, 0.0, 0.0},
+     {2.91, 1.2341, -0.10569, 0.022589, 0.0, 0.0, 0.0},
+     {2.61, 1.2516, -0.12085, 0.032236, -0.011459, 0.0, 0.0},
+     {2.45, 1.2596, -0.12829, 0.038533, -0.014681, 0.0072580, 0.0},
+     {2.36, 1.2640, -0.13239, 0.042217, -0.017803, 0.0081959, -0.0051848}},
+};
+
+int F(orc_holberg)(int fdorder, int maxrelerror, real *hc)
+{
+    int i;
+    if (fdorder < 2 || fdorder > 12 || (fdorder & 1)) return -1;
+    if (maxrelerror < 0 || maxrelerror > 4) return -2;
+    /* the reference table is float; in FP64 mode it is promoted unchanged */
+    for (i = 0; i <= 6; i++) hc[i] = (real)HCALL[maxrelerror][fdorder / 2 - 1][i];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * PML_pro.c:65-297 (CPML profiles, Komatitsch & Martin 2007).  All twenty
+ * vectors are 1-based [1..2FW] and must be zero-filled by the caller, as the
+ * reference's vector() does (util.c:77-87).  Reproduces Q1: the `a`
+ * coefficients are set only where the *integer* abs() of d is nonzero
+ * (PML_pro.c:153-154, 200-201, 250-251, 292-293).  Pinned by a numpy FP64
+ * brute-force evaluation of the closed forms in the tests.
+ * ---------------------------------------------------------------------- */
+void F(orc_pml_pro)(const orc_cfg *c, real *d_x, real *K_x, real *alpha_prime_x, real *a_x, real *b_x,
+                    real *d_x_half, real *K_x_half, real *alpha_prime_x_half, real *a_x_half, real *b_x_half,
+                    real *d_y, real *K_y, real *alpha_prime_y, real *a_y, real *b_y,
+                    real *d_y_half, real *K_y_half, real *alpha_prime_y_half, real *a_y_half, real *b_y_half)
+{
+    const int FW = c->fw, NXG = c->nx, NYG = c->ny;
+    const real DH = (real)c->dh, DT = (real)c->dt, DAMPING = (real)c->damping, FPML = (real)c->fpml;
+    const real npower = (real)c->npower, k_max_PML = (real)c->k_max_pml;
+    const real alpha_max_PML = 2.0 * PI * (FPML / 2.0);
+    real thickness_PML_x, thickness_PML_y, xoriginleft, xoriginright, yoriginbottom, yorigintop;
+    real Rcoef, d0_x, d0_y, xval, yval, abscissa_in_PML, abscissa_normalized;
+    int i, h;
+
+    thickness_PML_x = FW * DH;
+    thickness_PML_y = FW * DH;
+    Rcoef = 0.001;
+    d0_x = -(npower + 1) * DAMPING * log(Rcoef) / (2.0 * thickness_PML_x);
+    d0_y = -(npower + 1) * DAMPING * log(Rcoef) / (2.0 * thickness_PML_y);
+
+    xoriginleft = thickness_PML_x;
+    xoriginright = (NXG - 1) * DH - thickness_PML_x;
+
+    for (i = 1; i <= FW; i++) {
+        K_x[i] = 1.0;
+        K_x_half[i] = 1.0;
+        xval = DH * (i - 1);
+        abscissa_in_PML = xoriginleft - xval;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_x;
+            d_x[i] = d0_x * pow(abscissa_normalized, npower);
+            K_x[i] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_x[i] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        abscissa_in_PML = xoriginleft - (xval + DH / 2.0);
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_x;
+            d_x_half[i] = d0_x * pow(abscissa_normalized, npower);
+            K_x_half[i] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_x_half[i] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        if (alpha_prime_x[i] < 0.0) alpha_prime_x[i] = 0.0;
+        if (alpha_prime_x_half[i] < 0.0) alpha_prime_x_half[i] = 0.0;
+        b_x[i] = exp(-(d_x[i] / K_x[i] + alpha_prime_x[i]) * DT);
+        b_x_half[i] = exp(-(d_x_half[i] / K_x_half[i] + alpha_prime_x_half[i]) * DT);
+        if (abs((int)d_x[i]) > 1.0e-6)
+            a_x[i] = d_x[i] * (b_x[i] - 1.0) / (K_x[i] * (d_x[i] + K_x[i] * alpha_prime_x[i]));
+        if (abs((int)d_x_half[i]) > 1.0e-6)
+            a_x_half[i] = d_x_half[i] * (b_x_half[i] - 1.0) /
+                          (K_x_half[i] * (d_x_half[i] + K_x_half[i] * alpha_prime_x_half[i]));
+    }
+
+    for (i = NXG - FW + 1; i <= NXG; i++) {
+        h = i - NXG + 2 * FW;
+        K_x[h] = 1.0;
+        K_x_half[h] = 1.0;
+        xval = DH * (i - 1);
+        abscissa_in_PML = xval - xoriginright;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_x;
+            d_x[h] = d0_x * pow(abscissa_normalized, npower);
+            K_x[h] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_x[h] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        abscissa_in_PML = xval + DH / 2.0 - xoriginright;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_x;
+            d_x_half[h] = d0_x * pow(abscissa_normalized, npower);
+            K_x_half[h] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_x_half[h] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        if (alpha_prime_x[h] < 0.0) alpha_prime_x[h] = 0.0;
+        if (alpha_prime_x_half[h] < 0.0) alpha_prime_x_half[h] = 0.0;
+        b_x[h] = exp(-(d_x[h] / K_x[h] + alpha_prime_x[h]) * DT);
+        b_x_half[h] = exp(-(d_x_half[h] / K_x_half[h] + alpha_prime_x_half[h]) * DT);
+        if (abs((int)d_x[h]) > 1.0e-6)
+            a_x[h] = d_x[h] * (b_x[h] - 1.0) / (K_x[h] * (d_x[h] + K_x[h] * alpha_prime_x[h]));
+        if (abs((int)d_x_half[h]) > 1.0e-6)
+            a_x_half[h] = d_x_half[h] * (b_x_half[h] - 1.0) /
+                          (K_x_half[h] * (d_x_half[h] + K_x_half[h] * alpha_prime_x_half[h]));
+    }
+
+    yoriginbottom = thickness_PML_y;
+    yorigintop = (NYG - 1) * DH - thickness_PML_y;
+
+    for (i = 1; i <= FW; i++) {
+        K_y[i] = 1.0;
+        K_y_half[i] = 1.0;
+        yval = DH * (i - 1);
+        abscissa_in_PML = yoriginbottom - yval;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_y;
+            d_y[i] = d0_y * pow(abscissa_normalized, npower);
+            K_y[i] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_y[i] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        abscissa_in_PML = yoriginbottom - (yval + DH / 2.0);
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_y;
+            d_y_half[i] = d0_y * pow(abscissa_normalized, npower);
+            K_y_half[i] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_y_half[i] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        b_y[i] = exp(-(d_y[i] / K_y[i] + alpha_prime_y[i]) * DT);
+        b_y_half[i] = exp(-(d_y_half[i] / K_y_half[i] + alpha_prime_y_half[i]) * DT);
+        if (abs((int)d_y[i]) > 1.0e-6)
+            a_y[i] = d_y[i] * (b_y[i] - 1.0) / (K_y[i] * (d_y[i] + K_y[i] * alpha_prime_y[i]));
+        if (abs((int)d_y_half[i]) > 1.0e-6)
+            a_y_half[i] = d_y_half[i] * (b_y_half[i] - 1.0) /
+                          (K_y_half[i] * (d_y_half[i] + K_y_half[i] * alpha_prime_y_half[i]));
+    }
+
+    for (i = NYG - FW + 1; i <= NYG; i++) {
+        h = i - NYG + 2 * FW;
+        K_y[h] = 1.0;
+        K_y_half[h] = 1.0;
+        yval = DH * (i - 1);
+        abscissa_in_PML = yval - yorigintop;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_y;
+            d_y[h] = d0_y * pow(abscissa_normalized, npower);
+            K_y[h] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_y[h] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        abscissa_in_PML = yval + DH / 2.0 - yorigintop;
+        if (abscissa_in_PML >= 0.0) {
+            abscissa_normalized = abscissa_in_PML / thickness_PML_y;
+            /* PML_pro.c:281 uses d0_x here; d0_x == d0_y because both strips
+               have the same thickness, so the value is the same */
+            d_y_half[h] = d0_x * pow(abscissa_normalized, npower);
+            K_y_half[h] = 1.0 + (k_max_PML - 1.0) * pow(abscissa_normalized, npower);
+            alpha_prime_y_half[h] = alpha_max_PML * (1.0 - abscissa_normalized) + 0.1 * alpha_max_PML;
+        }
+        b_y[h] = exp(-(d_y[h] / K_y[h] + alpha_prime_y[h]) * DT);
+        b_y_half[h] = exp(-(d_y_half[h] / K_y_half[h] + alpha_prime_y_half[h]) * DT);
+        if (abs((int)d_y[h]) > 1.0e-6)
+            a_y[h] = d_y[h] * (b_y[h] - 1.0) / (K_y[h] * (d_y[h] + K_y[h] * alpha_prime_y[h]));
+        if (abs((int)d_y_half[h]) > 1.0e-6)
+            a_y_half[h] = d_y_half[h] * (b_y_half[h] - 1.0) /
+                          (K_y_half[h] * (d_y_half[h] + K_y_half[h] * alpha_prime_y_half[h]));
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * matcopy.c:12-137 / matcopy_elastic.c with one rank: every neighbour index
+ * wraps to the rank itself (initproc.c:52-55), so the global-edge model halo
+ * becomes a periodic copy, rows first and then columns (Q3).  Pinned by the
+ * test that compares against numpy.pad(mode="wrap") on the one-cell ring.
+ * ---------------------------------------------------------------------- */
+void F(orc_matcopy_wrap)(const orc_cfg *c, real *x)
+{
+    const int NX = c->nx, NY = c->ny;
+    int i, j;
+    for (i = 0; i <= NX + 1; i++) AT(x, c, NY + 1, i) = AT(x, c, 1, i);
+    for (i = 0; i <= NX + 1; i++) AT(x, c, 0, i) = AT(x, c, NY, i);
+    for (j = 0; j <= NY + 1; j++) AT(x, c, j, NX + 1) = AT(x, c, j, 1);
+    for (j = 0; j <= NY + 1; j++) AT(x, c, j, 0) = AT(x, c, j, NX);
+}
+
+/* ------------------------------------------------------------------------
+ * av_mue.c:28-51 (INVMAT1 == 1): mu = rho*vs*vs and harmonic averages to the
+ * x- and y-half points.  Pinned by a numpy brute force in the tests.
+ * ---------------------------------------------------------------------- */
+void F(orc_av_mue)(const orc_cfg *c, const real *u, const real *rho, real *uip, real *ujp)
+{
+    const int NX = c->nx, NY = c->ny;
+    int i, j;
+    real u1, u2, u3;
+    for (j = 1; j <= NY; j++) {
+        for (i = 1; i <= NX; i++) {
+            u1 = AT(rho, c, j, i) * AT(u, c, j, i) * AT(u, c, j, i);
+            u2 = AT(rho, c, j, i + 1) * AT(u, c, j, i + 1) * AT(u, c, j, i + 1);
+            u3 = AT(rho, c, j + 1, i) * AT(u, c, j + 1, i) * AT(u, c, j + 1, i);
+            AT(uip, c, j, i) = 2.0 / ((1.0 / u1) + (1.0 / u2));
+            AT(ujp, c, j, i) = 2.0 / ((1.0 / u1) + (1.0 / u3));
+            if ((u1 == 0.0) || (u2 == 0.0)) AT(uip, c, j, i) = 0.0;
+            if ((u1 == 0.0) || (u3 == 0.0)) AT(ujp, c, j, i) = 0.0;
+        }
+    }
+}
+
+/* av_tau.c:5-15.  Pinned by a numpy brute force. */
+void F(orc_av_tau)(const orc_cfg *c, const real *taus, real *tausipjp)
+{
+    int i, j;
+    for (j = 1; j <= c->ny; j++)
+        for (i = 1; i <= c->nx; i++)
+            AT(tausipjp, c, j, i) = 0.25 * (AT(taus, c, j, i) + AT(taus, c, j, i + 1) + AT(taus, c, j + 1, i) +
+                                            AT(taus, c, j + 1, i + 1));
+}
+
+/* ------------------------------------------------------------------------
+ * readmod.c:30-34 (eta[l] = DT/pts[l], pts[l] = 1/(2 pi FL[l])) and
+ * prepare_update_s.c:20-62 (INVMAT1 == 1).  Outputs per-cell fipjp, f, and
+ * per-mechanism planes dip[l], d[l] ([l][j][i]); per-mechanism scalars
+ * bip, bjm, cip, cjm (0-based l).  Pinned: with taus == 0 the outputs reduce
+ * to fipjp = uip*DT, f = ujp*DT, dip = d = 0 (checked in the tests), and the
+ * visco update with these reproduces the elastic update (test); the r/q
+ * recursion is checked against its closed-form single-cell decay.
+ * ---------------------------------------------------------------------- */
+void F(orc_prepare_update_s)(const orc_cfg *c, const real *pujp, const real *puip, const real *ptaus,
+                             const real *ptausipjp, real *fipjp, real *f, real *dip, real *d, real *bip, real *bjm,
+                             real *cip, real *cjm)
+{
+    const int NX = c->nx, NY = c->ny, L = c->L;
+    const real DT = (real)c->dt;
+    const size_t plane = F(orc_padded_size)(c);
+    real pts[9], eta[9], etaip[9], etajm[9], FL[9];
+    real sumu, ws, mujp, muip;
+    int i, j, l;
+
+    for (l = 1; l <= L; l++) FL[l] = (real)c->fl[l - 1];
+    for (l = 1; l <= L; l++) {
+        pts[l] = 1.0 / (2.0 * PI * FL[l]);
+        eta[l] = DT / pts[l];
+    }
+    ws = 2.0 * PI * FL[1];
+    sumu = 0.0;
+    for (l = 1; l <= L; l++) sumu = sumu + ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+    for (l = 1; l <= L; l++) {
+        etajm[l] = eta[l];
+        etaip[l] = eta[l];
+    }
+    for (l = 1; l <= L; l++) {
+        bip[l - 1] = 1.0 / (1.0 + (etaip[l] * 0.5));
+        bjm[l - 1] = 1.0 / (1.0 + (etajm[l] * 0.5));
+        cip[l - 1] = 1.0 - (etaip[l] * 0.5);
+        cjm[l - 1] = 1.0 - (etajm[l] * 0.5);
+    }
+    for (j = 1; j <= NY; j++) {
+        for (i = 1; i <= NX; i++) {
+            mujp = AT(pujp, c, j, i) / (1.0 + sumu * AT(ptaus, c, j, i));
+            muip = AT(puip, c, j, i) / (1.0 + sumu * AT(ptausipjp, c, j, i));
+            AT(fipjp, c, j, i) = muip * DT * (1.0 + L * AT(ptausipjp, c, j, i));
+            AT(f, c, j, i) = mujp * DT * (1.0 + L * AT(ptaus, c, j, i));
+            for (l = 1; l <= L; l++) {
+                AT(dip + (size_t)(l - 1) * plane, c, j, i) = muip * etaip[l] * AT(ptausipjp, c, j, i);
+                AT(d + (size_t)(l - 1) * plane, c, j, i) = mujp * etajm[l] * AT(ptaus, c, j, i);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * checkfd_ssg_visc.c:14-126 / checkfd_ssg_elastic.c: stability limit
+ * dtstab = DH/(sqrt(2)*gamma*cmax).  ts = TS (source duration).  Pinned by
+ * the closed form for a homogeneous elastic model (cmax = vs).
+ * ---------------------------------------------------------------------- */
+double F(orc_dtstab)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, double TS)
+{
+    const int NX = c->nx, NY = c->ny, L = c->L;
+    const real DT = (real)c->dt, DH = (real)c->dh;
+    real hc[7], pts, eta[9], ws, sumu, ts, c1, cmax = 0.0, gamma, dtstab, taus, pu_ref;
+    int i, j, l;
+    F(orc_holberg)(c->fdorder, c->maxrelerror, hc);
+    ws = 2.0 * PI / (real)TS;
+    for (l = 1; l <= L; l++) {
+        pts = 1.0 / (2.0 * PI * (real)c->fl[l - 1]);
+        eta[l] = DT / pts;
+    }
+    for (i = 1; i <= NX; i++) {
+        for (j = 1; j <= NY; j++) {
+            size_t k = (size_t)(j - 1) * NX + (size_t)(i - 1);
+            real rv = rho[k], uv = vs[k];
+            sumu = 0.0;
+            taus = (L > 0) ? (real)(2.0 / qs[k]) : (real)0.0;
+            for (l = 1; l <= L; l++) {
+                ts = DT / eta[l];
+                sumu = sumu + ((ws * ws * ts * ts * taus) / (1.0 + ws * ws * ts * ts));
+            }
+            pu_ref = rv * uv * uv;
+            if (L > 0)
+                c1 = sqrt(pu_ref * (1.0 + L * taus) / (rv * (1.0 + sumu)));
+            else
+                c1 = uv; /* checkfd_ssg_elastic.c: c = pu for INVMAT1 == 1 */
+            if (cmax < c1) cmax = c1;
+        }
+    }
+    gamma = fabs(hc[1]) + fabs(hc[2]) + fabs(hc[3]) + fabs(hc[4]) + fabs(hc[5]) + fabs(hc[6]);
+    dtstab = DH / (sqrt(2) * gamma * cmax);
+    return (double)dtstab;
+}
+
+/* ------------------------------------------------------------------------
+ * wavelet.c:12-85 for QUELLART 1 (DENISE's Ricker, Q9), 2 (Fuchs-Mueller),
+ * 4 (sin^3), 5 (Gaussian derivative) and 6 (spike).  signals[nt-1] holds
+ * sample nt = 1..NT.  Pinned by the closed forms in the tests.
+ * ---------------------------------------------------------------------- */
+int F(orc_wavelet)(int quellart, int NT, double dt_in, double tshift_in, double fc_in, double amp_in, real *out)
+{
+    const real DT = (real)dt_in, tshift = (real)tshift_in, fc = (real)fc_in, a = (real)amp_in;
+    real t, ts, ag, tau, amp = 0.0;
+    int nt;
+    for (nt = 1; nt <= NT; nt++) {
+        t = (real)nt * DT;
+        ts = 1.0 / fc;
+        ag = PI * PI * fc * fc;
+        switch (quellart) {
+        case 1:
+            tau = PI * (t - 1.5 * ts - tshift) / (1.5 * ts);
+            amp = (((1.0 - 4.0 * tau * tau) * exp(-2.0 * tau * tau)));
+            break;
+        case 2:
+            if ((t < tshift) || (t > (tshift + ts)))
+                amp = 0.0;
+            else
+                amp = ((sin(2.0 * PI * (t - tshift) * fc) - 0.5 * sin(4.0 * PI * (t - tshift) * fc)));
+            break;
+        case 4:
+            if ((t < tshift) || (t > (tshift + ts)))
+                amp = 0.0;
+            else
+                amp = pow(sin(PI * (t + tshift) / ts), 3.0);
+            break;
+        case 5:
+            ts = 1.2 / fc;
+            ag = PI * PI * fc * fc;
+            amp = -2.0 * ag * (t - ts) * exp(-ag * (t - ts) * (t - ts));
+            break;
+        case 6:
+            amp = 0.0;
+            if (nt == 1 + iround(tshift / DT)) amp = 1.0;
+            break;
+        default:
+            return -1;
+        }
+        out[nt - 1] = amp * a;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Butterworth filter: libcseife cseife.c:147-177 (framework), cseife_rfk.c
+ * (coefficients), cseife_rekfl.c (recursion), FP64.  orc_seife_butterworth
+ * filters x[0..n-1] in place (lowpass: method 1, highpass: method 2).
+ * Pinned bit-for-bit against libcseife compiled from the reference sources
+ * into oracle/_ref, and against scipy.signal.bilinear + lfilter.
+ * ---------------------------------------------------------------------- */
+static void orc_rekfl(double *x, int n, double f0, double f1, double f2, double g1, double g2)
+{
+    double xa = 0.0, xaa = 0.0, ya = 0.0, yaa = 0.0, xn, y;
+    int j;
+    for (j = 0; j < n; j++) {
+        xn = x[j];
+        y = f0 * xn + f1 * xa + f2 * xaa + g1 * ya + g2 * yaa;
+        x[j] = y;
+        xaa = xa;
+        xa = xn;
+        yaa = ya;
+        ya = y;
+    }
+}
+
+void F(orc_seife_butterworth)(double *x, int n, double dt, double t0, int m, int method)
+{
+    double t0dt = t0 / dt, zpi = 8.0 * atan(1.0), eps = zpi / t0dt, pih = 2.0 * atan(1.0);
+    int mm = m / 2, j, i;
+    if (method == 2) {
+        double x0 = x[0];
+        for (i = 0; i < n; ++i) x[i] -= x0;
+    }
+    if (m > 2 * mm) {
+        double g1 = (2.0 - eps) / (2.0 + eps), f0, f1;
+        if (method == 1) {
+            f0 = eps / (2.0 + eps);
+            f1 = f0;
+        } else {
+            f0 = 2.0 / (2.0 + eps);
+            f1 = -f0;
+        }
+        orc_rekfl(x, n, f0, f1, 0.0, g1, 0.0);
+    }
+    for (j = 1; j <= mm; ++j) {
+        double h = sin(pih * (2 * j - 1) / m);
+        double epsq = eps * eps;
+        double a = 1.0 - eps * h + epsq / 4.0;
+        double b = -2.0 + epsq / 2.0;
+        double cc = 1.0 + eps * h + epsq / 4.0;
+        double g1 = -b / cc, g2 = -a / cc, f0, f1, f2;
+        if (method == 1) {
+            f0 = epsq / 4.0 / cc;
+            f1 = f0 + f0;
+            f2 = f0;
+        } else {
+            f0 = 1.0 / cc;
+            f1 = -f0 - f0;
+            f2 = f0;
+        }
+        orc_rekfl(x, n, f0, f1, f2, g1, g2);
+    }
+}
+
+/* timedomain_filt.c:10-53 including Q8: the 1-based trace is handed to the
+ * 0-based filter, so the filter sees [0, s1, ..., s_{ns-1}] (the zeroed
+ * allocation cell util.c:143-146 first) and s_ns stays unfiltered.
+ * data is [ntr][ns] float, in place. */
+void F(orc_timedomain_filt)(float *data, int ntr, int ns, double dt_in, double fc_in, int order, int method)
+{
+    const float DT = (float)dt_in, fc = (float)fc_in;
+    double *seis = (double *)calloc((size_t)ns + 1, sizeof(double));
+    double T0 = 1.0 / fc;
+    int itr, j;
+    for (itr = 0; itr < ntr; itr++) {
+        float *row = data + (size_t)itr * ns;
+        seis[0] = 0.0;
+        for (j = 1; j <= ns; j++) seis[j] = (double)row[j - 1];
+        F(orc_seife_butterworth)(seis, ns, DT, T0, order, method);
+        for (j = 1; j <= ns; j++) row[j - 1] = (float)seis[j];
+    }
+    free(seis);
+}
+
+/* ------------------------------------------------------------------------
+ * update_v_PML.c:12-583 (FDORDER cases 2..12), one sweep over the interior.
+ * sw = 0 forward (vyp1 = acceleration), sw = 1 adjoint (vyp1 = old vy,
+ * GRAD_FORM 2).  Q6: for FDORDER 12 the last syy_y term uses hc[5].
+ * Source injection is separate (orc_inject).  psi_syy_y is [1..2FW][1..NX]
+ * (width NX), psi_sxy_x is [1..NY][1..2FW] (width 2FW); both 1-based and
+ * flattened with (row-1)*width + (col-1).
+ * Pinned by: FP64 energy conservation of the discrete leapfrog scheme (no
+ * CPML, no source), x-mirror symmetry, and the closed-form homogeneous
+ * response in the tests.
+ * ---------------------------------------------------------------------- */
+#define PSI_Y(p, c, row, col) ((p)[(size_t)((row)-1) * (c)->nx + (size_t)((col)-1)])
+#define PSI_X(p, c, row, col) ((p)[(size_t)((row)-1) * 2 * (c)->fw + (size_t)((col)-1)])
+
+void F(orc_update_v)(const orc_cfg *c, int sw, real *vy, real *vyp1, const real *syz, const real *sxy,
+                     const real *rho, const real *hc, const real *K_x, const real *a_x, const real *b_x,
+                     const real *K_y, const real *a_y, const real *b_y, real *psi_syy_y, real *psi_sxy_x)
+{
+    const int NX = c->nx, NY = c->ny, FW = c->fw, k = c->fdorder / 2;
+    const real DT = (real)c->dt, DH = (real)c->dh;
+    int j;
+#pragma omp parallel for schedule(static)
+    for (j = 1; j <= NY; j++) {
+        int i, m, h1;
+        real sxy_x, syy_y, b;
+        for (i = 1; i <= NX; i++) {
+            sxy_x = hc[1] * (AT(sxy, c, j, i) - AT(sxy, c, j, i - 1));
+            syy_y = hc[1] * (AT(syz, c, j, i) - AT(syz, c, j - 1, i));
+            for (m = 2; m <= k; m++) {
+                sxy_x = sxy_x + hc[m] * (AT(sxy, c, j, i + m - 1) - AT(sxy, c, j, i - m));
+                syy_y = syy_y + hc[(k == 6 && m == 6) ? 5 : m] * (AT(syz, c, j + m - 1, i) - AT(syz, c, j - m, i));
+            }
+            if (FW > 0) {
+                if (i <= FW) {
+                    PSI_X(psi_sxy_x, c, j, i) = b_x[i] * PSI_X(psi_sxy_x, c, j, i) + a_x[i] * sxy_x;
+                    sxy_x = sxy_x / K_x[i] + PSI_X(psi_sxy_x, c, j, i);
+                }
+                if (i >= NX - FW + 1) {
+                    h1 = i - NX + 2 * FW;
+                    PSI_X(psi_sxy_x, c, j, h1) = b_x[h1] * PSI_X(psi_sxy_x, c, j, h1) + a_x[h1] * sxy_x;
+                    sxy_x = sxy_x / K_x[h1] + PSI_X(psi_sxy_x, c, j, h1);
+                }
+                if ((!c->free_surf) && (j <= FW)) {
+                    PSI_Y(psi_syy_y, c, j, i) = b_y[j] * PSI_Y(psi_syy_y, c, j, i) + a_y[j] * syy_y;
+                    syy_y = syy_y / K_y[j] + PSI_Y(psi_syy_y, c, j, i);
+                }
+                if (j >= NY - FW + 1) {
+                    h1 = j - NY + 2 * FW;
+                    PSI_Y(psi_syy_y, c, h1, i) = b_y[h1] * PSI_Y(psi_syy_y, c, h1, i) + a_y[h1] * syy_y;
+                    syy_y = syy_y / K_y[h1] + PSI_Y(psi_syy_y, c, h1, i);
+                }
+            }
+            b = 1.0 / AT(rho, c, j, i);
+            if (AT(rho, c, j, i) < 1e-20) b = 0.0;
+            if (sw == 0) AT(vyp1, c, j, i) = b * (sxy_x + syy_y) / DH;
+            if (sw == 1) AT(vyp1, c, j, i) = AT(vy, c, j, i);
+            AT(vy, c, j, i) += b * DT * (sxy_x + syy_y) / DH;
+        }
+    }
+}
+
+/* update_v_PML.c:563-583: raw addition after the sweep (Q10). */
+void F(orc_inject)(const orc_cfg *c, real *vy, int n, const int *ii, const int *jj, const real *val)
+{
+    int l;
+    for (l = 0; l < n; l++) AT(vy, c, jj[l], ii[l]) += val[l];
+}
+
+/* ------------------------------------------------------------------------
+ * update_s_elastic_PML.c:12-446 (cases 2..12): dhi = DT/DH, half-grid CPML
+ * on all four strips, uy/uyx include the DT factor (Q5).  psi_vyy is
+ * [1..2FW][1..NX], psi_vyx is [1..NY][1..2FW].  uy/uyx may be NULL.
+ * Pinned by energy conservation (with orc_update_v) and by the visco kernel
+ * with taus = 0 (test).
+ * ---------------------------------------------------------------------- */
+void F(orc_update_s_elastic)(const orc_cfg *c, const real *vy, real *uy, real *uyx, real *syz, real *sxy,
+                             const real *uip, const real *ujp, const real *hc, const real *K_x_half,
+                             const real *a_x_half, const real *b_x_half, const real *K_y_half, const real *a_y_half,
+                             const real *b_y_half, real *psi_vyy, real *psi_vyx)
+{
+    const int NX = c->nx, NY = c->ny, FW = c->fw, k = c->fdorder / 2;
+    const real DT = (real)c->dt, DH = (real)c->dh;
+    real dhi = DT / DH;
+    int j;
+#pragma omp parallel for schedule(static)
+    for (j = 1; j <= NY; j++) {
+        int i, m, h1;
+        real vyx, vyy;
+        for (i = 1; i <= NX; i++) {
+            vyx = hc[1] * (AT(vy, c, j, i + 1) - AT(vy, c, j, i));
+            vyy = hc[1] * (AT(vy, c, j + 1, i) - AT(vy, c, j, i));
+            for (m = 2; m <= k; m++) {
+                vyx = vyx + hc[m] * (AT(vy, c, j, i + m) - AT(vy, c, j, i - m + 1));
+                vyy = vyy + hc[m] * (AT(vy, c, j + m, i) - AT(vy, c, j - m + 1, i));
+            }
+            vyx = vyx * dhi;
+            vyy = vyy * dhi;
+            if (FW > 0) {
+                if (i <= FW) {
+                    PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
+                    vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
+                }
+                if (i >= NX - FW + 1) {
+                    h1 = i - NX + 2 * FW;
+                    PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
+                    vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
+                }
+                if ((!c->free_surf) && (j <= FW)) {
+                    PSI_Y(psi_vyy, c, j, i) = b_y_half[j] * PSI_Y(psi_vyy, c, j, i) + a_y_half[j] * vyy;
+                    vyy = vyy / K_y_half[j] + PSI_Y(psi_vyy, c, j, i);
+                }
+                if (j >= NY - FW + 1) {
+                    h1 = j - NY + 2 * FW;
+                    PSI_Y(psi_vyy, c, h1, i) = b_y_half[h1] * PSI_Y(psi_vyy, c, h1, i) + a_y_half[h1] * vyy;
+                    vyy = vyy / K_y_half[h1] + PSI_Y(psi_vyy, c, h1, i);
+                }
+            }
+            if (uy) AT(uy, c, j, i) = vyy;
+            if (uyx) AT(uyx, c, j, i) = vyx;
+            AT(sxy, c, j, i) += AT(uip, c, j, i) * vyx;
+            AT(syz, c, j, i) += AT(ujp, c, j, i) * vyy;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * update_s_visc_PML.c:12-577 (cases 2..12): dhi = 1/DH, dthalbe = DT/2,
+ * x CPML half-grid, top CPML full-grid (Q2), bottom half-grid; memory
+ * variables r (x) and q (y) stored [l][j][i]; uy/uyx without DT (Q5).
+ * dip, d are [l][j][i] planes; bip, bjm, cip, cjm are 0-based [l].
+ * Pinned by: taus = 0 reduces to the elastic update (to rounding), and the
+ * single-cell memory-variable recursion matches its closed form (tests).
+ * ---------------------------------------------------------------------- */
+void F(orc_update_s_visc)(const orc_cfg *c, const real *vy, real *uy, real *uyx, real *syz, real *sxy, real *r,
+                          real *q, const real *fipjp, const real *f, const real *dip, const real *d,
+                          const real *bip, const real *bjm, const real *cip, const real *cjm, const real *hc,
+                          const real *K_x_half, const real *a_x_half, const real *b_x_half, const real *K_y,
+                          const real *a_y, const real *b_y, const real *K_y_half, const real *a_y_half,
+                          const real *b_y_half, real *psi_vyy, real *psi_vyx)
+{
+    const int NX = c->nx, NY = c->ny, FW = c->fw, k = c->fdorder / 2, L = c->L;
+    const real DT = (real)c->dt, DH = (real)c->dh;
+    const size_t plane = F(orc_padded_size)(c);
+    real dhi = 1.0 / DH, dthalbe = DT / 2.0;
+    int j;
+#pragma omp parallel for schedule(static)
+    for (j = 1; j <= NY; j++) {
+        int i, m, h1, l;
+        real vyx, vyy, sumr, sumq;
+        for (i = 1; i <= NX; i++) {
+            vyx = hc[1] * (AT(vy, c, j, i + 1) - AT(vy, c, j, i));
+            vyy = hc[1] * (AT(vy, c, j + 1, i) - AT(vy, c, j, i));
+            for (m = 2; m <= k; m++) {
+                vyx = vyx + hc[m] * (AT(vy, c, j, i + m) - AT(vy, c, j, i - m + 1));
+                vyy = vyy + hc[m] * (AT(vy, c, j + m, i) - AT(vy, c, j - m + 1, i));
+            }
+            vyx = vyx * dhi;
+            vyy = vyy * dhi;
+            if (FW > 0) {
+                if (i <= FW) {
+                    PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
+                    vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
+                }
+                if (i >= NX - FW + 1) {
+                    h1 = i - NX + 2 * FW;
+                    PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
+                    vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
+                }
+                if ((!c->free_surf) && (j <= FW)) {
+                    PSI_Y(psi_vyy, c, j, i) = b_y[j] * PSI_Y(psi_vyy, c, j, i) + a_y[j] * vyy;
+                    vyy = vyy / K_y[j] + PSI_Y(psi_vyy, c, j, i);
+                }
+                if (j >= NY - FW + 1) {
+                    h1 = j - NY + 2 * FW;
+                    PSI_Y(psi_vyy, c, h1, i) = b_y_half[h1] * PSI_Y(psi_vyy, c, h1, i) + a_y_half[h1] * vyy;
+                    vyy = vyy / K_y_half[h1] + PSI_Y(psi_vyy, c, h1, i);
+                }
+            }
+            sumr = sumq = 0.0;
+            for (l = 0; l < L; l++) {
+                sumr += AT(r + (size_t)l * plane, c, j, i);
+                sumq += AT(q + (size_t)l * plane, c, j, i);
+            }
+            AT(sxy, c, j, i) += (AT(fipjp, c, j, i) * vyx) + (dthalbe * sumr);
+            AT(syz, c, j, i) += (AT(f, c, j, i) * vyy) + (dthalbe * sumq);
+            sumr = sumq = 0.0;
+            for (l = 0; l < L; l++) {
+                real *rl = r + (size_t)l * plane, *ql = q + (size_t)l * plane;
+                AT(rl, c, j, i) = bip[l] * (AT(rl, c, j, i) * cip[l] - (AT(dip + (size_t)l * plane, c, j, i) * vyx));
+                AT(ql, c, j, i) = bjm[l] * (AT(ql, c, j, i) * cjm[l] - (AT(d + (size_t)l * plane, c, j, i) * vyy));
+                sumr += AT(rl, c, j, i);
+                sumq += AT(ql, c, j, i);
+            }
+            if (uy) AT(uy, c, j, i) = vyy;
+            if (uyx) AT(uyx, c, j, i) = vyx;
+            AT(sxy, c, j, i) += (dthalbe * sumr);
+            AT(syz, c, j, i) += (dthalbe * sumq);
+        }
+    }
+}
+
+/* ========================================================================
+ * Composed shot simulation (denise.c:1079-1346 forward, 1452-1651 adjoint,
+ * 1665-1694 per-shot preconditioning, 1743-1813 chain rule).
+ * ====================================================================== */
+
+typedef struct {
+    const orc_cfg *c;
+    real hc[7];
+    real *pml[20]; /* d,K,alpha,a,b for x, x_half, y, y_half */
+    real *vs, *rho, *taus, *uip, *ujp, *tausipjp, *fipjp, *f, *dip, *d;
+    real bip[8], bjm[8], cip[8], cjm[8];
+    /* wavefield state */
+    real *vy, *sxy, *syz, *vyp1, *uy, *uyx, *r, *q;
+    real *psi_syy_y, *psi_sxy_x, *psi_vyy, *psi_vyx;
+} orc_sim;
+
+#define PML_KX(s) ((s)->pml[1])
+#define PML_AX(s) ((s)->pml[3])
+#define PML_BX(s) ((s)->pml[4])
+#define PML_KXH(s) ((s)->pml[6])
+#define PML_AXH(s) ((s)->pml[8])
+#define PML_BXH(s) ((s)->pml[9])
+#define PML_KY(s) ((s)->pml[11])
+#define PML_AY(s) ((s)->pml[13])
+#define PML_BY(s) ((s)->pml[14])
+#define PML_KYH(s) ((s)->pml[16])
+#define PML_AYH(s) ((s)->pml[18])
+#define PML_BYH(s) ((s)->pml[19])
+
+static real *zalloc(size_t n) { return (real *)calloc(n ? n : 1, sizeof(real)); }
+
+/* model setup: readmod.c:82-104 (taus = 2/qs), matcopy, av_mue, av_tau,
+ * prepare_update_s, PML_pro, holbergcoeff.  vs/rho/qs are [NY][NX] row-major
+ * float (qs may be NULL for L == 0). */
+static int sim_init(orc_sim *s, const orc_cfg *c, const float *vs, const float *rho, const float *qs)
+{
+    const size_t P = F(orc_padded_size)(c);
+    const int NX = c->nx, NY = c->ny, L = c->L, FW = c->fw;
+    int i, j, t;
+    memset(s, 0, sizeof(*s));
+    s->c = c;
+    if (F(orc_holberg)(c->fdorder, c->maxrelerror, s->hc)) return -1;
+    for (t = 0; t < 20; t++) s->pml[t] = zalloc((size_t)2 * FW + 2);
+    if (FW > 0)
+        F(orc_pml_pro)(c, s->pml[0], s->pml[1], s->pml[2], s->pml[3], s->pml[4], s->pml[5], s->pml[6], s->pml[7],
+                       s->pml[8], s->pml[9], s->pml[10], s->pml[11], s->pml[12], s->pml[13], s->pml[14],
+                       s->pml[15], s->pml[16], s->pml[17], s->pml[18], s->pml[19]);
+    s->vs = zalloc(P);
+    s->rho = zalloc(P);
+    s->taus = zalloc(P);
+    s->uip = zalloc(P);
+    s->ujp = zalloc(P);
+    for (j = 1; j <= NY; j++)
+        for (i = 1; i <= NX; i++) {
+            size_t k = (size_t)(j - 1) * NX + (size_t)(i - 1);
+            AT(s->vs, c, j, i) = vs[k];
+            AT(s->rho, c, j, i) = rho[k];
+            if (L > 0) AT(s->taus, c, j, i) = 2.0 / qs[k];
+        }
+    F(orc_matcopy_wrap)(c, s->rho);
+    F(orc_matcopy_wrap)(c, s->vs);
+    F(orc_matcopy_wrap)(c, s->taus);
+    F(orc_av_mue)(c, s->vs, s->rho, s->uip, s->ujp);
+    if (L > 0) {
+        s->tausipjp = zalloc(P);
+        s->fipjp = zalloc(P);
+        s->f = zalloc(P);
+        s->dip = zalloc(P * (size_t)L);
+        s->d = zalloc(P * (size_t)L);
+        F(orc_av_tau)(c, s->taus, s->tausipjp);
+        F(orc_prepare_update_s)(c, s->ujp, s->uip, s->taus, s->tausipjp, s->fipjp, s->f, s->dip, s->d, s->bip,
+                                s->bjm, s->cip, s->cjm);
+    }
+    s->vy = zalloc(P);
+    s->sxy = zalloc(P);
+    s->syz = zalloc(P);
+    s->vyp1 = zalloc(P);
+    s->uy = zalloc(P);
+    s->uyx = zalloc(P);
+    if (L > 0) {
+        s->r = zalloc(P * (size_t)L);
+        s->q = zalloc(P * (size_t)L);
+    }
+    s->psi_syy_y = zalloc((size_t)2 * FW * NX);
+    s->psi_vyy = zalloc((size_t)2 * FW * NX);
+    s->psi_sxy_x = zalloc((size_t)2 * FW * NY);
+    s->psi_vyx = zalloc((size_t)2 * FW * NY);
+    return 0;
+}
+
+static void sim_free(orc_sim *s)
+{
+    int t;
+    for (t = 0; t < 20; t++) free(s->pml[t]);
+    free(s->vs); free(s->rho); free(s->taus); free(s->uip); free(s->ujp);
+    free(s->tausipjp); free(s->fipjp); free(s->f); free(s->dip); free(s->d);
+    free(s->vy); free(s->sxy); free(s->syz); free(s->vyp1); free(s->uy); free(s->uyx);
+    free(s->r); free(s->q);
+    free(s->psi_syy_y); free(s->psi_vyy); free(s->psi_sxy_x); free(s->psi_vyx);
+}
+
+/* zero_fdveps.c:11-52 / zero_fdveps_visc.c:11-64 */
+static void sim_zero(orc_sim *s)
+{
+    const orc_cfg *c = s->c;
+    const size_t P = F(orc_padded_size)(c);
+    const int FW = c->fw, L = c->L;
+    memset(s->vy, 0, P * sizeof(real));
+    memset(s->sxy, 0, P * sizeof(real));
+    memset(s->syz, 0, P * sizeof(real));
+    memset(s->vyp1, 0, P * sizeof(real));
+    memset(s->uy, 0, P * sizeof(real));
+    memset(s->uyx, 0, P * sizeof(real));
+    if (L > 0) {
+        memset(s->r, 0, P * (size_t)L * sizeof(real));
+        memset(s->q, 0, P * (size_t)L * sizeof(real));
+    }
+    memset(s->psi_syy_y, 0, (size_t)2 * FW * c->nx * sizeof(real));
+    memset(s->psi_vyy, 0, (size_t)2 * FW * c->nx * sizeof(real));
+    memset(s->psi_sxy_x, 0, (size_t)2 * FW * c->ny * sizeof(real));
+    memset(s->psi_vyx, 0, (size_t)2 * FW * c->ny * sizeof(real));
+}
+
+static void sim_step(orc_sim *s, int sw, int ninj, const int *ii, const int *jj, const real *val)
+{
+    const orc_cfg *c = s->c;
+    F(orc_update_v)(c, sw, s->vy, s->vyp1, s->syz, s->sxy, s->rho, s->hc, PML_KX(s), PML_AX(s), PML_BX(s),
+                    PML_KY(s), PML_AY(s), PML_BY(s), s->psi_syy_y, s->psi_sxy_x);
+    F(orc_inject)(c, s->vy, ninj, ii, jj, val);
+    if (c->L > 0)
+        F(orc_update_s_visc)(c, s->vy, s->uy, s->uyx, s->syz, s->sxy, s->r, s->q, s->fipjp, s->f, s->dip, s->d,
+                             s->bip, s->bjm, s->cip, s->cjm, s->hc, PML_KXH(s), PML_AXH(s), PML_BXH(s), PML_KY(s),
+                             PML_AY(s), PML_BY(s), PML_KYH(s), PML_AYH(s), PML_BYH(s), s->psi_vyy, s->psi_vyx);
+    else
+        F(orc_update_s_elastic)(c, s->vy, s->uy, s->uyx, s->syz, s->sxy, s->uip, s->ujp, s->hc, PML_KXH(s),
+                                PML_AXH(s), PML_BXH(s), PML_KYH(s), PML_AYH(s), PML_BYH(s), s->psi_vyy, s->psi_vyx);
+}
+
+/* number of stored steps, denise.c:287 */
+int F(orc_ntdtinv)(int nt, int dtinv) { return (int)ceil((float)nt / (float)dtinv); }
+
+/* ------------------------------------------------------------------------
+ * Forward modelling of one shot: denise.c:1125-1346.  Sources are nsrc
+ * points (grid indices, 1-based) with signals [nsrc][NT]; receivers are ntr
+ * points.  seis_out [ntr][NT] (REAL).  If snaps != NULL it receives the
+ * stored snapshots [NTDTINV][3][NY][NX] (order vyp1, uy, uyx) and Ws
+ * (padded, accumulated) the source-side energy.  Returns 0, or 1 if the
+ * centre-cell NaN probe (denise.c:1183-1185) fired.
+ * ---------------------------------------------------------------------- */
+int F(orc_forward)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nsrc, const int *src_i,
+                   const int *src_j, const float *signals, int ntr, const int *rec_i, const int *rec_j,
+                   real *seis_out, real *snaps, real *Ws)
+{
+    orc_sim s;
+    const int NT = c->nt, NX = c->nx, NY = c->ny;
+    real *val = zalloc((size_t)nsrc);
+    int nt, hin1 = 1, kstore = 0, l, i, j, rc = 0;
+    if (sim_init(&s, c, vs, rho, qs)) { free(val); return -1; }
+    sim_zero(&s);
+    for (nt = 1; nt <= NT; nt++) {
+        if (isnan(AT(s.vy, c, NY / 2, NX / 2))) { rc = 1; break; }
+        for (l = 0; l < nsrc; l++) val[l] = (real)signals[(size_t)l * NT + (size_t)(nt - 1)];
+        sim_step(&s, 0, nsrc, src_i, src_j, val);
+        for (l = 0; l < ntr; l++) seis_out[(size_t)l * NT + (size_t)(nt - 1)] = AT(s.vy, c, rec_j[l], rec_i[l]);
+        if (nt == hin1) {
+            if (snaps) {
+                real *base = snaps + (size_t)kstore * 3 * NX * NY;
+                for (j = 1; j <= NY; j++)
+                    for (i = 1; i <= NX; i++) {
+                        size_t o = (size_t)(j - 1) * NX + (size_t)(i - 1);
+                        base[o] = AT(s.vyp1, c, j, i);
+                        base[(size_t)NX * NY + o] = AT(s.uy, c, j, i);
+                        base[(size_t)2 * NX * NY + o] = AT(s.uyx, c, j, i);
+                    }
+            }
+            if (Ws) /* eprecond.c:8-14 */
+                for (j = 1; j <= NY; j++)
+                    for (i = 1; i <= NX; i++) AT(Ws, c, j, i) += AT(s.vy, c, j, i) * AT(s.vy, c, j, i);
+            kstore++;
+            hin1 += c->dtinv;
+        }
+    }
+    sim_free(&s);
+    free(val);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * calc_res.c:150-257 (LNORM 2, GRAD_FORM 2, RTM 0) and calc_energy.c:53-61.
+ * obs is modified in place (obs[i][1] := syn[i][1], Q7).  diff [ntr][ns] is
+ * time-reversed.  l2 and energy are accumulated in double here; the
+ * reference accumulates in float (noted where compared).
+ * ---------------------------------------------------------------------- */
+void F(orc_residual)(int ntr, int ns, double dt_in, const real *syn, real *obs, real *diff, double *l2,
+                     double *energy)
+{
+    const real DT = (real)dt_in;
+    int i, j, invtime;
+    double L2 = 0.0, en = 0.0;
+    for (i = 0; i < ntr; i++) {
+        const real *sy = syn + (size_t)i * ns;
+        real *ob = obs + (size_t)i * ns, *df = diff + (size_t)i * ns;
+        real intseis;
+        invtime = ns;
+        for (j = 1; j <= ns; j++) {
+            if (j == 1) ob[0] = sy[0];
+            intseis = sy[j - 1] - ob[j - 1];
+            df[invtime - 1] = intseis;
+            invtime--;
+        }
+        for (j = 0; j < ns; j++) L2 += (double)df[j] * (double)df[j];
+        intseis = 0.0;
+        for (j = 0; j < ns; j++) {
+            intseis += ob[j];
+            en += (double)(intseis * intseis * DT * DT);
+        }
+    }
+    *l2 += L2;
+    *energy += en;
+}
+
+/* ------------------------------------------------------------------------
+ * Adjoint (back-propagation) of one shot with the zero-lag correlation:
+ * denise.c:1452-1651.  Sources are at the receivers with the time-reversed
+ * residual diff [ntr][NT].  snaps as written by orc_forward.  grad_u,
+ * grad_rho, Wr are padded fields accumulated in place.
+ * ---------------------------------------------------------------------- */
+int F(orc_adjoint)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int ntr, const int *rec_i,
+                   const int *rec_j, const real *diff, const real *snaps, real *grad_u, real *grad_rho, real *Wr)
+{
+    orc_sim s;
+    const int NT = c->nt, NX = c->nx, NY = c->ny, dtinv = c->dtinv;
+    const int NTDTINV = F(orc_ntdtinv)(NT, dtinv);
+    real *val = zalloc((size_t)ntr);
+    int nt, hin = 1, l, i, j, rc = 0;
+    if (sim_init(&s, c, vs, rho, qs)) { free(val); return -1; }
+    sim_zero(&s);
+    for (nt = 1; nt <= NT; nt++) {
+        if (isnan(AT(s.vy, c, NY / 2, NX / 2))) { rc = 1; break; }
+        for (l = 0; l < ntr; l++) val[l] = diff[(size_t)l * NT + (size_t)(nt - 1)];
+        sim_step(&s, 1, ntr, rec_i, rec_j, val);
+        /* DTINV_help[NT-nt+1]: forward step NT-nt+1 was stored */
+        if (((NT - nt + 1) - 1) % dtinv == 0) {
+            const real *base = snaps + (size_t)(NTDTINV - hin) * 3 * NX * NY;
+            for (j = 1; j <= NY; j++)
+                for (i = 1; i <= NX; i++) {
+                    size_t o = (size_t)(j - 1) * NX + (size_t)(i - 1);
+                    AT(grad_rho, c, j, i) += (AT(s.vyp1, c, j, i) * base[o]);
+                    AT(grad_u, c, j, i) += (base[(size_t)2 * NX * NY + o] * AT(s.sxy, c, j, i)) +
+                                           (base[(size_t)NX * NY + o] * AT(s.syz, c, j, i));
+                }
+            for (j = 1; j <= NY; j++)
+                for (i = 1; i <= NX; i++) AT(Wr, c, j, i) += AT(s.vy, c, j, i) * AT(s.vy, c, j, i);
+            hin++;
+        }
+    }
+    sim_free(&s);
+    free(val);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * Full gradient over nshots (denise.c:746-816, 875-1694, 1743-1813):
+ *   C_vs, C_rho = float means of the model (i outer, j inner, Q14);
+ *   per shot: forward, residual, adjoint, eprecond1 (Q16), scaling;
+ *   chain rule to vs and rho (INVMAT1 = 1).
+ * Shots: src_i/src_j [nshots] (one source each), signals [nshots][NT];
+ * obs [nshots][ntr][NT] float (copied, then modified as calc_res does).
+ * Outputs (all [NY][NX] row-major REAL): grad_vs, grad_rho; optional
+ * We_last (the last shot's energy weights).  l2/energy in double.
+ * ---------------------------------------------------------------------- */
+int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
+                    const int *src_i, const int *src_j, const float *signals, int ntr, const int *rec_i,
+                    const int *rec_j, const float *obs, real *grad_vs, real *grad_rho, double *l2, double *energy,
+                    real *waveconv_u_out, real *waveconv_rho_out)
+{
+    const int NT = c->nt, NX = c->nx, NY = c->ny;
+    const size_t P = F(orc_padded_size)(c);
+    const int NTDTINV = F(orc_ntdtinv)(NT, c->dtinv);
+    real *snaps = zalloc((size_t)NTDTINV * 3 * NX * NY);
+    real *seis = zalloc((size_t)ntr * NT), *obsr = zalloc((size_t)ntr * NT), *diff = zalloc((size_t)ntr * NT);
+    real *Ws = zalloc(P), *Wr = zalloc(P), *We = zalloc(P);
+    real *gu_shot = zalloc(P), *grho_shot = zalloc(P), *gu = zalloc(P), *grho = zalloc(P);
+    real C_vs, C_rho, Vs_avg = 0.0, rho_avg = 0.0, maxWe, EPSILON_WE = 0.005, DT = (real)c->dt;
+    int Lcount = 1, ishot, i, j, rc = 0;
+    size_t t;
+
+    for (i = 1; i <= NX; i++)
+        for (j = 1; j <= NY; j++) {
+            Vs_avg += (real)vs[(size_t)(j - 1) * NX + (i - 1)];
+            rho_avg += (real)rho[(size_t)(j - 1) * NX + (i - 1)];
+            Lcount++;
+        }
+    Lcount--;
+    Vs_avg /= Lcount;
+    rho_avg /= Lcount;
+    C_vs = Vs_avg;
+    C_rho = rho_avg;
+    *l2 = 0.0;
+    *energy = 0.0;
+
+    for (ishot = 0; ishot < nshots; ishot++) {
+        memset(Ws, 0, P * sizeof(real));
+        memset(Wr, 0, P * sizeof(real));
+        memset(gu_shot, 0, P * sizeof(real));
+        memset(grho_shot, 0, P * sizeof(real));
+        rc = F(orc_forward)(c, vs, rho, qs, 1, &src_i[ishot], &src_j[ishot], signals + (size_t)ishot * NT, ntr,
+                            rec_i, rec_j, seis, snaps, Ws);
+        if (rc) break;
+        for (t = 0; t < (size_t)ntr * NT; t++) obsr[t] = (real)obs[(size_t)ishot * ntr * NT + t];
+        F(orc_residual)(ntr, NT, c->dt, seis, obsr, diff, l2, energy);
+        rc = F(orc_adjoint)(c, vs, rho, qs, ntr, rec_i, rec_j, diff, snaps, gu_shot, grho_shot, Wr);
+        if (rc) break;
+        /* eprecond1.c:15-70 */
+        maxWe = 0.0;
+        for (i = 1; i <= NX; i++)
+            for (j = 1; j <= NY; j++) {
+                AT(We, c, j, i) = sqrt(AT(Ws, c, j, i) * AT(Wr, c, j, i));
+                if (AT(We, c, j, i) > maxWe) maxWe = AT(We, c, j, i);
+            }
+        for (i = 1; i <= NX; i++)
+            for (j = 1; j <= NY; j++) AT(We, c, j, i) = AT(We, c, j, i) + (EPSILON_WE * maxWe);
+        /* denise.c:1671-1694 */
+        for (i = 1; i <= NX; i++)
+            for (j = 1; j <= NY; j++) {
+                AT(gu_shot, c, j, i) = AT(gu_shot, c, j, i) / (AT(We, c, j, i) * C_vs * C_vs);
+                AT(grho_shot, c, j, i) = AT(grho_shot, c, j, i) / (AT(We, c, j, i) * C_rho * C_rho);
+                AT(grho, c, j, i) += AT(grho_shot, c, j, i);
+                AT(gu, c, j, i) += AT(gu_shot, c, j, i);
+            }
+    }
+    /* denise.c:1756-1811, INVMAT 0, INVMAT1 1 */
+    for (j = 1; j <= NY; j++)
+        for (i = 1; i <= NX; i++) {
+            size_t o = (size_t)(j - 1) * NX + (size_t)(i - 1);
+            real pu = (real)vs[o], pr = (real)rho[o];
+            real gmu = -DT * AT(gu, c, j, i);
+            real grs = -DT * AT(grho, c, j, i);
+            if (waveconv_u_out) waveconv_u_out[o] = AT(gu, c, j, i);
+            if (waveconv_rho_out) waveconv_rho_out[o] = AT(grho, c, j, i);
+            grad_vs[o] = 2.0 * pr * pu * gmu;
+            grad_rho[o] = (pu * pu * gmu) + grs;
+        }
+    free(snaps); free(seis); free(obsr); free(diff); free(Ws); free(Wr); free(We);
+    free(gu_shot); free(grho_shot); free(gu); free(grho);
+    return rc;
+}
+
+/* ========================================================================
+ * Optimiser pieces (host-side in the reference), restated for the tests.
+ * ====================================================================== */
+
+/* solvelin.c:12-61 (Gauss elimination without pivoting, n = 3) and
+ * calc_opt_step.c:13-109.  L2t[1..3], epst[1..3] passed 0-based [0..2]. */
+double F(orc_calc_opt_step)(const real *L2t, const real *epst)
+{
+    real A[4][4], b[4], x[4], a, cc, opteps;
+    int i, k, n, m;
+    const int N = 3;
+    for (i = 1; i <= N; i++) {
+        A[i][3] = (epst[i - 1] * epst[i - 1]);
+        A[i][2] = (epst[i - 1]);
+        A[i][1] = (1.0);
+        b[i] = L2t[i - 1];
+    }
+    for (k = 1; k <= N - 1; k++)
+        for (n = k; n <= N - 1; n++) {
+            a = A[n + 1][k] / A[k][k];
+            for (m = 1; m <= N; m++) A[n + 1][m] = A[n + 1][m] - a * A[k][m];
+            b[n + 1] = b[n + 1] - a * b[k];
+        }
+    for (k = N; k >= 1; k--) {
+        cc = b[k];
+        for (m = N; m >= k + 1; m--) cc = cc - A[k][m] * x[m];
+        x[k] = cc / A[k][k];
+    }
+    opteps = -x[2] / (2.0 * x[3]);
+    if (((2.0 * x[3]) < 0.0) && (L2t[1] > L2t[2])) opteps = epst[2];
+    if (opteps > epst[2]) opteps = epst[2];
+    if (opteps < 0.0) opteps = epst[0];
+    return (double)opteps;
+}
+
+/* LBFGS.c:26-95 two-loop recursion in slot order (Q18).  y, s are
+ * [nslots][n]; q (in, destroyed) and r (out) are [n]; cur is the 1-based
+ * slot holding the newest pair.  Dots in REAL (float in the reference). */
+void F(orc_lbfgs)(int nslots, int n, const real *y, const real *s, real *q, real *r, int cur)
+{
+    real rho[64], alpha[64], gamma, sum_nom, sum_denom, beta;
+    int k, i;
+    sum_nom = sum_denom = 0.0;
+    for (i = 0; i < n; i++) sum_nom += y[(size_t)(cur - 1) * n + i] * s[(size_t)(cur - 1) * n + i];
+    for (i = 0; i < n; i++) sum_denom += y[(size_t)(cur - 1) * n + i] * y[(size_t)(cur - 1) * n + i];
+    gamma = sum_nom / sum_denom;
+    for (k = 1; k <= nslots; k++) {
+        sum_nom = 0.0;
+        for (i = 0; i < n; i++) sum_nom += y[(size_t)(k - 1) * n + i] * s[(size_t)(k - 1) * n + i];
+        rho[k] = (fabs(sum_nom) > 0.0) ? (real)(1.0 / sum_nom) : (real)0.0;
+    }
+    for (k = nslots; k >= 1; k--) {
+        sum_nom = 0.0;
+        for (i = 0; i < n; i++) sum_nom += s[(size_t)(k - 1) * n + i] * q[i];
+        alpha[k] = rho[k] * sum_nom;
+        for (i = 0; i < n; i++) q[i] = q[i] - alpha[k] * y[(size_t)(k - 1) * n + i];
+    }
+    for (i = 0; i < n; i++) r[i] = gamma * q[i];
+    for (k = 1; k <= nslots; k++) {
+        sum_nom = 0.0;
+        for (i = 0; i < n; i++) sum_nom += y[(size_t)(k - 1) * n + i] * r[i];
+        beta = rho[k] * sum_nom;
+        for (i = 0; i < n; i++) r[i] = r[i] + s[(size_t)(k - 1) * n + i] * (alpha[k] - beta);
+    }
+}
+
+/* PCG.c:24-101 (Polak-Ribiere, PCG_BETA 2) for nclass classes of length
+ * n each: dir = new + beta*dir with beta reset to 0 when negative. */
+void F(orc_pcg)(int nclass, int n, const real *gnew, const real *gold, real *dir)
+{
+    int k, i;
+    for (k = 0; k < nclass; k++) {
+        real betaz = 0.0, betan = 0.0, beta = 0.0;
+        const real *pn = gnew + (size_t)k * n, *po = gold + (size_t)k * n;
+        real *pd = dir + (size_t)k * n;
+        for (i = 0; i < n; i++) {
+            betaz += pn[i] * (pn[i] - po[i]);
+            betan += po[i] * po[i];
+        }
+        if (betan != 0.0f) beta = betaz / betan;
+        if (beta < 0.0) beta = 0.0;
+        for (i = 0; i < n; i++) pd[i] = pn[i] + beta * pd[i];
+    }
+}
+
+/* calc_mat_change_test.c:41-191 for INVMAT1 == 1 (Q17): returns
+ * EPSILON_u; writes unp1/rhonp1 ([NY][NX]).  iter/INV_*_ITER gates apply. */
+double F(orc_calc_mat_change)(int nx, int ny, const real *gu, const real *grho, const real *u, const real *rho,
+                              real *unp1, real *rhonp1, double eps_scale_in, int iter, int inv_vs_iter,
+                              int inv_rho_iter, double scalerho_in, double vsl, double vsu, double rhol,
+                              double rhou, real *eps_rho_out)
+{
+    const real eps_scale = (real)eps_scale_in, SCALERHO = (real)scalerho_in;
+    const real VSLOWERLIM = (real)vsl, VSUPPERLIM = (real)vsu, RHOLOWERLIM = (real)rhol, RHOUPPERLIM = (real)rhou;
+    real umax = 0.0, gradmax_u = 0.0, rhomax = 0.0, gradmax_rho = 0.0, EPSILON_u, EPSILON_rho;
+    int i, j;
+#define E(a, j, i) ((a)[(size_t)((j)-1) * nx + (size_t)((i)-1)])
+    for (i = 1; i <= nx; i++)
+        for (j = 1; j <= ny; j++) {
+            if (E(u, j, i) > umax) umax = E(u, j, i);
+            if ((i * j == 1) || (gradmax_u == 0.0))
+                gradmax_u = fabs(E(gu, j, i));
+            else if (fabs(E(gu, j, i)) > gradmax_u)
+                gradmax_u = fabs(E(gu, j, i));
+        }
+    for (i = 1; i <= nx; i++)
+        for (j = 1; j <= ny; j++) {
+            if (E(rho, j, i) > rhomax) rhomax = E(rho, j, i);
+            if ((i * j == 1) || (gradmax_rho == 0.0))
+                gradmax_rho = fabs(E(grho, j, i));
+            else if (fabs(E(grho, j, i)) > gradmax_rho)
+                gradmax_rho = fabs(E(grho, j, i));
+        }
+    EPSILON_u = eps_scale * (umax / gradmax_u);
+    if (iter < inv_vs_iter) EPSILON_u = 0.0;
+    EPSILON_rho = eps_scale * (rhomax / gradmax_rho) * SCALERHO;
+    if (iter < inv_rho_iter) EPSILON_rho = 0.0;
+    for (i = 1; i <= nx; i++)
+        for (j = 1; j <= ny; j++) {
+            E(unp1, j, i) = E(u, j, i) - EPSILON_u * E(gu, j, i);
+            E(rhonp1, j, i) = E(rho, j, i) - EPSILON_rho * E(grho, j, i);
+            if ((E(unp1, j, i) < VSLOWERLIM) && (E(unp1, j, i) > 1e-6)) E(unp1, j, i) = E(u, j, i);
+            if (E(unp1, j, i) > VSUPPERLIM) E(unp1, j, i) = E(u, j, i);
+            if (E(rhonp1, j, i) < RHOLOWERLIM) E(rhonp1, j, i) = E(rho, j, i);
+            if (E(rhonp1, j, i) > RHOUPPERLIM) E(rhonp1, j, i) = E(rho, j, i);
+            if (inv_vs_iter > iter) E(unp1, j, i) = E(u, j, i);
+            if (inv_rho_iter > iter) E(rhonp1, j, i) = E(rho, j, i);
+            if (E(unp1, j, i) < 0.0) E(unp1, j, i) = E(u, j, i);
+            if (E(rhonp1, j, i) < 0.0) E(rhonp1, j, i) = E(rho, j, i);
+        }
+#undef E
+    if (eps_rho_out) *eps_rho_out = EPSILON_rho;
+    return (double)EPSILON_u;
+}

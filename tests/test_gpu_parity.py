@@ -1,0 +1,155 @@
+"""GPU parity of the fused HIP time step against the CPU oracle.
+
+Seismograms and gradients from the HIP path are compared with the oracle's
+FP32 restatement of the reference (bar: seismograms 1e-5, gradients 1e-4
+relative L2; the kernels keep the reference's operation order, so the
+expected difference is zero or a few ulps) and the error against the FP64
+oracle is printed for every case.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from common import model, oracle, product, rel_l2, ricker
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def say(*a):
+    print(*a)
+    sys.stdout.flush()
+
+
+# name, nx, ny, fdorder, L, free_surf, fw, nt, fc
+FWD_CASES = [
+    ("el_fd4_fs", 120, 50, 4, 0, 1, 10, 500, 60.0),
+    ("el_fd2_nofs", 90, 40, 2, 0, 0, 8, 300, 40.0),
+    ("el_fd6_fs", 100, 45, 6, 0, 1, 10, 300, 60.0),
+    ("el_fd8_nofs", 100, 45, 8, 0, 0, 10, 300, 60.0),
+    ("el_fd10_fs", 100, 45, 10, 0, 1, 10, 250, 60.0),
+    ("el_fd12_nofs", 100, 45, 12, 0, 0, 10, 250, 60.0),
+    ("visc_fd8_nofs", 120, 50, 8, 2, 0, 10, 400, 60.0),
+    ("visc_fd4_fs", 110, 48, 4, 1, 1, 10, 300, 60.0),
+    ("visc_fd12_fs", 100, 45, 12, 4, 1, 10, 250, 60.0),
+    ("visc_fd6_nofs_L3", 100, 45, 6, 3, 0, 10, 250, 60.0),
+]
+
+DT = 1e-4
+DH = 0.25
+
+
+def setup_case(nx, ny, fdorder, L, free_surf, fw, nt, fc, nshots=2, dtinv=1, snap_precision=0, seed=3,
+               nts=None):
+    qs = 20.0 if L > 0 else None
+    vs, rho, q = model(nx, ny, DH, seed=seed, qs=qs)
+    fl = [6.25, 50.0, 12.0, 30.0][:L]
+    dn = product()
+    ctx = dn.Context(nx, ny, fdorder=fdorder, maxrelerror=1, fw=fw, free_surf=free_surf, L=L, fl=fl, dt=DT, dh=DH,
+                     damping=300.0, fpml=fc, dtinv=dtinv, snap_precision=snap_precision)
+    ctx.set_model(vs, rho, q)
+    rec_i = np.arange(fw + 3, nx - fw - 2, 3, dtype=np.int32)
+    rec_j = np.full_like(rec_i, 2)
+    ctx.set_receivers(rec_i, rec_j)
+    src_i = np.linspace(fw + 5, nx - fw - 5, nshots).astype(np.int32)
+    src_j = np.full(nshots, 3, np.int32)
+    sig = np.stack([ricker(nt, DT, fc, tshift=0.001 * s) for s in range(nshots)])
+    ntv = None if nts is None else np.asarray(nts, np.int32)
+    ctx.set_shots(src_i, src_j, sig, ntv)
+    from pyoracle import make_cfg
+    cfg = make_cfg(nx, ny, fdorder=fdorder, maxrelerror=1, fw=fw, free_surf=free_surf, L=L, nt=nt, dtinv=dtinv,
+                   dt=DT, dh=DH, damping=300.0, fpml=fc, fl=fl)
+    return ctx, cfg, dict(vs=vs, rho=rho, qs=q, rec_i=rec_i, rec_j=rec_j, src_i=src_i, src_j=src_j, sig=sig)
+
+
+@pytest.mark.parametrize("case", FWD_CASES, ids=[c[0] for c in FWD_CASES])
+def test_forward_seismograms(case):
+    name, nx, ny, fdorder, L, fs, fw, nt, fc = case
+    say("case", name)
+    ctx, cfg, d = setup_case(nx, ny, fdorder, L, fs, fw, nt, fc)
+    seis = ctx.forward()
+    o32, o64 = oracle("f32"), oracle("f64")
+    worst = 0.0
+    for s in range(len(d["src_i"])):
+        ref32, _, _ = o32.forward(cfg, d["vs"], d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1],
+                                  d["sig"][s], d["rec_i"], d["rec_j"])
+        ref64, _, _ = o64.forward(cfg, d["vs"], d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1],
+                                  d["sig"][s], d["rec_i"], d["rec_j"])
+        e32 = rel_l2(seis[s], ref32)
+        e64 = rel_l2(seis[s], ref64)
+        o_err = rel_l2(ref32, ref64)
+        say("  shot %d: rel L2 vs oracle f32 %.3e (max abs %.3e), vs f64 %.3e (oracle f32 vs f64 %.3e)" %
+            (s, e32, np.max(np.abs(seis[s] - ref32)), e64, o_err))
+        assert np.all(np.isfinite(seis[s]))
+        assert np.max(np.abs(ref32)) > 0
+        worst = max(worst, e32)
+    assert worst <= SEIS_TOL
+
+
+def test_uneven_record_lengths_in_one_batch():
+    say("case uneven nt batch")
+    nts = [400, 250, 330]
+    ctx, cfg, d = setup_case(110, 46, 8, 2, 1, 10, 400, 60.0, nshots=3, nts=nts)
+    seis = ctx.forward()
+    o32 = oracle("f32")
+    from pyoracle import make_cfg
+    for s, n in enumerate(nts):
+        c2 = make_cfg(110, 46, fdorder=8, maxrelerror=1, fw=10, free_surf=1, L=2, nt=n, dt=DT, dh=DH, damping=300.0,
+                      fpml=60.0, fl=[6.25, 50.0])
+        ref, _, _ = o32.forward(c2, d["vs"], d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1],
+                                d["sig"][s][:n], d["rec_i"], d["rec_j"])
+        e = rel_l2(seis[s][:, :n], ref)
+        say("  shot %d nt=%d rel L2 %.3e" % (s, n, e))
+        assert e <= SEIS_TOL
+        assert np.all(seis[s][:, n:] == 0.0)
+
+
+def test_batched_equals_sequential():
+    say("case batch vs one-by-one")
+    ctx, cfg, d = setup_case(100, 44, 8, 2, 1, 10, 300, 60.0, nshots=4)
+    a = ctx.forward()
+    ctx.set_batch(1)
+    b = ctx.forward()
+    assert np.array_equal(a, b)
+
+
+GRAD_CASES = [
+    ("el_fd4_dtinv1", 100, 44, 4, 0, 1, 10, 400, 60.0, 1),
+    ("visc_fd8_dtinv3", 100, 44, 8, 2, 1, 10, 400, 60.0, 3),
+    ("visc_fd8_nofs_dtinv4", 96, 40, 8, 2, 0, 10, 350, 60.0, 4),
+]
+
+
+@pytest.mark.parametrize("case", GRAD_CASES, ids=[c[0] for c in GRAD_CASES])
+def test_gradient(case):
+    name, nx, ny, fdorder, L, fs, fw, nt, fc, dtinv = case
+    say("case", name)
+    ctx, cfg, d = setup_case(nx, ny, fdorder, L, fs, fw, nt, fc, nshots=2, dtinv=dtinv)
+    # observed data: the true model has an extra +10% box in vs
+    vt = d["vs"].copy()
+    vt[ny // 3: ny // 2, nx // 3: nx // 2] *= 1.1
+    o32, o64 = oracle("f32"), oracle("f64")
+    obs = np.zeros((2, len(d["rec_i"]), nt), np.float32)
+    for s in range(2):
+        ref, _, _ = o32.forward(cfg, vt, d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1], d["sig"][s],
+                                d["rec_i"], d["rec_j"])
+        obs[s] = ref
+    ctx.set_observed(obs)
+    l2, en = ctx.gradient()
+    gvs, grho, raw_u, raw_rho = ctx.get_gradient()
+    r32 = o32.gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    r64 = o64.gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    ev = rel_l2(gvs, r32["grad_vs"])
+    er = rel_l2(grho, r32["grad_rho"])
+    say("  grad_vs rel L2 vs f32 %.3e vs f64 %.3e (oracle f32 vs f64 %.3e)" %
+        (ev, rel_l2(gvs, r64["grad_vs"]), rel_l2(r32["grad_vs"], r64["grad_vs"])))
+    say("  grad_rho rel L2 vs f32 %.3e vs f64 %.3e (oracle f32 vs f64 %.3e)" %
+        (er, rel_l2(grho, r64["grad_rho"]), rel_l2(r32["grad_rho"], r64["grad_rho"])))
+    say("  l2 %.9e oracle %.9e; energy %.9e oracle %.9e" % (l2, r32["l2"], en, r32["energy"]))
+    assert np.max(np.abs(r32["grad_vs"])) > 0
+    assert ev <= GRAD_TOL and er <= GRAD_TOL
+    assert abs(l2 - r32["l2"]) <= 1e-9 * abs(r32["l2"])
+    assert abs(en - r32["energy"]) <= 1e-9 * abs(r32["energy"])
