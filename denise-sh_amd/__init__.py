@@ -17,7 +17,8 @@ EXPORTS = [
     "dn_set_globals", "update_v_PML", "update_s_elastic_PML", "update_s_visc_PML", "seismo_ssg", "psource",
     "dn_create", "dn_destroy", "dn_last_error", "dn_set_model", "dn_get_model", "dn_set_receivers",
     "dn_set_shots", "dn_set_observed", "dn_set_batch", "dn_forward", "dn_gradient", "dn_get_gradient",
-    "dn_forward_fields", "dn_last_timing",
+    "dn_forward_fields", "dn_last_timing", "dn_holbergcoeff", "dn_pml_pro", "dn_relaxation_constants",
+    "dn_wavelet", "dn_timedomain_filt",
 ]
 
 
@@ -254,3 +255,57 @@ def set_globals(**kw):
     rc = lib().dn_set_globals(C.byref(g))
     if rc != 0:
         raise DeniseError(lib().dn_last_error(None).decode())
+
+
+# ---------------------------------------------------------------------------
+# Host setup routines (no device needed)
+# ---------------------------------------------------------------------------
+PML_NAMES = ("K_x", "a_x", "b_x", "K_x_half", "a_x_half", "b_x_half", "K_y", "a_y", "b_y", "K_y_half", "a_y_half",
+             "b_y_half")
+
+
+def holbergcoeff(fdorder, maxrelerror):
+    hc = np.zeros(7, np.float32)
+    if lib().dn_holbergcoeff(fdorder, maxrelerror, _f(hc)) != 0:
+        raise DeniseError("wrong FDORDER/MAXRELERROR")
+    return hc
+
+
+def pml_pro(nx, ny, fw, dh, dt, damping, fpml, npower=2.0, k_max_pml=1.0):
+    out = np.zeros((12, 2 * fw + 1), np.float32)
+    f = C.c_float
+    lib().dn_pml_pro.argtypes = [C.c_int, C.c_int, C.c_int, f, f, f, f, f, f, C.POINTER(C.c_float)]
+    rc = lib().dn_pml_pro(nx, ny, fw, dh, dt, damping, fpml, npower, k_max_pml, _f(out))
+    if rc != 0:
+        raise DeniseError("dn_pml_pro failed")
+    return dict(zip(PML_NAMES, out))
+
+
+def relaxation_constants(fl, dt):
+    L = len(fl)
+    fl = np.ascontiguousarray(fl, np.float32)
+    eta, bip, bjm, cip, cjm = [np.zeros(L, np.float32) for _ in range(5)]
+    sumu = C.c_float()
+    lib().dn_relaxation_constants.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_float] + [C.POINTER(C.c_float)] * 6
+    rc = lib().dn_relaxation_constants(L, _f(fl), dt, _f(eta), C.byref(sumu), _f(bip), _f(bjm), _f(cip), _f(cjm))
+    if rc != 0:
+        raise DeniseError("dn_relaxation_constants failed")
+    return dict(eta=eta, sumu=np.float32(sumu.value), bip=bip, bjm=bjm, cip=cip, cjm=cjm)
+
+
+def wavelet(quellart, nt, dt, tshift, fc, amp):
+    out = np.zeros(nt, np.float32)
+    f = C.c_float
+    lib().dn_wavelet.argtypes = [C.c_int, C.c_int, f, f, f, f, C.POINTER(C.c_float)]
+    if lib().dn_wavelet(quellart, nt, dt, tshift, fc, amp, _f(out)) != 0:
+        raise DeniseError("unknown source wavelet")
+    return out
+
+
+def timedomain_filt(data, dt, fc, order, method):
+    d = np.ascontiguousarray(data, np.float32).copy()
+    f = C.c_float
+    lib().dn_timedomain_filt.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, f, f, C.c_int, C.c_int]
+    if lib().dn_timedomain_filt(_f(d), d.shape[0], d.shape[1], dt, fc, order, method) != 0:
+        raise DeniseError("dn_timedomain_filt failed")
+    return d

@@ -268,3 +268,44 @@ void timedomain_filt(float *data, int ntr, int ns, float DT, float fc, int order
 }
 
 }  // namespace dn
+
+// ---------------------------------------------------------------------------
+// C-ABI exports of the host setup (include/denise_sh.h)
+// ---------------------------------------------------------------------------
+#include "../../include/denise_sh.h"
+
+extern "C" int dn_holbergcoeff(int fdorder, int maxrelerror, float *hc)
+{
+    return hc ? dn::fd_coefficients(fdorder, maxrelerror, hc) : -1;
+}
+
+extern "C" int dn_pml_pro(int nx, int ny, int fw, float dh, float dt, float damping, float fpml, float npower,
+                          float k_max_pml, float *out)
+{
+    if (!out || fw < 0) return -1;
+    dn::CpmlIn in{nx, ny, fw, dh, dt, damping, fpml, npower, k_max_pml};
+    dn::CpmlOut co;
+    dn::cpml_profiles(in, co);
+    for (int v = 0; v < 12; ++v) std::memcpy(out + (size_t)v * co.n, co.v[v].data(), (size_t)co.n * sizeof(float));
+    return 0;
+}
+
+extern "C" int dn_relaxation_constants(int L, const float *fl, float dt, float *eta, float *sumu, float *bip,
+                                       float *bjm, float *cip, float *cjm)
+{
+    if (L < 1 || L > dn::kSetupMaxL || !fl) return -1;
+    dn::relaxation_constants(L, fl, dt, eta, sumu, bip, bjm, cip, cjm);
+    return 0;
+}
+
+extern "C" int dn_wavelet(int quellart, int nt, float dt, float tshift, float fc, float amp, float *out)
+{
+    return out ? dn::wavelet(quellart, nt, dt, tshift, fc, amp, out) : -1;
+}
+
+extern "C" int dn_timedomain_filt(float *data, int ntr, int ns, float dt, float fc, int order, int method)
+{
+    if (!data || ntr < 0 || ns < 1 || order < 1 || (method != 1 && method != 2)) return -1;
+    dn::timedomain_filt(data, ntr, ns, dt, fc, order, method);
+    return 0;
+}

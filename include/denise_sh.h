@@ -90,6 +90,32 @@ void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvy, float **vy,
 void psource(int nt, float **sxx, float **syy, float **srcpos_loc, float **signals, int nsrc);
 
 /* ------------------------------------------------------------------------
+ * Host-side setup routines of the path (no device needed).  The reference
+ * versions read globals; these take the same quantities as arguments.
+ * ---------------------------------------------------------------------- */
+
+/* holbergcoeff.c:19-90: hc[0..6] for FDORDER and MAXRELERROR. */
+int dn_holbergcoeff(int fdorder, int maxrelerror, float *hc);
+
+/* PML_pro.c:65-297: twelve 1-based vectors of length 2*fw+1 written to
+ * out[12][2*fw+1] in the order K_x, a_x, b_x, K_x_half, a_x_half, b_x_half,
+ * K_y, a_y, b_y, K_y_half, a_y_half, b_y_half. */
+int dn_pml_pro(int nx, int ny, int fw, float dh, float dt, float damping, float fpml, float npower,
+               float k_max_pml, float *out);
+
+/* readmod.c:30-34 + prepare_update_s.c:20-56: eta[L], sumu, bip, bjm,
+ * cip, cjm [L] for relaxation frequencies fl[L]. */
+int dn_relaxation_constants(int L, const float *fl, float dt, float *eta, float *sumu, float *bip, float *bjm,
+                            float *cip, float *cjm);
+
+/* wavelet.c:12-96 for QUELLART 1, 2, 4, 5, 6: out[nt-1] = sample nt. */
+int dn_wavelet(int quellart, int nt, float dt, float tshift, float fc, float amp, float *out);
+
+/* timedomain_filt.c:10-53 (Butterworth, libcseife) on data [ntr][ns] in
+ * place; method 1 lowpass, 2 highpass. */
+int dn_timedomain_filt(float *data, int ntr, int ns, float dt, float fc, int order, int method);
+
+/* ------------------------------------------------------------------------
  * Layer 2: handle API (fast path).  Not in the reference: it replaces the
  * per-step call sequence of denise.c:1179-1346 / 1493-1651 and the shot
  * loop denise.c:875-1702 with resident, batched execution.

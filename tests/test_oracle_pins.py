@@ -1,0 +1,517 @@
+"""Pins of the CPU oracle (oracle/sh_oracle.c) by what the mathematics and the
+reference's own third-party code fix, and parity of the product's host-side
+setup with the oracle.  CPU only.
+
+What pins what:
+  holberg          Taylor rows = solution of the staggered-grid moment
+                   equations; Holberg rows = max phase-velocity error E over
+                   the band [0, 2 pi / ppw] (their defining property).
+  pml_pro          INRIA d0 reflection identity, power-law profile shape,
+                   CPML steady-state identity 1/K + a/(1-b) = alpha/(d+K alpha),
+                   and the integer-abs quirk (a = 0 where 0 < d < 1).
+  butterworth      bit-equal to libcseife built from the reference sources
+                   (oracle/_ref) and equal to scipy's bilinear transform of
+                   the analog Butterworth sections (no prewarp) + lfilter.
+  wavelet          DENISE Ricker == textbook Ricker with peak frequency
+                   fc*sqrt(2)/1.5 delayed by 1.5/fc + tshift.
+  update_v/update_s  FP64 leapfrog energy conservation (skew-adjoint
+                   operators with the zero frame) for FDORDER 2..10; FDORDER 12
+                   breaks it by the hc[5] typo (quirk present); travel time of
+                   a pulse in a homogeneous medium.
+  update_s_visc    taus = 0 reduces to the elastic update; memory variables
+                   decay by ((1-eta/2)/(1+eta/2))^n, the Crank-Nicolson
+                   approximation of exp(-t/tau_sigma).
+  residual/energy  time reversal identity and closed-form energy of a
+                   constant trace.
+  lbfgs/pcg/opt    secant condition and textbook two-loop agreement;
+                   A-conjugacy of PR directions with exact line search;
+                   vertex of an exact parabola; relative step rule.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from common import ROOT, model, oracle, product, rel_l2
+
+O32 = oracle("f32")
+O64 = oracle("f64")
+
+
+def cfg(**kw):
+    from pyoracle import make_cfg
+    return make_cfg(**kw)
+
+
+# --------------------------------------------------------------------------
+# holbergcoeff
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_taylor_coefficients_solve_moment_equations(fdorder):
+    k = fdorder // 2
+    A = np.array([[(2 * m - 1) ** (2 * n - 1) for m in range(1, k + 1)] for n in range(1, k + 1)], float)
+    b = np.zeros(k)
+    b[0] = 1.0
+    c = np.linalg.solve(A, b)
+    hc = O64.holberg(fdorder, 0)
+    np.testing.assert_allclose(hc[1:k + 1], c, rtol=2e-7)
+    assert np.all(hc[k + 1:] == 0)
+
+
+@pytest.mark.parametrize("maxrelerror,E", [(1, 0.001), (2, 0.005), (3, 0.01), (4, 0.03)])
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_holberg_rows_meet_phase_velocity_error(fdorder, maxrelerror, E):
+    hc = O64.holberg(fdorder, maxrelerror)
+    k = fdorder // 2
+    kap = np.linspace(1e-6, 2 * np.pi / hc[0], 20000)
+    m = np.arange(1, k + 1)[:, None]
+    ph = np.sum(hc[1:k + 1, None] * 2 * np.sin((2 * m - 1) * kap / 2), axis=0) / kap
+    err = np.max(np.abs(ph - 1))
+    assert 0.9 * E <= err <= 1.06 * E
+
+
+def test_product_holberg_equals_oracle():
+    dn = product()
+    for me in range(5):
+        for fd in (2, 4, 6, 8, 10, 12):
+            assert np.array_equal(dn.holbergcoeff(fd, me), O32.holberg(fd, me))
+    with pytest.raises(Exception):
+        dn.holbergcoeff(7, 1)
+
+
+# --------------------------------------------------------------------------
+# PML_pro
+# --------------------------------------------------------------------------
+PML_CFGS = [dict(nx=400, ny=120, fw=20, dh=0.25, dt=1e-4, damping=300.0, fpml=40.0),
+            dict(nx=1200, ny=300, fw=20, dh=0.2, dt=1e-4, damping=350.0, fpml=25.0),
+            dict(nx=64, ny=64, fw=10, dh=25.0, dt=1e-2, damping=1000.0, fpml=6.5)]
+
+
+@pytest.mark.parametrize("pc", PML_CFGS)
+def test_pml_profiles_physics(pc):
+    c = cfg(nx=pc["nx"], ny=pc["ny"], fw=pc["fw"], dh=pc["dh"], dt=pc["dt"], damping=pc["damping"],
+            fpml=pc["fpml"], fdorder=4)
+    P = O64.pml(c)
+    fw = pc["fw"]
+    Lw = fw * np.float32(pc["dh"])
+    N = 2.0
+    # left full-grid: distance into the layer at point i is L - dh*(i-1)
+    d = P["d_x"][1:fw + 1]
+    x = Lw - pc["dh"] * np.arange(fw)
+    d0 = d[0] / (x[0] / Lw) ** N
+    np.testing.assert_allclose(d, d0 * (x / Lw) ** N, rtol=1e-5)
+    # INRIA identity: theoretical reflection of the continuous profile is 0.001
+    np.testing.assert_allclose(np.exp(-2.0 * d0 * Lw / ((N + 1) * pc["damping"])), 0.001, rtol=1e-5)
+    for side in ("x", "x_half", "y", "y_half"):
+        d, K, al, a, b = (P["%s_%s" % (q, side)][1:2 * fw + 1] for q in ("d", "K", "alpha", "a", "b"))
+        set_ = a != 0
+        # steady-state response of the recursive convolution equals the CPML stretch at omega = 0
+        lhs = 1.0 / K[set_] + a[set_] / (1.0 - b[set_])
+        rhs = al[set_] / (d[set_] + K[set_] * al[set_])
+        np.testing.assert_allclose(lhs, rhs, rtol=1e-8)
+        # Q1: integer abs() -> no a where 0 < d < 1
+        small = (d > 0) & (d < 1)
+        assert np.all(a[small] == 0)
+        assert np.all(a[d >= 1] != 0)
+        assert np.all((b > 0) & (b <= 1))
+    # the quirk is exercised by these profiles (innermost half-grid points)
+    assert any(np.any((P[k] > 0) & (P[k] < 1)) for k in ("d_x", "d_x_half", "d_y", "d_y_half"))
+
+
+@pytest.mark.parametrize("pc", PML_CFGS)
+def test_product_pml_equals_oracle(pc):
+    dn = product()
+    got = dn.pml_pro(pc["nx"], pc["ny"], pc["fw"], pc["dh"], pc["dt"], pc["damping"], pc["fpml"])
+    c = cfg(nx=pc["nx"], ny=pc["ny"], fw=pc["fw"], dh=pc["dh"], dt=pc["dt"], damping=pc["damping"],
+            fpml=pc["fpml"], fdorder=4)
+    ref = O32.pml(c)
+    for k, v in got.items():
+        assert np.array_equal(v[1:], ref[k][1:2 * pc["fw"] + 1]), k
+
+
+# --------------------------------------------------------------------------
+# Butterworth / timedomain_filt
+# --------------------------------------------------------------------------
+def _cseife_ref():
+    p = os.path.join(ROOT, "oracle", "_ref", "libcseife_ref.so")
+    if not os.path.exists(p):
+        pytest.skip("oracle/_ref/libcseife_ref.so not built (reference tree absent)")
+    L = C.CDLL(p)
+    for f in ("seife_lpb", "seife_hpb"):
+        getattr(L, f).argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.c_double, C.c_int]
+    return L
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 6])
+@pytest.mark.parametrize("method", [1, 2])
+def test_butterworth_equals_libcseife(order, method):
+    L = _cseife_ref()
+    rng = np.random.default_rng(order * 10 + method)
+    x = rng.standard_normal(3000)
+    y_ref = x.copy()
+    f = L.seife_lpb if method == 1 else L.seife_hpb
+    f(y_ref.ctypes.data_as(C.POINTER(C.c_double)), len(y_ref), 1e-4, 1.0 / 30.0, order)
+    y = O64.butterworth(x, 1e-4, 1.0 / 30.0, order, method)
+    assert np.array_equal(y, y_ref)
+
+
+@pytest.mark.parametrize("order", [2, 4, 5])
+def test_butterworth_is_bilinear_analog_butterworth(order):
+    from scipy import signal
+    dt, fc = 1e-4, 60.0
+    w0 = 2 * np.pi * fc
+    rng = np.random.default_rng(order)
+    x = rng.standard_normal(2000)
+    y = x.copy()
+    m = order // 2
+    if order % 2:
+        b, a = signal.bilinear([w0], [1.0, w0], fs=1.0 / dt)
+        y = signal.lfilter(b, a, y)
+    for j in range(1, m + 1):
+        h = np.sin(np.pi / 2 * (2 * j - 1) / order)
+        b, a = signal.bilinear([w0 ** 2], [1.0, 2 * h * w0, w0 ** 2], fs=1.0 / dt)
+        y = signal.lfilter(b, a, y)
+    got = O64.butterworth(x, dt, 1.0 / fc, order, 1)
+    np.testing.assert_allclose(got, y, rtol=1e-9, atol=1e-12 * np.max(np.abs(y)))
+
+
+@pytest.mark.parametrize("method,order", [(1, 4), (2, 4), (1, 3)])
+def test_timedomain_filt_offset_quirk_and_product(method, order):
+    rng = np.random.default_rng(7)
+    data = rng.standard_normal((5, 600)).astype(np.float32)
+    got = O32.timedomain_filt(data, 1e-4, 40.0, order, method)
+    # Q8: the filter sees [0, s1 .. s_{ns-1}], the last sample is untouched
+    for t in range(5):
+        buf = np.concatenate([[0.0], data[t, :-1].astype(np.float64)])
+        y = O64.butterworth(buf, float(np.float32(1e-4)), 1.0 / float(np.float32(40.0)), order, method)
+        np.testing.assert_array_equal(got[t, :-1], y[1:].astype(np.float32))
+        assert got[t, -1] == data[t, -1]
+    prod = product().timedomain_filt(data, 1e-4, 40.0, order, method)
+    assert np.array_equal(prod, got)
+
+
+# --------------------------------------------------------------------------
+# wavelet
+# --------------------------------------------------------------------------
+def test_denise_ricker_is_textbook_ricker():
+    dt, fc, ts = 1e-4, 40.0, 0.002
+    nt = 1000
+    w = O64.wavelet(1, nt, dt, ts, fc, 1.0)
+    t = np.arange(1, nt + 1) * np.float32(dt)
+    fp = fc * np.sqrt(2.0) / 1.5
+    tt = t - 1.5 / fc - ts
+    ref = (1 - 2 * (np.pi * fp * tt) ** 2) * np.exp(-(np.pi * fp * tt) ** 2)
+    np.testing.assert_allclose(w, ref, atol=2e-6)
+    assert abs(t[np.argmax(w)] - (1.5 / fc + ts)) <= dt
+
+
+def test_spike_and_product_wavelets():
+    dn = product()
+    s = O32.wavelet(6, 100, 1e-3, 0.0105, 10.0, 2.0)
+    assert np.count_nonzero(s) == 1 and s[int(np.floor(0.0105 / np.float32(1e-3) + 0.5))] == 2.0
+    for qa in (1, 2, 4, 5, 6):
+        for fc, ts in ((25.0, 0.0), (40.0, 0.003)):
+            a = dn.wavelet(qa, 700, 1e-4, ts, fc, 1.5)
+            b = O32.wavelet(qa, 700, 1e-4, ts, fc, 1.5)
+            assert np.array_equal(a, b), qa
+
+
+# --------------------------------------------------------------------------
+# stencils
+# --------------------------------------------------------------------------
+def _random_state(c, o, seed):
+    nd = c.fdorder // 2 + 1
+    sh = o.padded_shape(c)
+    inner = (slice(nd, nd + c.ny), slice(nd, nd + c.nx))
+    rng = np.random.default_rng(seed)
+    F = {}
+    for k, s in (("vy", 1.0), ("sxy", 1e3), ("syz", 1e3)):
+        a = np.zeros(sh, o.dt)
+        a[inner] = rng.standard_normal((c.ny, c.nx)) * s
+        F[k] = a
+    return F, inner
+
+
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_leapfrog_energy_conservation(fdorder):
+    nx, ny = 40, 30
+    c = cfg(nx=nx, ny=ny, fdorder=fdorder, fw=0, free_surf=1, L=0, dt=1e-4, dh=0.25)
+    vs, rho, _ = model(nx, ny, 0.25, seed=2)
+    P = O64.model_setup(c, vs, rho)
+    hc = O64.holberg(fdorder, 1)
+    pml = O64.pml(c)
+    F, I = _random_state(c, O64, 4)
+    psi = np.zeros(8)
+    vyp1 = np.zeros_like(F["vy"])
+    uy = np.zeros_like(F["vy"])
+    uyx = np.zeros_like(F["vy"])
+    rhoI, uip, ujp = P["rho"][I], P["uip"][I], P["ujp"][I]
+    E = []
+    es_prev = None
+    for n in range(150):
+        vold = F["vy"][I].copy()
+        O64.update_v(c, 0, F["vy"], vyp1, F["syz"], F["sxy"], P["rho"], hc, pml, psi, psi)
+        if es_prev is not None:
+            E.append(0.5 * np.sum(rhoI * vold * F["vy"][I]) + es_prev)
+        O64.update_s_elastic(c, F["vy"], uy, uyx, F["syz"], F["sxy"], P["uip"], P["ujp"], hc, pml, psi, psi)
+        es_prev = 0.5 * np.sum(F["sxy"][I] ** 2 / uip + F["syz"][I] ** 2 / ujp)
+    E = np.array(E)
+    drift = np.max(np.abs(E - E[0])) / E[0]
+    if fdorder < 12:
+        assert drift < 1e-12
+    else:
+        assert drift > 1e-6  # Q6: hc[5] used twice in update_v_PML.c:469 breaks adjointness
+
+
+def test_travel_time_homogeneous():
+    """Peak of a Ricker pulse arrives after distance/vs (+-2%) in a homogeneous medium."""
+    nx, ny, dh, vs0 = 220, 220, 0.5, 200.0
+    c = cfg(nx=nx, ny=ny, fdorder=8, fw=20, free_surf=0, L=0, nt=2200, dt=1e-4, dh=dh, damping=200.0, fpml=30.0)
+    vs = np.full((ny, nx), vs0, np.float32)
+    rho = np.full((ny, nx), 2000.0, np.float32)
+    fc = 30.0
+    sig = O32.wavelet(1, c.nt, 1e-4, 0.0, fc, 1.0)
+    ri = np.array([110 + 20, 110 + 50], np.int32)
+    rj = np.array([110, 110], np.int32)
+    seis, _, _ = O64.forward(c, vs, rho, None, np.array([110], np.int32), np.array([110], np.int32), sig, ri, rj)
+    t = np.arange(1, c.nt + 1) * 1e-4
+    # far-field 2-D line source: peak delayed by r/c (plus a fixed phase lag common to both receivers)
+    tp = [t[np.argmax(np.abs(seis[k]))] for k in range(2)]
+    dt_meas = tp[1] - tp[0]
+    assert abs(dt_meas - 30 * dh / vs0) <= 0.02 * 30 * dh / vs0
+
+
+def test_visco_with_zero_taus_equals_elastic():
+    nx, ny = 40, 30
+    cv = cfg(nx=nx, ny=ny, fdorder=8, fw=6, free_surf=1, L=2, dt=1e-4, dh=0.25, fl=[6.25, 50.0])
+    ce = cfg(nx=nx, ny=ny, fdorder=8, fw=6, free_surf=1, L=0, dt=1e-4, dh=0.25)
+    vs, rho, _ = model(nx, ny, 0.25, seed=2)
+    qs = np.full((ny, nx), np.inf, np.float32)
+    for o, tol in ((O64, 1e-12), (O32, 2e-6)):
+        Pv = o.model_setup(cv, vs, rho, qs)
+        Pe = o.model_setup(ce, vs, rho)
+        hc = o.holberg(8, 1)
+        pml = o.pml(cv)
+        Fv, I = _random_state(cv, o, 9)
+        Fe = {k: v.copy() for k, v in Fv.items()}
+        z = lambda: np.zeros_like(Fv["vy"])
+        r = np.zeros((2,) + Fv["vy"].shape, o.dt)
+        q = np.zeros_like(r)
+        pa, pb = np.zeros(12 * nx, o.dt), np.zeros(12 * ny, o.dt)
+        pa2, pb2 = pa.copy(), pb.copy()
+        o.update_s_visc(cv, Fv["vy"], z(), z(), Fv["syz"], Fv["sxy"], r, q, Pv["fipjp"], Pv["f"], Pv["dip"], Pv["d"],
+                        Pv["bip"], Pv["bjm"], Pv["cip"], Pv["cjm"], hc, pml, pa, pb)
+        o.update_s_elastic(ce, Fe["vy"], z(), z(), Fe["syz"], Fe["sxy"], Pe["uip"], Pe["ujp"], hc, pml, pa2, pb2)
+        # visco uses dhi = 1/DH with fipjp = mu*DT; elastic dhi = DT/DH with mu
+        assert rel_l2(Fv["sxy"][I], Fe["sxy"][I]) < tol
+        assert rel_l2(Fv["syz"][I], Fe["syz"][I]) < tol
+        assert np.all(r == 0) and np.all(q == 0)
+
+
+def test_memory_variable_decay():
+    nx, ny = 20, 16
+    fl = [6.25, 50.0]
+    c = cfg(nx=nx, ny=ny, fdorder=4, fw=0, free_surf=1, L=2, dt=1e-4, dh=0.25, fl=fl)
+    vs, rho, _ = model(nx, ny, 0.25, seed=2)
+    P = O64.model_setup(c, vs, rho, np.full((ny, nx), 20.0, np.float32))
+    hc = O64.holberg(4, 1)
+    pml = O64.pml(c)
+    sh = O64.padded_shape(c)
+    r = np.ones((2,) + sh)
+    q = np.ones((2,) + sh) * 2.0
+    z = lambda: np.zeros(sh)
+    psi = np.zeros(4)
+    vy, sxy, syz = z(), z(), z()
+    n = 50
+    for _ in range(n):
+        O64.update_s_visc(c, vy, z(), z(), syz, sxy, r, q, P["fipjp"], P["f"], P["dip"], P["d"], P["bip"], P["bjm"],
+                          P["cip"], P["cjm"], hc, pml, psi, psi)
+    for l, f in enumerate(fl):
+        eta = np.float64(np.float32(1e-4)) * 2 * np.pi * f
+        g = (1 - eta / 2) / (1 + eta / 2)
+        np.testing.assert_allclose(r[l, 5, 5], g ** n, rtol=1e-10)
+        np.testing.assert_allclose(q[l, 5, 5], 2 * g ** n, rtol=1e-10)
+        # Crank-Nicolson: per-step factor matches exp(-eta) to O(eta^3)
+        assert abs(g - np.exp(-eta)) <= eta ** 3 / 12 * 1.01
+
+
+# --------------------------------------------------------------------------
+# residual / energy
+# --------------------------------------------------------------------------
+def test_residual_reversal_and_energy_closed_form():
+    ntr, ns, dt = 3, 50, 1e-3
+    rng = np.random.default_rng(3)
+    syn = rng.standard_normal((ntr, ns))
+    obs = np.ones((ntr, ns))
+    diff = np.zeros((ntr, ns))
+    l2 = C.c_double(0.0)
+    en = C.c_double(0.0)
+    f = O64.fn("orc_residual")
+    rp = C.POINTER(C.c_double)
+    obs_c = obs.copy()
+    f(ntr, ns, C.c_double(dt), syn.ctypes.data_as(rp), obs_c.ctypes.data_as(rp), diff.ctypes.data_as(rp),
+      C.byref(l2), C.byref(en))
+    # Q7: the first observed sample is replaced by the synthetic one
+    exp = syn - obs
+    exp[:, 0] = 0.0
+    np.testing.assert_array_equal(diff, exp[:, ::-1])
+    np.testing.assert_allclose(l2.value, np.sum(exp ** 2), rtol=1e-14)
+    # energy of the time-integrated observed trace: obs' = [syn0, 1, 1, ...]
+    cum = np.cumsum(np.concatenate([syn[:, :1], np.ones((ntr, ns - 1))], axis=1), axis=1)
+    np.testing.assert_allclose(en.value, np.sum(cum ** 2) * dt * dt, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# optimiser pieces
+# --------------------------------------------------------------------------
+def test_calc_opt_step_parabola_vertex():
+    a, b, cc = 2.0, -3.0, 1.5  # vertex at 1.0
+    eps = np.array([0.0, 0.8, 1.6])
+    L2t = a + b * eps + cc * eps ** 2
+    assert abs(O64.calc_opt_step(L2t, eps) - 1.0) < 1e-12
+    # vertex beyond the bracket is clamped to epst[3]
+    eps2 = np.array([0.0, 0.2, 0.4])
+    assert O64.calc_opt_step(a + b * eps2 + cc * eps2 ** 2, eps2) == 0.4
+
+
+def _textbook_two_loop(S, Y, g):
+    q = g.copy()
+    al = []
+    for s, y in zip(S[::-1], Y[::-1]):
+        a = (s @ q) / (y @ s)
+        al.append(a)
+        q -= a * y
+    r = (S[-1] @ Y[-1]) / (Y[-1] @ Y[-1]) * q
+    for (s, y), a in zip(zip(S, Y), al[::-1]):
+        b = (y @ r) / (y @ s)
+        r += s * (a - b)
+    return r
+
+
+def test_lbfgs_two_loop():
+    rng = np.random.default_rng(0)
+    n, m = 30, 5
+    A = rng.standard_normal((n, n))
+    A = A @ A.T + n * np.eye(n)
+    S = [rng.standard_normal(n) for _ in range(m)]
+    Y = [A @ s for s in S]
+    nslots = 8  # slots m+1..nslots empty (rho = 0), as before the ring wraps
+    ys = np.zeros((nslots, n))
+    ss = np.zeros((nslots, n))
+    ys[:m] = Y
+    ss[:m] = S
+    g = rng.standard_normal(n)
+    r = O64.lbfgs(ys, ss, g, cur=m)
+    np.testing.assert_allclose(r, _textbook_two_loop(S, Y, g), rtol=1e-10)
+    # secant condition of the newest pair
+    np.testing.assert_allclose(O64.lbfgs(ys, ss, Y[-1], cur=m), S[-1], rtol=1e-9)
+
+
+def test_pcg_polak_ribiere_conjugacy():
+    rng = np.random.default_rng(1)
+    n = 12
+    A = rng.standard_normal((n, n))
+    A = A @ A.T + n * np.eye(n)
+    bvec = rng.standard_normal(n)
+    x = np.zeros(n)
+    gold = np.zeros((1, n))
+    d = np.zeros((1, n))
+    dirs = []
+    for _ in range(4):
+        g = A @ x - bvec
+        gnew = -g[None, :]          # descent direction as stored by the reference
+        d = O64.pcg(gnew, gold, d)
+        p = d[0]
+        alpha = -(g @ p) / (p @ A @ p)  # exact line search
+        x = x + alpha * p
+        dirs.append(p.copy())
+        gold = gnew
+    for i in range(len(dirs)):
+        for j in range(i):
+            assert abs(dirs[i] @ A @ dirs[j]) < 1e-8 * np.linalg.norm(A @ dirs[i]) * np.linalg.norm(dirs[j])
+
+
+def test_calc_mat_change_rule():
+    f = O64.fn("orc_calc_mat_change")
+    f.restype = C.c_double
+    rp = C.POINTER(C.c_double)
+    nx, ny = 4, 3
+    u = np.full((ny, nx), 200.0)
+    rho = np.full((ny, nx), 1800.0)
+    gu = np.zeros((ny, nx))
+    grho = np.zeros((ny, nx))
+    gu[1, 2] = -4.0
+    gu[0, 0] = 2.0
+    grho[2, 3] = 10.0
+    un = np.zeros_like(u)
+    rn = np.zeros_like(u)
+    er = C.c_double()
+    eps_u = f(nx, ny, gu.ctypes.data_as(rp), grho.ctypes.data_as(rp), u.ctypes.data_as(rp), rho.ctypes.data_as(rp),
+              un.ctypes.data_as(rp), rn.ctypes.data_as(rp), C.c_double(0.01), 5, 0, 0, C.c_double(0.5),
+              C.c_double(0.0), C.c_double(205.0), C.c_double(0.0), C.c_double(5000.0), C.byref(er))
+    assert abs(eps_u - 0.01 * 200.0 / 4.0) < 1e-12
+    assert abs(er.value - 0.01 * 1800.0 / 10.0 * 0.5) < 1e-12
+    # +2.0 cell: 200 + 0.5*4 = 202 within the limit; -4 cell: 200 + 2 = 202 -> kept; check revert above 205
+    assert un[1, 2] == 202.0 and un[0, 0] == 199.0
+    f(nx, ny, gu.ctypes.data_as(rp), grho.ctypes.data_as(rp), u.ctypes.data_as(rp), rho.ctypes.data_as(rp),
+      un.ctypes.data_as(rp), rn.ctypes.data_as(rp), C.c_double(0.02), 5, 0, 0, C.c_double(0.5), C.c_double(0.0),
+      C.c_double(203.0), C.c_double(0.0), C.c_double(5000.0), C.byref(er))
+    assert un[1, 2] == 200.0  # 204 > VSUPPERLIM 203: reverted, not clamped (Q17)
+
+
+def test_dtstab_homogeneous_closed_form():
+    c = cfg(nx=30, ny=20, fdorder=8, maxrelerror=1, fw=5, dt=1e-4, dh=0.25)
+    vs = np.full((20, 30), 300.0, np.float32)
+    rho = np.full((20, 30), 1800.0, np.float32)
+    hc = O64.holberg(8, 1)
+    ref = 0.25 / (np.sqrt(2) * np.sum(np.abs(hc[1:])) * 300.0)
+    np.testing.assert_allclose(O64.dtstab(c, vs, rho, None, 0.025), ref, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# product host pieces that need no device
+# --------------------------------------------------------------------------
+def test_relaxation_constants_equal_oracle():
+    dn = product()
+    fl = [6.25, 50.0]
+    c = cfg(nx=10, ny=8, fdorder=4, fw=2, L=2, dt=1e-4, dh=0.2, fl=fl)
+    vs, rho, _ = model(10, 8, 0.2, seed=1)
+    P = O32.model_setup(c, vs, rho, np.full((8, 10), 20.0, np.float32))
+    got = dn.relaxation_constants(fl, 1e-4)
+    for k in ("bip", "bjm", "cip", "cjm"):
+        assert np.array_equal(got[k], P[k][:2]), k
+
+
+def test_library_exports_every_declared_symbol():
+    dn = product()
+    hdr = open(os.path.join(ROOT, "include", "denise_sh.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_]\w*)\s*\(", hdr)) - {"if", "sizeof"}
+    assert names, "no declarations parsed"
+    L = C.CDLL(dn.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(dn.EXPORTS) == names
+
+
+def test_seismo_ssg_and_psource_host_semantics():
+    dn = product()
+    nx, ny, nt = 8, 6, 10
+    dn.set_globals(NX=nx, NY=ny, FDORDER=4, FW=2, FREE_SURF=1, L=0, DT=1e-3, DH=1.0, NT=nt)
+    vy = dn.NRMatrix(-2, ny + 3, -2, nx + 3, np.arange((ny + 6) * (nx + 6), dtype=np.float32).reshape(ny + 6, nx + 6))
+    rec = dn.NRIMatrix(1, 3, 1, 2, np.array([[2, 5], [3, 1], [1, 2]], np.int32))
+    sec = dn.NRMatrix(1, 2, 1, nt)
+    L = dn.lib()
+    L.seismo_ssg.restype = None
+    L.seismo_ssg(4, 2, rec.ptr, sec.ptr, vy.ptr, None, None, None)
+    assert sec[1, 4] == vy[3, 2] and sec[2, 4] == vy[1, 5]
+    sxx = dn.NRMatrix(-2, ny + 3, -2, nx + 3)
+    syy = dn.NRMatrix(-2, ny + 3, -2, nx + 3)
+    src = dn.NRMatrix(1, 8, 1, 1, np.array([[3], [2], [0], [0], [10], [1], [0], [1]], np.float32))
+    sig = dn.NRMatrix(1, 1, 1, nt, np.arange(1, nt + 1, dtype=np.float32)[None, :] ** 2)
+    L.psource.restype = None
+    L.psource(5, sxx.ptr, syy.ptr, src.ptr, sig.ptr, 1)
+    amp = np.float32((np.float32(36.0) - np.float32(16.0)) / np.float32(1e-3))
+    assert sxx[2, 3] == amp and syy[2, 3] == amp
