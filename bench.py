@@ -1,0 +1,176 @@
+"""Benchmark of the SH adjoint-gradient hot path on one MI355X.
+
+Workload (viscoelastic FWI configuration): NX=1200, NY=300, DH=0.2 m,
+FDORDER=8 (Holberg, MAXRELERROR 1), L=2 relaxation mechanisms (Q=20,
+FL = 6.25/50 Hz), 20-point CPML, free surface on top, DT=1e-4 s, NT=10000
+(1 s records), 40 shots along the surface, 25 Hz Ricker, 240 receivers at
+1 m spacing, snapshots every DTINV=4 steps.  True model: vs gradient
+100->350 m/s with a +-10% checkerboard of 8 m cells; the gradient is taken
+at the 1-D starting model; rho 1700->2000 kg/m^3.  Observed data are
+synthesised on the device from the true model before timing (synthetic,
+seeded).
+
+One step = one misfit gradient over all 40 shots: forward modelling with
+snapshot storage, residuals, adjoint modelling with the zero-lag
+correlation, energy preconditioning and the chain rule.  The metric counts
+cell-updates of the forward and the adjoint sweeps (NX*NY*NT*2 per shot).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT]
+       [--shots S] [--snap fp32|bf16|fp16] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def _package():
+    import importlib.util
+    name = "denise_sh_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "denise-sh_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(nt):
+    nx, ny, dh = 1200, 300, 0.2
+    j = np.arange(ny, dtype=np.float64)[:, None]
+    i = np.arange(nx, dtype=np.float64)[None, :]
+    vs1d = 100.0 + 250.0 * j / (ny - 1) + 0.0 * i
+    cell = 8.0 / dh  # 8 m checkerboard cells
+    cb = np.where(((np.floor(i / cell) + np.floor(j / cell)) % 2) == 0, 1.0, -1.0)
+    vs_true = (vs1d * (1.0 + 0.1 * cb)).astype(np.float32)
+    vs_start = vs1d.astype(np.float32)
+    rho = (1700.0 + 300.0 * j / (ny - 1) + 0.0 * i).astype(np.float32)
+    qs = np.full((ny, nx), 20.0, np.float32)
+    nshots = 40
+    src_x = np.linspace(3.0, nx * dh - 3.0, nshots)
+    src_i = np.floor(src_x / dh + 0.5).astype(np.int32)          # iround(x/DH), splitsrc.c
+    src_j = np.full(nshots, int(np.floor(0.5 / dh + 0.5)), np.int32)
+    rec_x = 0.5 + np.arange(240) * 1.0
+    rec_i = np.floor(rec_x / dh + 0.5).astype(np.int32)          # receiver.c:40
+    rec_j = np.full(240, int(np.floor(0.5 / dh + 0.5)), np.int32)
+    return dict(nx=nx, ny=ny, dh=dh, dt=1e-4, nt=nt, vs_true=vs_true, vs_start=vs_start, rho=rho, qs=qs,
+                src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j, fdorder=8, L=2, fl=[6.25, 50.0], fw=20,
+                fc=25.0, dtinv=4)
+
+
+def ricker(dn, nt, dt, fc):
+    return dn.wavelet(1, nt, dt, 0.0, fc, 1.0)
+
+
+def cpu_baseline(W, nt_sample):
+    """Oracle (FP32 restatement of the reference) on the host cores: one shot,
+    forward + adjoint over nt_sample steps of the same grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, make_cfg
+    o = Oracle("f32")
+    cfg = make_cfg(W["nx"], W["ny"], fdorder=W["fdorder"], fw=W["fw"], free_surf=1, L=W["L"], nt=nt_sample,
+                   dtinv=W["dtinv"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], fl=W["fl"])
+    sig = o.wavelet(1, nt_sample, W["dt"], 0.0, W["fc"], 1.0).astype(np.float32)[None]
+    obs = np.zeros((1, len(W["rec_i"]), nt_sample), np.float32)
+    t0 = time.perf_counter()
+    o.gradient(cfg, W["vs_start"], W["rho"], W["qs"], W["src_i"][:1], W["src_j"][:1], sig, W["rec_i"], W["rec_j"],
+               obs)
+    sec = time.perf_counter() - t0
+    cells = 2.0 * W["nx"] * W["ny"] * nt_sample
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=cells / sec / 1e9, unit="Gcell-updates/s", cores=threads, kind="port",
+                sample="1 shot of the same grid, forward+adjoint gradient over %d steps (%.1f s)" % (nt_sample, sec))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=10000)
+    ap.add_argument("--shots", type=int, default=40)
+    ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs (--gpus %d) are not provided" % args.gpus,
+              file=sys.stderr)
+        sys.exit(3)
+
+    dn = _package()
+    W = workload(args.nt)
+    prec = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snap]
+    ctx = dn.Context(W["nx"], W["ny"], fdorder=W["fdorder"], maxrelerror=1, fw=W["fw"], free_surf=1, L=W["L"],
+                     fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], dtinv=W["dtinv"],
+                     snap_precision=prec)
+    ctx.set_receivers(W["rec_i"], W["rec_j"])
+    sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
+    ctx.set_shots(W["src_i"][:args.shots], W["src_j"][:args.shots], sig)
+    # observed data from the true model (untimed setup, on the device)
+    ctx.set_model(W["vs_true"], W["rho"], W["qs"])
+    obs = ctx.forward()
+    ctx.set_observed(obs)
+    ctx.set_model(W["vs_start"], W["rho"], W["qs"])
+    for _ in range(args.warmup):
+        ctx.gradient()
+    ms = {0: 0.0, 1: 0.0}
+    la = {0: 0, 1: 0}
+    ce = {0: 0, 1: 0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        l2, en = ctx.gradient()
+        for p in (0, 1):
+            m, l, c = ctx.pass_timing(p)
+            ms[p] += m
+            la[p] += l
+            ce[p] += c
+    wall = time.perf_counter() - t0
+    cells = float(ce[0] + ce[1])
+    value = cells / wall / 1e9
+    # dominant kernel: the fused step kernel of the pass with the larger total time
+    dom = 1 if ms[1] >= ms[0] else 0
+    per_launch_ms = ms[dom] / max(la[dom], 1)
+    units_per_launch = ce[dom] / max(la[dom], 1)          # cell-updates per launch
+    bytes_per_cell = 12.0 * (3 + 2 * W["L"]) + (36.0 if dom == 1 else 20.0) / W["dtinv"]
+    if prec != 0:
+        bytes_per_cell -= (6.0 / W["dtinv"])
+    achieved = units_per_launch * bytes_per_cell / (per_launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s fwd+adj",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded checkerboard true model, observed data modelled on the device)",
+        "config": {"workload": "visco SH gradient 1200x300 FDORDER8 L2, %d shots x %d steps, DTINV %d, %s snapshots"
+                               % (args.shots, args.nt, W["dtinv"], args.snap),
+                   "nx": W["nx"], "ny": W["ny"], "shots": args.shots, "nt": args.nt, "fdorder": 8, "L": 2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_step<4,visco,%s>" % ("adjoint" if dom else "forward"),
+                     "ms_per_launch": round(per_launch_ms, 4), "bytes_per_cell_update": bytes_per_cell,
+                     "cell_updates_per_launch": units_per_launch},
+        "misfit_l2": l2,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -18,7 +18,11 @@ EXPORTS = [
     "dn_create", "dn_destroy", "dn_last_error", "dn_set_model", "dn_get_model", "dn_set_receivers",
     "dn_set_shots", "dn_set_observed", "dn_set_batch", "dn_forward", "dn_gradient", "dn_get_gradient",
     "dn_forward_fields", "dn_last_timing", "dn_holbergcoeff", "dn_pml_pro", "dn_relaxation_constants",
-    "dn_wavelet", "dn_timedomain_filt",
+    "dn_wavelet", "dn_timedomain_filt", "dn_pass_timing", "dn_set_shots_multi", "dn_set_signals",
+    "dn_par_read", "dn_par_free", "dn_par_get_int", "dn_par_get_float", "dn_par_get_floats", "dn_par_get_string",
+    "dn_workflow_read", "dn_workflow_free", "dn_workflow_nstages", "dn_workflow_get", "dn_read_sources",
+    "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
+    "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction",
 ]
 
 
@@ -174,6 +178,15 @@ class Context:
         out = [np.zeros((self.ny, self.nx), np.float32) for _ in range(3)]
         self._chk(lib().dn_forward_fields(self._h, shot, nsteps, *[_f(a) for a in out]))
         return out  # vy, sxy, syz
+
+    def pass_timing(self, adjoint):
+        ms = C.c_double()
+        la = C.c_longlong()
+        ce = C.c_longlong()
+        lib().dn_pass_timing.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                         C.POINTER(C.c_longlong)]
+        self._chk(lib().dn_pass_timing(self._h, 1 if adjoint else 0, C.byref(ms), C.byref(la), C.byref(ce)))
+        return ms.value, la.value, ce.value
 
     def last_timing(self):
         ms = C.c_double()

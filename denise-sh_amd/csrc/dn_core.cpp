@@ -185,6 +185,7 @@ static void free_batch(dn_ctx *c)
     dfree_t(c->d_seis); dfree_t(c->d_resid); dfree_t(c->d_snap); dfree_t(c->d_snap16); dfree_t(c->d_snap_scale);
     dfree_t(c->d_stage); dfree_t(c->d_stage_max);
     dfree_t(c->d_Ws); dfree_t(c->d_Wr); dfree_t(c->d_gu_shot); dfree_t(c->d_grho_shot); dfree_t(c->d_part);
+    dfree_t(c->d_shot_map);
     c->batch_cap = 0;
     c->batch_nt_max = 0;
 }
@@ -193,6 +194,7 @@ extern "C" void dn_destroy(dn_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dn_fwi_destroy(c);
     free_batch(c);
     dfree_t(c->d_pml);
     dfree_t(c->d_vs); dfree_t(c->d_rho); dfree_t(c->d_taus); dfree_t(c->d_uip); dfree_t(c->d_ujp);
@@ -321,36 +323,60 @@ extern "C" int dn_set_receivers(dn_ctx *c, int ntr, const int *ri, const int *rj
     return 0;
 }
 
-extern "C" int dn_set_shots(dn_ctx *c, int nshots, int nt_max, const int *nt, const int *si, const int *sj,
-                            const float *signals)
+extern "C" int dn_set_shots_multi(dn_ctx *c, int nshots, int nt_max, const int *nt, const int *src_off,
+                                  const int *si, const int *sj, const float *signals)
 {
-    if (!c || nshots < 1 || nt_max < 1 || !nt || !si || !sj || !signals)
+    if (!c || nshots < 1 || nt_max < 1 || !nt || !src_off || !si || !sj || !signals)
         return dn_fail(c, "dn_set_shots: bad arguments");
+    if (src_off[0] != 0) return dn_fail(c, "dn_set_shots: src_off[0] must be 0");
     for (int s = 0; s < nshots; ++s) {
         if (nt[s] < 1 || nt[s] > nt_max) return dn_fail(c, "shot %d: nt=%d outside 1..%d", s + 1, nt[s], nt_max);
-        if (si[s] < 1 || si[s] > c->p.nx || sj[s] < 1 || sj[s] > c->p.ny)
-            return dn_fail(c, "shot %d: source (%d,%d) outside the grid", s + 1, si[s], sj[s]);
+        if (src_off[s + 1] < src_off[s]) return dn_fail(c, "shot %d: bad source offsets", s + 1);
     }
+    const int nsrc = src_off[nshots];
+    for (int l = 0; l < nsrc; ++l)
+        if (si[l] < 1 || si[l] > c->p.nx || sj[l] < 1 || sj[l] > c->p.ny)
+            return dn_fail(c, "source %d at (%d,%d) outside the grid", l + 1, si[l], sj[l]);
     dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
     dfree_t(c->d_obs);
-    std::vector<int> off(nshots + 1);
-    for (int s = 0; s <= nshots; ++s) off[s] = s;
     HIPCHK(c, dalloc(&c->d_shot_nt, nshots));
     HIPCHK(c, dalloc(&c->d_src_off, nshots + 1));
-    HIPCHK(c, dalloc(&c->d_src_i, nshots));
-    HIPCHK(c, dalloc(&c->d_src_j, nshots));
-    HIPCHK(c, dalloc(&c->d_sig, (size_t)nshots * nt_max));
+    HIPCHK(c, dalloc(&c->d_src_i, std::max(1, nsrc)));
+    HIPCHK(c, dalloc(&c->d_src_j, std::max(1, nsrc)));
+    HIPCHK(c, dalloc(&c->d_sig, (size_t)std::max(1, nsrc) * nt_max));
     HIPCHK(c, hipMemcpy(c->d_shot_nt, nt, nshots * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_src_off, off.data(), (nshots + 1) * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_src_i, si, nshots * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_src_j, sj, nshots * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_sig, signals, (size_t)nshots * nt_max * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_src_off, src_off, (nshots + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (nsrc > 0) {
+        HIPCHK(c, hipMemcpy(c->d_src_i, si, nsrc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_src_j, sj, nsrc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_sig, signals, (size_t)nsrc * nt_max * sizeof(float), hipMemcpyHostToDevice));
+    }
     c->nshots = nshots;
     c->nt_max = nt_max;
     c->shot_nt.assign(nt, nt + nshots);
-    c->src_i.assign(si, si + nshots);
-    c->src_j.assign(sj, sj + nshots);
+    c->src_i.assign(si, si + nsrc);
+    c->src_j.assign(sj, sj + nsrc);
+    c->shot_l2.assign(nshots, 0.0);
+    c->shot_energy.assign(nshots, 0.0);
+    c->cur_map.clear();
     free_batch(c);
+    return 0;
+}
+
+extern "C" int dn_set_shots(dn_ctx *c, int nshots, int nt_max, const int *nt, const int *si, const int *sj,
+                            const float *signals)
+{
+    if (!c || nshots < 1) return dn_fail(c, "dn_set_shots: bad arguments");
+    std::vector<int> off(nshots + 1);
+    for (int s = 0; s <= nshots; ++s) off[s] = s;
+    return dn_set_shots_multi(c, nshots, nt_max, nt, off.data(), si, sj, signals);
+}
+
+extern "C" int dn_set_signals(dn_ctx *c, const float *signals)
+{
+    if (!c || !signals || c->nshots < 1) return dn_fail(c, "dn_set_signals: set shots first");
+    const size_t n = c->src_i.size() * (size_t)c->nt_max;
+    HIPCHK(c, hipMemcpy(c->d_sig, signals, n * sizeof(float), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -429,6 +455,7 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
     HIPCHK(c, dalloc(&c->d_gu_shot, S * nxny));
     HIPCHK(c, dalloc(&c->d_grho_shot, S * nxny));
     HIPCHK(c, dalloc(&c->d_part, S * c->ntr * 2));
+    HIPCHK(c, dalloc(&c->d_shot_map, S));
     c->batch_has_snap = gradient;
     if (gradient) {
         const size_t ns = (size_t)dn_nsnap(c);
@@ -462,7 +489,7 @@ static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
     return std::max(std::min(best, c->g.ny), std::min(4 * c->K, c->g.ny));
 }
 
-void dn_fill_step_common(dn_ctx *c, StepArgs &a, int shot0, int n)
+void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
 {
     const Geo &g = c->g;
     a = StepArgs{};
@@ -498,8 +525,9 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int shot0, int n)
     a.q = c->d_q;
     a.pvx = c->d_pvx;
     a.pvy = c->d_pvy;
-    a.shot_nt = c->d_shot_nt + shot0;
-    a.src_off = c->d_src_off + shot0;  // offsets are absolute indices into src arrays
+    a.shot_map = c->d_shot_map;
+    a.shot_nt = c->d_shot_nt;
+    a.src_off = c->d_src_off;
     a.src_i = c->d_src_i;
     a.src_j = c->d_src_j;
     a.src_sig = c->d_sig;
@@ -532,13 +560,19 @@ int dn_zero_state(dn_ctx *c, int n)
 
 // One pass (forward or adjoint) over shots [shot0, shot0+n) resident in batch
 // slots 0..n-1.  store: forward snapshots + Ws; seis: forward seismograms.
-int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool record)
+int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record)
 {
+    const int n = (int)shots.size();
+    if (n < 1 || n > c->batch_cap) return dn_fail(c, "dn_run_pass: %d shots for a batch of %d", n, c->batch_cap);
     int ntmax = 0;
-    for (int s = 0; s < n; ++s) ntmax = std::max(ntmax, c->shot_nt[shot0 + s]);
+    for (int s = 0; s < n; ++s) ntmax = std::max(ntmax, c->shot_nt[shots[s]]);
+    if (c->cur_map != shots) {
+        HIPCHK(c, hipMemcpyAsync(c->d_shot_map, shots.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        c->cur_map = shots;
+    }
     if (dn_zero_state(c, n)) return -1;
     StepArgs a;
-    dn_fill_step_common(c, a, shot0, n);
+    dn_fill_step_common(c, a, n);
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     const int nsnap = dn_nsnap(c);
     const int kind = c->p.snap_precision;
@@ -592,7 +626,12 @@ int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool reco
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->ms_steps += ms;
-    for (int s = 0; s < n; ++s) c->cell_updates += (long long)c->g.nx * c->g.ny * c->shot_nt[shot0 + s];
+    long long cells = 0;
+    for (int s = 0; s < n; ++s) cells += (long long)c->g.nx * c->g.ny * c->shot_nt[shots[s]];
+    c->cell_updates += cells;
+    c->ms_pass[adjoint ? 1 : 0] += ms;
+    c->launches_pass[adjoint ? 1 : 0] += ntmax;
+    c->cells_pass[adjoint ? 1 : 0] += cells;
     c->cur_final = cur;
     // NaN probe of the centre cell (denise.c:1183-1185)
     std::vector<float> centre(n);
@@ -601,7 +640,55 @@ int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool reco
         HIPCHK(c, hipMemcpy(&centre[s], c->d_vy[cur] + (size_t)s * c->g.plane + off, sizeof(float),
                             hipMemcpyDeviceToHost));
     for (int s = 0; s < n; ++s)
-        if (std::isnan(centre[s])) return dn_fail(c, "Simulation is unstable ! (shot %d)", shot0 + s + 1);
+        if (std::isnan(centre[s])) return dn_fail(c, "Simulation is unstable ! (shot %d)", shots[s] + 1);
+    return 0;
+}
+
+int dn_batch_residual(dn_ctx *c, const std::vector<int> &shots)
+{
+    const int n = (int)shots.size();
+    HIPCHK(c, launch_residual(n, c->ntr, c->nt_max, c->d_shot_map, c->d_shot_nt, c->d_seis, c->d_obs, c->d_resid,
+                              c->d_part, c->p.dt, c->stream));
+    std::vector<double> part((size_t)n * c->ntr * 2);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < n; ++s) {
+        double l2 = 0.0, en = 0.0;
+        for (int r = 0; r < c->ntr; ++r) {
+            l2 += part[2 * ((size_t)s * c->ntr + r)];
+            en += part[2 * ((size_t)s * c->ntr + r) + 1];
+        }
+        c->shot_l2[shots[s]] = l2;
+        c->shot_energy[shots[s]] = en;
+    }
+    return 0;
+}
+
+void dn_reset_timing(dn_ctx *c)
+{
+    c->ms_steps = 0.0;
+    c->launches = 0;
+    c->cell_updates = 0;
+    for (int k = 0; k < 2; ++k) {
+        c->ms_pass[k] = 0.0;
+        c->launches_pass[k] = 0;
+        c->cells_pass[k] = 0;
+    }
+}
+
+int dn_forward_shots(dn_ctx *c, const std::vector<int> &shots, float *seis)
+{
+    for (size_t b0 = 0; b0 < shots.size();) {
+        if (dn_ensure_batch(c, (int)(shots.size() - b0), false)) return -1;
+        const size_t n = std::min((size_t)c->batch_cap, shots.size() - b0);
+        std::vector<int> part(shots.begin() + b0, shots.begin() + b0 + n);
+        if (dn_run_pass(c, part, false, false, true)) return -1;
+        if (seis)
+            HIPCHK(c, hipMemcpy(seis + b0 * c->ntr * c->nt_max, c->d_seis, n * c->ntr * c->nt_max * sizeof(float),
+                                hipMemcpyDeviceToHost));
+        if (c->d_obs && dn_batch_residual(c, part)) return -1;
+        b0 += n;
+    }
     return 0;
 }
 
@@ -609,18 +696,36 @@ extern "C" int dn_forward(dn_ctx *c, int shot0, int nshots, float *seis)
 {
     if (!c) return -1;
     if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_forward: bad shot range");
-    c->ms_steps = 0.0;
-    c->launches = 0;
-    c->cell_updates = 0;
-    for (int b0 = shot0; b0 < shot0 + nshots;) {
-        if (dn_ensure_batch(c, shot0 + nshots - b0, false)) return -1;
-        const int n = std::min(c->batch_cap, shot0 + nshots - b0);
-        if (dn_run_pass(c, b0, n, false, false, true)) return -1;
-        if (seis)
-            HIPCHK(c, hipMemcpy(seis + (size_t)(b0 - shot0) * c->ntr * c->nt_max, c->d_seis,
-                                (size_t)n * c->ntr * c->nt_max * sizeof(float), hipMemcpyDeviceToHost));
+    dn_reset_timing(c);
+    std::vector<int> shots(nshots);
+    for (int s = 0; s < nshots; ++s) shots[s] = shot0 + s;
+    return dn_forward_shots(c, shots, seis);
+}
+
+int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
+{
+    if (!c->d_obs) return dn_fail(c, "gradient: observed data not set");
+    if (!c->have_C) return dn_fail(c, "gradient: model not set");
+    const size_t nxny = (size_t)c->g.nx * c->g.ny;
+    if (!accumulate) {
+        HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_grho, nxny, 0.0f, c->stream));
+    }
+    for (size_t b0 = 0; b0 < shots.size();) {
+        if (dn_ensure_batch(c, (int)(shots.size() - b0), true)) return -1;
+        const size_t n = std::min((size_t)c->batch_cap, shots.size() - b0);
+        std::vector<int> part(shots.begin() + b0, shots.begin() + b0 + n);
+        if (dn_run_pass(c, part, false, true, true)) return -1;
+        if (dn_batch_residual(c, part)) return -1;
+        if (dn_run_pass(c, part, true, false, false)) return -1;
+        HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->d_Ws, c->d_Wr, c->d_gu_shot,
+                                             c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits,
+                                             c->stream));
         b0 += n;
     }
+    HIPCHK(c, launch_chain_rule(c->g, c->g.nx, c->g.ny, c->d_gu, c->d_grho, c->d_vs, c->d_rho, c->p.dt,
+                                c->d_gvs_out, c->d_grho_out, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -628,41 +733,15 @@ extern "C" int dn_gradient(dn_ctx *c, int shot0, int nshots, int accumulate, dou
 {
     if (!c) return -1;
     if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_gradient: bad shot range");
-    if (!c->d_obs) return dn_fail(c, "dn_gradient: observed data not set");
-    if (!c->have_C) return dn_fail(c, "dn_gradient: model not set");
-    const size_t nxny = (size_t)c->g.nx * c->g.ny;
-    if (!accumulate) {
-        HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));
-        HIPCHK(c, launch_fill(c->d_grho, nxny, 0.0f, c->stream));
-    }
-    c->ms_steps = 0.0;
-    c->launches = 0;
-    c->cell_updates = 0;
+    dn_reset_timing(c);
+    std::vector<int> shots(nshots);
+    for (int s = 0; s < nshots; ++s) shots[s] = shot0 + s;
+    if (dn_gradient_shots(c, shots, accumulate != 0)) return -1;
     double L2 = 0.0, EN = 0.0;
-    std::vector<double> part;
-    for (int b0 = shot0; b0 < shot0 + nshots;) {
-        if (dn_ensure_batch(c, shot0 + nshots - b0, true)) return -1;
-        const int n = std::min(c->batch_cap, shot0 + nshots - b0);
-        if (dn_run_pass(c, b0, n, false, true, true)) return -1;
-        HIPCHK(c, launch_residual(n, c->ntr, c->nt_max, c->d_shot_nt + b0, c->d_seis,
-                                  c->d_obs + (size_t)b0 * c->ntr * c->nt_max, c->d_resid, c->d_part, c->p.dt,
-                                  c->stream));
-        part.resize((size_t)n * c->ntr * 2);
-        HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (size_t t = 0; t < (size_t)n * c->ntr; ++t) {
-            L2 += part[2 * t];
-            EN += part[2 * t + 1];
-        }
-        if (dn_run_pass(c, b0, n, true, false, false)) return -1;
-        HIPCHK(c, launch_eprecond_accumulate(n, c->g.nx, c->g.ny, c->d_Ws, c->d_Wr, c->d_gu_shot, c->d_grho_shot,
-                                             c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits, c->stream));
-        b0 += n;
+    for (int s : shots) {
+        L2 += c->shot_l2[s];
+        EN += c->shot_energy[s];
     }
-    HIPCHK(c, launch_chain_rule(c->g, c->g.nx, c->g.ny, c->d_gu, c->d_grho, c->d_vs, c->d_rho, c->p.dt,
-                                c->d_gvs_out, c->d_grho_out, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (l2) *l2 = L2;
     if (energy) *energy = EN;
     return 0;
@@ -686,7 +765,7 @@ extern "C" int dn_forward_fields(dn_ctx *c, int shot, int nsteps, float *vy, flo
     const int saved = c->shot_nt[shot];
     c->shot_nt[shot] = std::min(nsteps, saved);
     HIPCHK(c, hipMemcpy(c->d_shot_nt + shot, &c->shot_nt[shot], sizeof(int), hipMemcpyHostToDevice));
-    const int rc = dn_run_pass(c, shot, 1, false, false, false);
+    const int rc = dn_run_pass(c, std::vector<int>{shot}, false, false, false);
     c->shot_nt[shot] = saved;
     HIPCHK(c, hipMemcpy(c->d_shot_nt + shot, &saved, sizeof(int), hipMemcpyHostToDevice));
     if (rc) return rc;
@@ -704,5 +783,14 @@ extern "C" int dn_last_timing(dn_ctx *c, double *ms, long long *launches, long l
     if (ms) *ms = c->ms_steps;
     if (launches) *launches = c->launches;
     if (cells) *cells = c->cell_updates;
+    return 0;
+}
+
+extern "C" int dn_pass_timing(dn_ctx *c, int adjoint, double *ms, long long *launches, long long *cells)
+{
+    if (!c || adjoint < 0 || adjoint > 1) return -1;
+    if (ms) *ms = c->ms_pass[adjoint];
+    if (launches) *launches = c->launches_pass[adjoint];
+    if (cells) *cells = c->cells_pass[adjoint];
     return 0;
 }

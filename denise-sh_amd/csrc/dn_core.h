@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+struct dn_fwi_state;
+
 struct dn_ctx {
     dn_params p{};
     int K = 1;
@@ -51,14 +53,28 @@ struct dn_ctx {
     unsigned int *d_stage_max = nullptr;
     float *d_Ws = nullptr, *d_Wr = nullptr, *d_gu_shot = nullptr, *d_grho_shot = nullptr;
     double *d_part = nullptr;
+    int *d_shot_map = nullptr;
+    std::vector<int> cur_map;        // absolute shot of each batch slot in the last pass
+    std::vector<double> shot_l2, shot_energy;   // per-shot misfit sums of the last residual pass
     int cur_final = 0;
     // timing of the last pass set
     double ms_steps = 0.0;
     long long launches = 0, cell_updates = 0;
+    double ms_pass[2] = {0.0, 0.0};          // [forward, adjoint]
+    long long launches_pass[2] = {0, 0}, cells_pass[2] = {0, 0};
+    // FWI optimiser state (dn_fwi.cpp) and an optional gradient preconditioner
+    dn_fwi_state *fwi = nullptr;
+    int (*precond_hook)(dn_ctx *, float *grad_vs, float *grad_rho) = nullptr;
 };
 
 int dn_fail(dn_ctx *c, const char *fmt, ...);
 int dn_prepare_model(dn_ctx *c);
 int dn_ensure_batch(dn_ctx *c, int want, bool gradient);
-int dn_run_pass(dn_ctx *c, int shot0, int n, bool adjoint, bool store, bool record);
+int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record);
+// residual of the batch slots of the last forward pass: resid + per-shot l2/energy
+int dn_batch_residual(dn_ctx *c, const std::vector<int> &shots);
+int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate);
+int dn_forward_shots(dn_ctx *c, const std::vector<int> &shots, float *seis_host);
 int dn_nsnap(const dn_ctx *c);
+void dn_reset_timing(dn_ctx *c);
+void dn_fwi_destroy(dn_ctx *c);

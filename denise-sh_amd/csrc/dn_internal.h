@@ -62,8 +62,9 @@ struct StepArgs {
     const float *psx_in;  float *psx_out;   // psi_sxy_x [s][ny][2fw] (ping-pong)
     const float *psy_in;  float *psy_out;   // psi_syy_y [s][2fw][nx] (ping-pong)
     float *pvx, *pvy;                       // psi_vyx [s][ny][2fw], psi_vyy [s][2fw][nx]
-    // shot tables
-    const int *shot_nt;                     // [nshots] time steps of each shot
+    // shot tables (absolute shot index of batch slot s: shot_map[s])
+    const int *shot_map;
+    const int *shot_nt;                     // [all shots] time steps of each shot
     const int *src_off, *src_i, *src_j;     // sources of shot s: src_off[s]..src_off[s+1]-1
     const float *src_sig;                   // [nsrc_total][nt_max]
     int nt_max;
@@ -104,8 +105,8 @@ struct ModelPrepArgs {
 hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st);
 
 // per-shot residual (calc_res / calc_energy): one thread per trace
-hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn,
-                           const float *obs, float *resid, double *l2_energy /*[2]*/, float DT, hipStream_t st);
+hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+                           const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st);
 
 // energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)
 hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,

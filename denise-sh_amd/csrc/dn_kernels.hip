@@ -79,7 +79,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const int strip = tile % a.nstrips;
     const int seg = tile / a.nstrips;
     const Geo &g = a.g;
-    const int NTs = a.shot_nt[s];
+    const int S = a.shot_map[s];
+    const int NTs = a.shot_nt[S];
     const int nt = a.nt;
     if (nt > NTs) return;  // shot finished (uneven record lengths)
 
@@ -108,8 +109,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             slot = (sf - 1) / a.dtinv;
         }
     }
-    const int sb = ADJ ? 0 : a.src_off[s];
-    const int se = ADJ ? 0 : a.src_off[s + 1];
+    const int sb = ADJ ? 0 : a.src_off[S];
+    const int se = ADJ ? 0 : a.src_off[S + 1];
 
     const float *__restrict__ vy_in = a.vy_in + so;
     const float *__restrict__ sxy_in = a.sxy_in + so;
@@ -490,15 +491,16 @@ hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st)
 // calc_energy.c:53-61).  One thread per (shot, trace); per-trace FP64 partial
 // sums are written in place and reduced on the host in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn, const float *obs,
-                           float *resid, double *part, float DT)
+__global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+                           const float *syn, const float *obs, float *resid, double *part, float DT)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nshots * ntr) return;
-    const int s = t / ntr;
-    const int ns = shot_nt[s];
+    const int s = t / ntr, r = t % ntr;
+    const int S = shot_map[s];
+    const int ns = shot_nt[S];
     const float *sy = syn + (long long)t * nt_max;
-    const float *ob = obs + (long long)t * nt_max;
+    const float *ob = obs + ((long long)S * ntr + r) * nt_max;
     float *df = resid + (long long)t * nt_max;
     double l2 = 0.0, en = 0.0;
     float intseis = 0.0f;
@@ -515,12 +517,12 @@ __global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_nt, 
     part[2 * (long long)t + 1] = en;
 }
 
-hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_nt, const float *syn, const float *obs,
-                           float *resid, double *part, float DT, hipStream_t st)
+hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+                           const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st)
 {
     const int n = nshots * ntr;
-    hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, ntr, nt_max, shot_nt, syn, obs,
-                       resid, part, DT);
+    hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, ntr, nt_max, shot_map, shot_nt, syn,
+                       obs, resid, part, DT);
     return hipGetLastError();
 }
 

@@ -161,6 +161,16 @@ int dn_set_receivers(dn_ctx *ctx, int ntr, const int *rec_i, const int *rec_j);
 int dn_set_shots(dn_ctx *ctx, int nshots, int nt_max, const int *nt, const int *src_i, const int *src_j,
                  const float *signals);
 
+/* Shots with several simultaneous point forces (RUN_MULTIPLE_SHOTS = 0 or
+ * several sources per shot): the sources of shot s are src_off[s] ..
+ * src_off[s+1]-1, signals host [src_off[nshots]][nt_max]. */
+int dn_set_shots_multi(dn_ctx *ctx, int nshots, int nt_max, const int *nt, const int *src_off, const int *src_i,
+                       const int *src_j, const float *signals);
+
+/* Replaces the source signals (same layout as the last dn_set_shots*),
+ * e.g. after the per-stage low-pass filter of the workflow. */
+int dn_set_signals(dn_ctx *ctx, const float *signals);
+
 /* Observed data host [nshots][ntr][nt_max] (samples 1..nt[s] used). */
 int dn_set_observed(dn_ctx *ctx, const float *obs);
 
@@ -191,6 +201,94 @@ int dn_forward_fields(dn_ctx *ctx, int shot, int nsteps, float *vy, float *sxy, 
 /* Timing of the last dn_forward/dn_gradient: device milliseconds of the
  * time-step kernels and number of kernel launches. */
 int dn_last_timing(dn_ctx *ctx, double *ms_steps, long long *launches, long long *cell_updates);
+
+/* Same, split by pass: adjoint = 0 forward passes, 1 adjoint passes.  ms is
+ * measured with HIP events on the library's stream around each pass's
+ * time-step launches; launches counts fused step launches. */
+int dn_pass_timing(dn_ctx *ctx, int adjoint, double *ms, long long *launches, long long *cell_updates);
+
+/* ------------------------------------------------------------------------
+ * Drop-in file formats (host side)
+ * ---------------------------------------------------------------------- */
+typedef struct dn_par dn_par;
+typedef struct dn_workflow dn_workflow;
+
+/* read_par.c / read_par_json.c: a ".json" file name selects the JSON reader
+ * (denise.c:186-191), anything else the positional .inp reader.  Values are
+ * looked up by the reference's variable names (REFRECX/REFRECY for REFREC,
+ * FL as a list of L floats). */
+dn_par *dn_par_read(const char *path);
+void dn_par_free(dn_par *par);
+int dn_par_get_int(const dn_par *par, const char *key, int *out);
+int dn_par_get_float(const dn_par *par, const char *key, float *out);
+int dn_par_get_floats(const dn_par *par, const char *key, float *out, int max);   /* returns the count */
+int dn_par_get_string(const dn_par *par, const char *key, char *buf, int n);
+
+/* read_par_inv.c: workflow table; columns by the reference's variable names
+ * (PRO, TIME_FILT, FC_START, FC_END, ORDER, ..., SCALERHO). */
+dn_workflow *dn_workflow_read(const char *path);
+void dn_workflow_free(dn_workflow *w);
+int dn_workflow_nstages(const dn_workflow *w);
+int dn_workflow_get(const dn_workflow *w, int stage, const char *column, float *out);
+
+/* sources.c:36-77: srcpos [nsrc][8] = x, y, 0, tshift, fc, amp, azimuth, type
+ * (file columns x z y tshift fc amp azimuth type). */
+int dn_read_sources(const char *path, int max, float *srcpos, int *nsrc);
+/* receiver.c:28-74: grid indices, duplicates removed. */
+int dn_read_receivers(const char *path, float dh, float refrec_x, float refrec_y, int max, int *rec_i, int *rec_j,
+                      int *ntr);
+/* readmod.c / writemod.c: x-major float32 files <-> row-major [ny][nx]. */
+int dn_read_model_file(const char *path, int nx, int ny, float *out);
+int dn_write_model_file(const char *path, int nx, int ny, const float *in);
+int dn_read_model(const char *mfile, int nx, int ny, int L, float *vs, float *rho, float *qs);
+/* outseis_glob.c (SEIS_FORMAT 1) / inseis.c: SU traces, data [ntr][ns]. */
+int dn_write_su(const char *path, int ntr, int ns, const float *data, const int *rec_i, const int *rec_j, float dh,
+                float refrec_x, float refrec_y, float src_x, float src_y, float dt, int ndt);
+int dn_read_su(const char *path, int ntr, int ns, float *out);
+
+/* ------------------------------------------------------------------------
+ * FWI iteration on the device (denise.c:695-2018, line_search.c,
+ * calc_mat_change_test.c, calc_opt_step.c, LBFGS.c, PCG.c and the
+ * store/extract helpers).  Dot products and misfit sums are FP64; the
+ * L-BFGS history is kept in HBM.
+ * ---------------------------------------------------------------------- */
+typedef struct dn_fwi_opts {
+    int grad_method;      /* GRAD_METHOD: 1 PCG (Polak-Ribiere), 2 L-BFGS          */
+    int nlbfgs;           /* L-BFGS ring size (the reference fixes 20)             */
+    float eps_scale;      /* EPS_SCALE: first relative step length                 */
+    int stepmax;          /* STEPMAX                                               */
+    float scalefac;       /* SCALEFAC                                              */
+    int testshot_start, testshot_end, testshot_incr;   /* TESTSHOT_*, 1-based      */
+    float vs_lower, vs_upper, rho_lower, rho_upper;    /* VS/RHO LOWER/UPPERLIM    */
+    int inv_vs_iter, inv_rho_iter;                     /* INV_VS_ITER, INV_RHO_ITER */
+    float scalerho;       /* SCALERHO                                              */
+    int min_iter;         /* MIN_ITER                                              */
+    float pro;            /* PRO: relative misfit change that ends a stage          */
+} dn_fwi_opts;
+
+/* One line of the reference's misfit log (denise.c:1929-1933) plus extras. */
+typedef struct dn_fwi_record {
+    int stage, iter, iter_true;
+    float opteps_vs;      /* applied relative step (0 when no update)              */
+    float epst1[3];       /* trial steps epst1[1..3]                               */
+    float L2t[4];         /* normalised misfits L2t[1..4]                          */
+    float M2_vs, M2_rho;  /* model misfit against the starting model               */
+    float eps_u, eps_rho; /* absolute steps EPSILON_u, EPSILON_rho of the update   */
+    double l2, energy;    /* FP64 misfit and energy of the test shots              */
+    int step1, step3;     /* line-search flags                                     */
+    int test_forwards;    /* shot simulations spent in the line search             */
+    int next_stage;       /* 1: the abort criterion ended the stage                */
+} dn_fwi_record;
+
+/* Starts FWI on the current model, or a new stage (optimiser reset) when
+ * called again.  Needs model, receivers, shots and observed data. */
+int dn_fwi_start(dn_ctx *ctx, const dn_fwi_opts *opts);
+/* One iteration: gradient over all shots, direction, line search on the
+ * test shots, model update; fills rec. */
+int dn_fwi_iteration(dn_ctx *ctx, dn_fwi_record *rec);
+/* Current FWI model / last search direction, host [ny][nx]. */
+int dn_fwi_get_model(dn_ctx *ctx, float *vs, float *rho);
+int dn_fwi_get_direction(dn_ctx *ctx, float *grad_vs, float *grad_rho);
 
 #ifdef __cplusplus
 }

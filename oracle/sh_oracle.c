@@ -550,12 +550,37 @@ void F(orc_timedomain_filt)(float *data, int ntr, int ns, double dt_in, double f
  * Source injection is separate (orc_inject).  psi_syy_y is [1..2FW][1..NX]
  * (width NX), psi_sxy_x is [1..NY][1..2FW] (width 2FW); both 1-based and
  * flattened with (row-1)*width + (col-1).
+ * Each row is processed in three passes that keep every cell's operations
+ * and their order: (1) the stencil sums for the whole row, (2) the CPML
+ * corrections on the strip cells, in the reference's left, right, top,
+ * bottom order, (3) the update.  Passes 1 and 3 vectorise.
  * Pinned by: FP64 energy conservation of the discrete leapfrog scheme (no
- * CPML, no source), x-mirror symmetry, and the closed-form homogeneous
- * response in the tests.
+ * CPML, no source) and the travel time in a homogeneous medium (tests).
  * ---------------------------------------------------------------------- */
 #define PSI_Y(p, c, row, col) ((p)[(size_t)((row)-1) * (c)->nx + (size_t)((col)-1)])
 #define PSI_X(p, c, row, col) ((p)[(size_t)((row)-1) * 2 * (c)->fw + (size_t)((col)-1)])
+
+/* stencil sums of update_v for row j, columns 1..NX, into sx[1..NX], sy[1..NX] */
+static inline __attribute__((always_inline)) void v_sums(const orc_cfg *c, const int k, int j,
+                                                         const real *restrict syz, const real *restrict sxy,
+                                                         const real *restrict hc, real *restrict sx,
+                                                         real *restrict sy)
+{
+    const int NX = c->nx, W = WID(c);
+    const real *sr = &AT(sxy, c, j, 0);
+    int i, m;
+    for (i = 1; i <= NX; i++) {
+        real a = hc[1] * (sr[i] - sr[i - 1]);
+        real b = hc[1] * (AT(syz, c, j, i) - AT(syz, c, j - 1, i));
+        for (m = 2; m <= k; m++) {
+            a = a + hc[m] * (sr[i + m - 1] - sr[i - m]);
+            b = b + hc[(k == 6 && m == 6) ? 5 : m] * (AT(syz, c, j + m - 1, i) - AT(syz, c, j - m, i));
+        }
+        sx[i] = a;
+        sy[i] = b;
+    }
+    (void)W;
+}
 
 void F(orc_update_v)(const orc_cfg *c, int sw, real *vy, real *vyp1, const real *syz, const real *sxy,
                      const real *rho, const real *hc, const real *K_x, const real *a_x, const real *b_x,
@@ -566,16 +591,20 @@ void F(orc_update_v)(const orc_cfg *c, int sw, real *vy, real *vyp1, const real 
     int j;
 #pragma omp parallel for schedule(static)
     for (j = 1; j <= NY; j++) {
-        int i, m, h1;
-        real sxy_x, syy_y, b;
-        for (i = 1; i <= NX; i++) {
-            sxy_x = hc[1] * (AT(sxy, c, j, i) - AT(sxy, c, j, i - 1));
-            syy_y = hc[1] * (AT(syz, c, j, i) - AT(syz, c, j - 1, i));
-            for (m = 2; m <= k; m++) {
-                sxy_x = sxy_x + hc[m] * (AT(sxy, c, j, i + m - 1) - AT(sxy, c, j, i - m));
-                syy_y = syy_y + hc[(k == 6 && m == 6) ? 5 : m] * (AT(syz, c, j + m - 1, i) - AT(syz, c, j - m, i));
-            }
-            if (FW > 0) {
+        real sx[NX + 1], sy[NX + 1];
+        int i, h1;
+        switch (k) {
+        case 1: v_sums(c, 1, j, syz, sxy, hc, sx, sy); break;
+        case 2: v_sums(c, 2, j, syz, sxy, hc, sx, sy); break;
+        case 3: v_sums(c, 3, j, syz, sxy, hc, sx, sy); break;
+        case 4: v_sums(c, 4, j, syz, sxy, hc, sx, sy); break;
+        case 5: v_sums(c, 5, j, syz, sxy, hc, sx, sy); break;
+        default: v_sums(c, 6, j, syz, sxy, hc, sx, sy); break;
+        }
+        if (FW > 0) {
+            for (i = 1; i <= NX; i++) {
+                if (!(i <= FW || i >= NX - FW + 1 || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
+                real sxy_x = sx[i], syy_y = sy[i];
                 if (i <= FW) {
                     PSI_X(psi_sxy_x, c, j, i) = b_x[i] * PSI_X(psi_sxy_x, c, j, i) + a_x[i] * sxy_x;
                     sxy_x = sxy_x / K_x[i] + PSI_X(psi_sxy_x, c, j, i);
@@ -594,12 +623,21 @@ void F(orc_update_v)(const orc_cfg *c, int sw, real *vy, real *vyp1, const real 
                     PSI_Y(psi_syy_y, c, h1, i) = b_y[h1] * PSI_Y(psi_syy_y, c, h1, i) + a_y[h1] * syy_y;
                     syy_y = syy_y / K_y[h1] + PSI_Y(psi_syy_y, c, h1, i);
                 }
+                sx[i] = sxy_x;
+                sy[i] = syy_y;
             }
-            b = 1.0 / AT(rho, c, j, i);
-            if (AT(rho, c, j, i) < 1e-20) b = 0.0;
-            if (sw == 0) AT(vyp1, c, j, i) = b * (sxy_x + syy_y) / DH;
-            if (sw == 1) AT(vyp1, c, j, i) = AT(vy, c, j, i);
-            AT(vy, c, j, i) += b * DT * (sxy_x + syy_y) / DH;
+        }
+        {
+            real *restrict v = &AT(vy, c, j, 0);
+            real *restrict v1 = &AT(vyp1, c, j, 0);
+            const real *restrict rr = &AT(rho, c, j, 0);
+            for (i = 1; i <= NX; i++) {
+                real b = 1.0 / rr[i];
+                if (rr[i] < 1e-20) b = 0.0;
+                if (sw == 0) v1[i] = b * (sx[i] + sy[i]) / DH;
+                if (sw == 1) v1[i] = v[i];
+                v[i] += b * DT * (sx[i] + sy[i]) / DH;
+            }
         }
     }
 }
@@ -609,6 +647,71 @@ void F(orc_inject)(const orc_cfg *c, real *vy, int n, const int *ii, const int *
 {
     int l;
     for (l = 0; l < n; l++) AT(vy, c, jj[l], ii[l]) += val[l];
+}
+
+/* stencil sums of the stress updates for row j: vyx, vyy (before *dhi) */
+static inline __attribute__((always_inline)) void s_sums(const orc_cfg *c, const int k, int j,
+                                                         const real *restrict vy, const real *restrict hc,
+                                                         real *restrict gx, real *restrict gy)
+{
+    const int NX = c->nx;
+    const real *vr = &AT(vy, c, j, 0);
+    int i, m;
+    for (i = 1; i <= NX; i++) {
+        real a = hc[1] * (vr[i + 1] - vr[i]);
+        real b = hc[1] * (AT(vy, c, j + 1, i) - AT(vy, c, j, i));
+        for (m = 2; m <= k; m++) {
+            a = a + hc[m] * (vr[i + m] - vr[i - m + 1]);
+            b = b + hc[m] * (AT(vy, c, j + m, i) - AT(vy, c, j - m + 1, i));
+        }
+        gx[i] = a;
+        gy[i] = b;
+    }
+}
+
+#define S_SUMS_DISPATCH                                                                                         \
+    switch (k) {                                                                                                \
+    case 1: s_sums(c, 1, j, vy, hc, gx, gy); break;                                                             \
+    case 2: s_sums(c, 2, j, vy, hc, gx, gy); break;                                                             \
+    case 3: s_sums(c, 3, j, vy, hc, gx, gy); break;                                                             \
+    case 4: s_sums(c, 4, j, vy, hc, gx, gy); break;                                                             \
+    case 5: s_sums(c, 5, j, vy, hc, gx, gy); break;                                                             \
+    default: s_sums(c, 6, j, vy, hc, gx, gy); break;                                                            \
+    }
+
+/* CPML of the stress updates on the strip cells of row j (reference order).
+ * top_full selects the full-grid y profile on top (Q2, viscoelastic). */
+static inline void s_cpml_row(const orc_cfg *c, int j, real *gx, real *gy, const real *K_x_half,
+                              const real *a_x_half, const real *b_x_half, const real *Kt, const real *at,
+                              const real *bt, const real *K_y_half, const real *a_y_half, const real *b_y_half,
+                              real *psi_vyy, real *psi_vyx)
+{
+    const int NX = c->nx, NY = c->ny, FW = c->fw;
+    int i, h1;
+    for (i = 1; i <= NX; i++) {
+        if (!(i <= FW || i >= NX - FW + 1 || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
+        real vyx = gx[i], vyy = gy[i];
+        if (i <= FW) {
+            PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
+            vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
+        }
+        if (i >= NX - FW + 1) {
+            h1 = i - NX + 2 * FW;
+            PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
+            vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
+        }
+        if ((!c->free_surf) && (j <= FW)) {
+            PSI_Y(psi_vyy, c, j, i) = bt[j] * PSI_Y(psi_vyy, c, j, i) + at[j] * vyy;
+            vyy = vyy / Kt[j] + PSI_Y(psi_vyy, c, j, i);
+        }
+        if (j >= NY - FW + 1) {
+            h1 = j - NY + 2 * FW;
+            PSI_Y(psi_vyy, c, h1, i) = b_y_half[h1] * PSI_Y(psi_vyy, c, h1, i) + a_y_half[h1] * vyy;
+            vyy = vyy / K_y_half[h1] + PSI_Y(psi_vyy, c, h1, i);
+        }
+        gx[i] = vyx;
+        gy[i] = vyy;
+    }
 }
 
 /* ------------------------------------------------------------------------
@@ -625,45 +728,34 @@ void F(orc_update_s_elastic)(const orc_cfg *c, const real *vy, real *uy, real *u
 {
     const int NX = c->nx, NY = c->ny, FW = c->fw, k = c->fdorder / 2;
     const real DT = (real)c->dt, DH = (real)c->dh;
-    real dhi = DT / DH;
+    const real dhi = DT / DH;
     int j;
 #pragma omp parallel for schedule(static)
     for (j = 1; j <= NY; j++) {
-        int i, m, h1;
-        real vyx, vyy;
+        real gx[NX + 1], gy[NX + 1];
+        int i;
+        S_SUMS_DISPATCH
         for (i = 1; i <= NX; i++) {
-            vyx = hc[1] * (AT(vy, c, j, i + 1) - AT(vy, c, j, i));
-            vyy = hc[1] * (AT(vy, c, j + 1, i) - AT(vy, c, j, i));
-            for (m = 2; m <= k; m++) {
-                vyx = vyx + hc[m] * (AT(vy, c, j, i + m) - AT(vy, c, j, i - m + 1));
-                vyy = vyy + hc[m] * (AT(vy, c, j + m, i) - AT(vy, c, j - m + 1, i));
-            }
-            vyx = vyx * dhi;
-            vyy = vyy * dhi;
-            if (FW > 0) {
-                if (i <= FW) {
-                    PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
-                    vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
-                }
-                if (i >= NX - FW + 1) {
-                    h1 = i - NX + 2 * FW;
-                    PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
-                    vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
-                }
-                if ((!c->free_surf) && (j <= FW)) {
-                    PSI_Y(psi_vyy, c, j, i) = b_y_half[j] * PSI_Y(psi_vyy, c, j, i) + a_y_half[j] * vyy;
-                    vyy = vyy / K_y_half[j] + PSI_Y(psi_vyy, c, j, i);
-                }
-                if (j >= NY - FW + 1) {
-                    h1 = j - NY + 2 * FW;
-                    PSI_Y(psi_vyy, c, h1, i) = b_y_half[h1] * PSI_Y(psi_vyy, c, h1, i) + a_y_half[h1] * vyy;
-                    vyy = vyy / K_y_half[h1] + PSI_Y(psi_vyy, c, h1, i);
+            gx[i] = gx[i] * dhi;
+            gy[i] = gy[i] * dhi;
+        }
+        if (FW > 0)
+            s_cpml_row(c, j, gx, gy, K_x_half, a_x_half, b_x_half, K_y_half, a_y_half, b_y_half, K_y_half,
+                       a_y_half, b_y_half, psi_vyy, psi_vyx);
+        {
+            real *restrict sx = &AT(sxy, c, j, 0), *restrict sz = &AT(syz, c, j, 0);
+            const real *restrict ui = &AT(uip, c, j, 0), *restrict uj = &AT(ujp, c, j, 0);
+            if (uy) {
+                real *restrict u1 = &AT(uy, c, j, 0), *restrict u2 = &AT(uyx, c, j, 0);
+                for (i = 1; i <= NX; i++) {
+                    u1[i] = gy[i];
+                    u2[i] = gx[i];
                 }
             }
-            if (uy) AT(uy, c, j, i) = vyy;
-            if (uyx) AT(uyx, c, j, i) = vyx;
-            AT(sxy, c, j, i) += AT(uip, c, j, i) * vyx;
-            AT(syz, c, j, i) += AT(ujp, c, j, i) * vyy;
+            for (i = 1; i <= NX; i++) {
+                sx[i] += ui[i] * gx[i];
+                sz[i] += uj[i] * gy[i];
+            }
         }
     }
 }
@@ -686,60 +778,62 @@ void F(orc_update_s_visc)(const orc_cfg *c, const real *vy, real *uy, real *uyx,
     const int NX = c->nx, NY = c->ny, FW = c->fw, k = c->fdorder / 2, L = c->L;
     const real DT = (real)c->dt, DH = (real)c->dh;
     const size_t plane = F(orc_padded_size)(c);
-    real dhi = 1.0 / DH, dthalbe = DT / 2.0;
+    const real dhi = 1.0 / DH, dthalbe = DT / 2.0;
     int j;
 #pragma omp parallel for schedule(static)
     for (j = 1; j <= NY; j++) {
-        int i, m, h1, l;
-        real vyx, vyy, sumr, sumq;
+        real gx[NX + 1], gy[NX + 1];
+        int i, l;
+        S_SUMS_DISPATCH
         for (i = 1; i <= NX; i++) {
-            vyx = hc[1] * (AT(vy, c, j, i + 1) - AT(vy, c, j, i));
-            vyy = hc[1] * (AT(vy, c, j + 1, i) - AT(vy, c, j, i));
-            for (m = 2; m <= k; m++) {
-                vyx = vyx + hc[m] * (AT(vy, c, j, i + m) - AT(vy, c, j, i - m + 1));
-                vyy = vyy + hc[m] * (AT(vy, c, j + m, i) - AT(vy, c, j - m + 1, i));
-            }
-            vyx = vyx * dhi;
-            vyy = vyy * dhi;
-            if (FW > 0) {
-                if (i <= FW) {
-                    PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
-                    vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
-                }
-                if (i >= NX - FW + 1) {
-                    h1 = i - NX + 2 * FW;
-                    PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
-                    vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
-                }
-                if ((!c->free_surf) && (j <= FW)) {
-                    PSI_Y(psi_vyy, c, j, i) = b_y[j] * PSI_Y(psi_vyy, c, j, i) + a_y[j] * vyy;
-                    vyy = vyy / K_y[j] + PSI_Y(psi_vyy, c, j, i);
-                }
-                if (j >= NY - FW + 1) {
-                    h1 = j - NY + 2 * FW;
-                    PSI_Y(psi_vyy, c, h1, i) = b_y_half[h1] * PSI_Y(psi_vyy, c, h1, i) + a_y_half[h1] * vyy;
-                    vyy = vyy / K_y_half[h1] + PSI_Y(psi_vyy, c, h1, i);
+            gx[i] = gx[i] * dhi;
+            gy[i] = gy[i] * dhi;
+        }
+        if (FW > 0)
+            s_cpml_row(c, j, gx, gy, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y, K_y_half, a_y_half, b_y_half,
+                       psi_vyy, psi_vyx);
+        {
+            real *restrict sx = &AT(sxy, c, j, 0), *restrict sz = &AT(syz, c, j, 0);
+            const real *restrict fi = &AT(fipjp, c, j, 0), *restrict ff = &AT(f, c, j, 0);
+            if (uy) {
+                real *restrict u1 = &AT(uy, c, j, 0), *restrict u2 = &AT(uyx, c, j, 0);
+                for (i = 1; i <= NX; i++) {
+                    u1[i] = gy[i];
+                    u2[i] = gx[i];
                 }
             }
-            sumr = sumq = 0.0;
+            real sr[NX + 1], sq[NX + 1];
+            for (i = 1; i <= NX; i++) sr[i] = sq[i] = 0.0;
             for (l = 0; l < L; l++) {
-                sumr += AT(r + (size_t)l * plane, c, j, i);
-                sumq += AT(q + (size_t)l * plane, c, j, i);
+                const real *restrict rl = &AT(r + (size_t)l * plane, c, j, 0);
+                const real *restrict ql = &AT(q + (size_t)l * plane, c, j, 0);
+                for (i = 1; i <= NX; i++) {
+                    sr[i] += rl[i];
+                    sq[i] += ql[i];
+                }
             }
-            AT(sxy, c, j, i) += (AT(fipjp, c, j, i) * vyx) + (dthalbe * sumr);
-            AT(syz, c, j, i) += (AT(f, c, j, i) * vyy) + (dthalbe * sumq);
-            sumr = sumq = 0.0;
+            for (i = 1; i <= NX; i++) {
+                sx[i] += (fi[i] * gx[i]) + (dthalbe * sr[i]);
+                sz[i] += (ff[i] * gy[i]) + (dthalbe * sq[i]);
+                sr[i] = sq[i] = 0.0;
+            }
             for (l = 0; l < L; l++) {
-                real *rl = r + (size_t)l * plane, *ql = q + (size_t)l * plane;
-                AT(rl, c, j, i) = bip[l] * (AT(rl, c, j, i) * cip[l] - (AT(dip + (size_t)l * plane, c, j, i) * vyx));
-                AT(ql, c, j, i) = bjm[l] * (AT(ql, c, j, i) * cjm[l] - (AT(d + (size_t)l * plane, c, j, i) * vyy));
-                sumr += AT(rl, c, j, i);
-                sumq += AT(ql, c, j, i);
+                real *restrict rl = &AT(r + (size_t)l * plane, c, j, 0);
+                real *restrict ql = &AT(q + (size_t)l * plane, c, j, 0);
+                const real *restrict dl = &AT(dip + (size_t)l * plane, c, j, 0);
+                const real *restrict el = &AT(d + (size_t)l * plane, c, j, 0);
+                const real bi = bip[l], bj = bjm[l], ci = cip[l], cj = cjm[l];
+                for (i = 1; i <= NX; i++) {
+                    rl[i] = bi * (rl[i] * ci - (dl[i] * gx[i]));
+                    ql[i] = bj * (ql[i] * cj - (el[i] * gy[i]));
+                    sr[i] += rl[i];
+                    sq[i] += ql[i];
+                }
             }
-            if (uy) AT(uy, c, j, i) = vyy;
-            if (uyx) AT(uyx, c, j, i) = vyx;
-            AT(sxy, c, j, i) += (dthalbe * sumr);
-            AT(syz, c, j, i) += (dthalbe * sumq);
+            for (i = 1; i <= NX; i++) {
+                sx[i] += (dthalbe * sr[i]);
+                sz[i] += (dthalbe * sq[i]);
+            }
         }
     }
 }
