@@ -22,7 +22,7 @@ EXPORTS = [
     "dn_par_read", "dn_par_free", "dn_par_get_int", "dn_par_get_float", "dn_par_get_floats", "dn_par_get_string",
     "dn_workflow_read", "dn_workflow_free", "dn_workflow_nstages", "dn_workflow_get", "dn_read_sources",
     "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
-    "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction",
+    "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction", "dn_checkfd",
 ]
 
 
@@ -322,3 +322,28 @@ def timedomain_filt(data, dt, fc, order, method):
     if lib().dn_timedomain_filt(_f(d), d.shape[0], d.shape[1], dt, fc, order, method) != 0:
         raise DeniseError("dn_timedomain_filt failed")
     return d
+
+
+class CheckFD(C.Structure):
+    _fields_ = [("cmin", C.c_float), ("cmax", C.c_float), ("dtstab", C.c_float), ("dhstab", C.c_float),
+                ("dt_unstable", C.c_int), ("dh_dispersive", C.c_int), ("fw_too_large", C.c_int),
+                ("fw_small", C.c_int)]
+
+
+def checkfd(vs, rho, qs, fdorder, maxrelerror, fl, dt, dh, ts, fw):
+    """checkfd_ssg_visc.c / checkfd_ssg_elastic.c: returns (status, dict)."""
+    vs = np.ascontiguousarray(vs, np.float32)
+    rho = np.ascontiguousarray(rho, np.float32)
+    L = len(fl)
+    flv = np.ascontiguousarray(fl if L else [0.0], np.float32)
+    qsv = np.ascontiguousarray(qs if qs is not None else np.zeros(1), np.float32)
+    ny, nx = vs.shape
+    f = C.c_float
+    fn = lib().dn_checkfd
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), f, f, f, C.c_int,
+                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(CheckFD)]
+    out = CheckFD()
+    rc = fn(nx, ny, fdorder, maxrelerror, L, _f(flv), dt, dh, ts, fw, _f(vs), _f(rho), _f(qsv), C.byref(out))
+    if rc < 0:
+        raise DeniseError("dn_checkfd: bad arguments")
+    return rc, {k: getattr(out, k) for k, _ in CheckFD._fields_}

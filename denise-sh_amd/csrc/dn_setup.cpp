@@ -309,3 +309,63 @@ extern "C" int dn_timedomain_filt(float *data, int ntr, int ns, float dt, float 
     dn::timedomain_filt(data, ntr, ns, dt, fc, order, method);
     return 0;
 }
+
+// checkfd_ssg_visc.c:14-142 / checkfd_ssg_elastic.c:14-130 with INVMAT1 = 1
+// (the model holds vs, readmod.c:41-60): phase-velocity extremes, the
+// dispersion and stability limits, and the absorbing-frame width check.
+// Returns 1 where the reference calls err() (DT > dtstab, FW wider than the
+// grid), 0 otherwise; warnings are reported through the flags.
+extern "C" int dn_checkfd(int nx, int ny, int fdorder, int maxrelerror, int L, const float *fl, float dt, float dh,
+                          float ts_src, int fw, const float *vs, const float *rho, const float *qs,
+                          dn_checkfd_result *out)
+{
+    if (nx < 1 || ny < 1 || !vs || !rho || !out || L < 0 || L > dn::kSetupMaxL || (L > 0 && (!fl || !qs)))
+        return -1;
+    float hc[7];
+    if (dn::fd_coefficients(fdorder, maxrelerror, hc) != 0) return -1;
+    float eta[dn::kSetupMaxL + 1] = {0};
+    for (int l = 1; l <= L; ++l) {
+        const float pts = 1.0 / (2.0 * dn::kPi * fl[l - 1]);   // readmod.c:32-33
+        eta[l] = dt / pts;
+    }
+    const float ws = 2.0 * dn::kPi / ts_src;
+    float cmax = 0.0f, cmin = 1e9f;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j) {
+            const size_t k = (size_t)(j - 1) * nx + (size_t)(i - 1);
+            float c;
+            if (L > 0) {
+                const float taus = 2.0 / qs[k];                  // readmod.c:99
+                float sumu = 0.0f;
+                for (int l = 1; l <= L; ++l) {
+                    const float t = dt / eta[l];
+                    sumu = sumu + ((ws * ws * t * t * taus) / (1.0 + ws * ws * t * t));
+                }
+                const float pu_ref = rho[k] * vs[k] * vs[k];
+                c = std::sqrt(pu_ref / (rho[k] * (1.0 + sumu)));
+                if (cmin > c) cmin = c;
+                c = std::sqrt(pu_ref * (1.0 + L * taus) / (rho[k] * (1.0 + sumu)));
+                if (cmax < c) cmax = c;
+            } else {
+                c = vs[k];
+                if (cmax < c) cmax = c;
+                if (cmin > c) cmin = c;
+            }
+        }
+    const float fmax = 2.0 / ts_src;
+    const float dhstab = cmin / (hc[0] * fmax);
+    // C's fabs takes and returns double: the sum is formed in double
+    const float gamma = std::fabs((double)hc[1]) + std::fabs((double)hc[2]) + std::fabs((double)hc[3]) +
+                        std::fabs((double)hc[4]) + std::fabs((double)hc[5]) + std::fabs((double)hc[6]);
+    const float dtstab = dh / (std::sqrt(2.0) * gamma * cmax);
+    out->cmin = cmin;
+    out->cmax = cmax;
+    out->dtstab = dtstab;
+    out->dhstab = dhstab;
+    out->dh_dispersive = dh > dhstab;
+    out->dt_unstable = dt > dtstab;
+    out->fw_too_large = (fw > nx) || (fw > ny);
+    out->fw_small = fw < 10;
+    return (out->dt_unstable || out->fw_too_large) ? 1 : 0;
+}
+

@@ -115,6 +115,18 @@ int dn_wavelet(int quellart, int nt, float dt, float tshift, float fc, float amp
  * place; method 1 lowpass, 2 highpass. */
 int dn_timedomain_filt(float *data, int ntr, int ns, float dt, float fc, int order, int method);
 
+/* checkfd_ssg_visc.c:14-142 (L > 0) / checkfd_ssg_elastic.c:14-130 (L = 0),
+ * INVMAT1 = 1: vs, rho, qs host [ny][nx]; ts_src = TS (source duration).
+ * Returns 1 where the reference aborts (DT > dtstab or FW > NX, NY),
+ * 0 otherwise, -1 on bad arguments. */
+typedef struct {
+    float cmin, cmax;     /* S-wave phase velocity at f = 0 / f = inf */
+    float dtstab, dhstab; /* stability limit for DT, recommended DH */
+    int dt_unstable, dh_dispersive, fw_too_large, fw_small;
+} dn_checkfd_result;
+int dn_checkfd(int nx, int ny, int fdorder, int maxrelerror, int L, const float *fl, float dt, float dh, float ts_src,
+               int fw, const float *vs, const float *rho, const float *qs, dn_checkfd_result *out);
+
 /* ------------------------------------------------------------------------
  * Layer 2: handle API (fast path).  Not in the reference: it replaces the
  * per-step call sequence of denise.c:1179-1346 / 1493-1651 and the shot

@@ -515,3 +515,34 @@ def test_seismo_ssg_and_psource_host_semantics():
     L.psource(5, sxx.ptr, syy.ptr, src.ptr, sig.ptr, 1)
     amp = np.float32((np.float32(36.0) - np.float32(16.0)) / np.float32(1e-3))
     assert sxx[2, 3] == amp and syy[2, 3] == amp
+
+
+@pytest.mark.parametrize("L", [0, 2])
+def test_checkfd_equals_oracle_and_flags(L):
+    dn = product()
+    fl = [6.25, 50.0][:L]
+    c = cfg(nx=40, ny=30, fdorder=8, maxrelerror=1, fw=10, dt=1e-4, dh=0.2, L=L, fl=fl)
+    vs, rho, _ = model(40, 30, 0.2, seed=3)
+    qs = np.full((30, 40), 20.0, np.float32) if L else None
+    rc, r = dn.checkfd(vs, rho, qs, 8, 1, fl, 1e-4, 0.2, 0.04, 10)
+    assert rc == 0 and not r["dt_unstable"] and not r["fw_too_large"]
+    assert np.float32(r["dtstab"]) == np.float32(O32.dtstab(c, vs, rho, qs, 0.04))
+    if L == 0:
+        assert r["cmax"] == vs.max() and r["cmin"] == vs.min()
+    else:
+        # homogeneous closed form of the relaxed/unrelaxed velocities (GSLS, tau = 2/Q)
+        rc, h = dn.checkfd(np.full((4, 5), 300.0), np.full((4, 5), 1800.0), np.full((4, 5), 20.0), 8, 1, fl,
+                           1e-4, 0.2, 0.04, 1)
+        ws = 2 * np.pi / 0.04
+        tl = 1 / (2 * np.pi * np.array(fl))
+        sumu = np.sum(ws ** 2 * tl ** 2 * 0.1 / (1 + ws ** 2 * tl ** 2))
+        np.testing.assert_allclose(h["cmin"], 300 / np.sqrt(1 + sumu), rtol=1e-5)
+        np.testing.assert_allclose(h["cmax"], 300 * np.sqrt((1 + L * 0.1) / (1 + sumu)), rtol=1e-5)
+        assert h["fw_small"] == 1
+    # DT above the limit and a frame wider than the grid abort (err() in the reference)
+    rc, r2 = dn.checkfd(vs, rho, qs, 8, 1, fl, 1.01 * r["dtstab"], 0.2, 0.04, 10)
+    assert rc == 1 and r2["dt_unstable"]
+    rc, r3 = dn.checkfd(vs, rho, qs, 8, 1, fl, 1e-4, 0.2, 0.04, 31)
+    assert rc == 1 and r3["fw_too_large"]
+    hc = dn.holbergcoeff(8, 1)
+    np.testing.assert_allclose(r["dhstab"], r["cmin"] / (hc[0] * 2 / 0.04), rtol=1e-6)
