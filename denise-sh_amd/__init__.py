@@ -23,6 +23,7 @@ EXPORTS = [
     "dn_workflow_read", "dn_workflow_free", "dn_workflow_nstages", "dn_workflow_get", "dn_read_sources",
     "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
     "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction", "dn_checkfd",
+    "dn_set_exec", "dn_autotune",
 ]
 
 
@@ -70,6 +71,10 @@ def lib():
         L.dn_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_longlong)]
         L.dn_set_globals.argtypes = [C.POINTER(Globals)]
+        L.dn_set_exec.argtypes = [C.c_void_p, C.c_int]
+        L.dn_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip]
+        L.dn_set_shots_multi.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, ip, fp]
+        L.dn_set_signals.argtypes = [C.c_void_p, fp]
         _lib = L
     return _lib
 
@@ -148,10 +153,31 @@ class Context:
         self._chk(lib().dn_set_shots(self._h, ns, ntm, _i(nt), _i(si), _i(sj), _f(sig)))
         self.nshots, self.nt_max = ns, ntm
 
+    def set_shots_multi(self, src_off, src_i, src_j, signals, nt):
+        """Shots with several sources: those of shot s are src_off[s]..src_off[s+1]-1."""
+        sig = np.ascontiguousarray(signals, np.float32)
+        off = np.ascontiguousarray(src_off, np.int32)
+        nt = np.ascontiguousarray(nt, np.int32)
+        ns, ntm = len(nt), sig.shape[1]
+        self._chk(lib().dn_set_shots_multi(self._h, ns, ntm, _i(nt), _i(off), _i(np.ascontiguousarray(src_i, np.int32)),
+                                           _i(np.ascontiguousarray(src_j, np.int32)), _f(sig)))
+        self.nshots, self.nt_max = ns, ntm
+
+    def set_signals(self, signals):
+        self._chk(lib().dn_set_signals(self._h, _f(np.ascontiguousarray(signals, np.float32))))
+
     def set_observed(self, obs):
         obs = np.ascontiguousarray(obs, np.float32)
         assert obs.shape == (self.nshots, self.ntr, self.nt_max)
         self._chk(lib().dn_set_observed(self._h, _f(obs)))
+
+    def set_exec(self, graph_steps=-1):
+        self._chk(lib().dn_set_exec(self._h, graph_steps))
+
+    def autotune(self, nshots, nsteps=40):
+        rows, graph = C.c_int(), C.c_int()
+        self._chk(lib().dn_autotune(self._h, nshots, nsteps, C.byref(rows), C.byref(graph)))
+        return rows.value, graph.value
 
     def set_batch(self, max_shots=0, rows_per_segment=0):
         self._chk(lib().dn_set_batch(self._h, max_shots, rows_per_segment))

@@ -172,8 +172,15 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
     return c;
 }
 
+void dn_drop_graphs(dn_ctx *c)
+{
+    for (auto &gr : c->graphs) (void)hipGraphExecDestroy(gr.exec);
+    c->graphs.clear();
+}
+
 static void free_batch(dn_ctx *c)
 {
+    dn_drop_graphs(c);
     for (int k = 0; k < 2; ++k) {
         dfree_t(c->d_vy[k]);
         dfree_t(c->d_sxy[k]);
@@ -204,6 +211,7 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_rec_row); dfree_t(c->d_rec_tab);
     dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
     dfree_t(c->d_obs);
+    dfree_t(c->d_ntbase);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -399,6 +407,74 @@ extern "C" int dn_set_batch(dn_ctx *c, int max_shots, int rows_per_segment)
     return 0;
 }
 
+extern "C" int dn_set_exec(dn_ctx *c, int graph_steps)
+{
+    if (!c) return -1;
+    if (graph_steps < -1) return dn_fail(c, "dn_set_exec: graph_steps must be >= -1");
+    c->graph_steps = graph_steps;
+    dn_drop_graphs(c);
+    return 0;
+}
+
+extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int *graph_out)
+{
+    if (!c) return -1;
+    if (nshots < 1 || nshots > c->nshots || nsteps < 1) return dn_fail(c, "dn_autotune: bad arguments");
+    if (!c->have_C) return dn_fail(c, "dn_autotune: model not set");
+    if (dn_ensure_batch(c, nshots, false)) return -1;
+    const int n = std::min(nshots, c->batch_cap);
+    std::vector<int> shots(n);
+    for (int s = 0; s < n; ++s) shots[s] = s;
+    const int saved_rows = c->rows_req, saved_graph = c->graph_steps;
+    const int cand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 20};
+    auto time_it = [&](int rows, int graph, double *ms) -> int {
+        c->rows_req = rows;
+        c->graph_steps = graph;
+        dn_drop_graphs(c);
+        c->nt_limit = nsteps;
+        dn_reset_timing(c);
+        int rc = dn_run_pass(c, shots, false, false, false);   // first pass: warm-up and capture
+        if (rc == 0) {
+            dn_reset_timing(c);
+            rc = dn_run_pass(c, shots, false, false, false);
+        }
+        c->nt_limit = 0;
+        *ms = c->ms_steps;
+        return rc;
+    };
+    double best = 1e300;
+    int best_rows = 0;
+    std::vector<int> tried;
+    for (int R : cand) {
+        const int Rr = std::min(R, c->g.ny);
+        if (Rr < std::min(2 * c->K, c->g.ny) || std::find(tried.begin(), tried.end(), Rr) != tried.end()) continue;
+        tried.push_back(Rr);
+        double ms = 0.0;
+        if (time_it(Rr, 0, &ms)) {
+            c->rows_req = saved_rows;
+            c->graph_steps = saved_graph;
+            return -1;
+        }
+        if (ms < best) {
+            best = ms;
+            best_rows = Rr;
+        }
+    }
+    double ms_graph = 0.0;
+    if (time_it(best_rows, -1, &ms_graph)) {
+        c->rows_req = saved_rows;
+        c->graph_steps = saved_graph;
+        return -1;
+    }
+    c->rows_req = best_rows;
+    c->graph_steps = (ms_graph <= best) ? -1 : 0;
+    dn_drop_graphs(c);
+    dn_reset_timing(c);
+    if (rows_out) *rows_out = best_rows;
+    if (graph_out) *graph_out = c->graph_steps;
+    return 0;
+}
+
 int dn_nsnap(const dn_ctx *c) { return (c->nt_max + c->p.dtinv - 1) / c->p.dtinv; }
 
 // bytes of per-shot device state for a gradient batch
@@ -558,14 +634,93 @@ int dn_zero_state(dn_ctx *c, int n)
     return 0;
 }
 
-// One pass (forward or adjoint) over shots [shot0, shot0+n) resident in batch
-// slots 0..n-1.  store: forward snapshots + Ws; seis: forward seismograms.
+// Enqueue `count` time steps starting at step nt0 (absolute, or an offset to
+// the device counter nt_base inside a captured chunk).  cur selects the
+// ping-pong half holding the current fields and is advanced per step.
+static int enqueue_steps(dn_ctx *c, StepArgs &a, int n, bool adjoint, bool pack_mode, int nsnap, int nt0, int count,
+                         const int *nt_base, int &cur)
+{
+    const int kind = c->p.snap_precision;
+    for (int k = 0; k < count; ++k) {
+        const int nt = nt0 + k;
+        a.nt = nt;
+        a.nt_base = nt_base;
+        a.vy_in = c->d_vy[cur];
+        a.sxy_in = c->d_sxy[cur];
+        a.syz_in = c->d_syz[cur];
+        a.vy_out = c->d_vy[cur ^ 1];
+        a.sxy_out = c->d_sxy[cur ^ 1];
+        a.syz_out = c->d_syz[cur ^ 1];
+        a.psx_in = c->d_psx[cur];
+        a.psx_out = c->d_psx[cur ^ 1];
+        a.psy_in = c->d_psy[cur];
+        a.psy_out = c->d_psy[cur ^ 1];
+        // inside a chunk the counter is a multiple of dtinv, so the store
+        // pattern depends on the offset only
+        const bool pack = pack_mode && ((nt - 1) % c->p.dtinv) == 0;
+        if (pack) HIPCHK(c, launch_fill_u32(c->d_stage_max, (size_t)n * 3, 0u, c->stream));
+        HIPCHK(c, launch_step(a, c->K, c->p.L > 0, adjoint, c->stream));
+        if (pack)
+            HIPCHK(c, launch_snap_pack(n, c->g.nx, c->g.ny, c->d_stage, c->d_stage_max, c->d_snap16, c->d_snap_scale,
+                                       a.snap_shot_stride, nsnap, nt_base, nt, c->p.dtinv, kind, c->stream));
+        cur ^= 1;
+    }
+    return 0;
+}
+
+static int chunk_steps(const dn_ctx *c)
+{
+    if (c->graph_steps == 0) return 0;
+    const int target = c->graph_steps > 0 ? c->graph_steps : 64;
+    const int dt = c->p.dtinv;
+    const int l = (dt % 2 == 0) ? dt : 2 * dt;   // lcm(2, dtinv): ping-pong parity and store pattern repeat
+    return l * std::max(1, (target + l - 1) / l);
+}
+
+// Captured chunk for this batch shape (built on first use): `steps` time
+// steps reading the device counter, followed by the counter advance.
+static int get_graph(dn_ctx *c, StepArgs a, int n, bool adjoint, bool store, bool record, bool pack_mode,
+                     int nsnap, int steps, hipGraphExec_t *out)
+{
+    const int kind = c->p.snap_precision;
+    for (auto &gr : c->graphs)
+        if (gr.n == n && gr.R == a.R && gr.steps == steps && gr.kind == kind && gr.adjoint == adjoint &&
+            gr.store == store && gr.record == record) {
+            *out = gr.exec;
+            return 0;
+        }
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int cur = 0;
+    int rc = enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, 1, steps, c->d_ntbase, cur);
+    if (rc == 0 && launch_advance(c->d_ntbase, steps, c->stream) != hipSuccess)
+        rc = dn_fail(c, "graph capture: counter advance failed");
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return dn_fail(c, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return dn_fail(c, "graph instantiate failed: %s", hipGetErrorString(ei));
+    c->graphs.push_back({n, a.R, steps, kind, adjoint, store, record, exec});
+    *out = exec;
+    return 0;
+}
+
+// One pass (forward or adjoint) over the given shots, resident in batch
+// slots 0..n-1.  store: forward snapshots + Ws; record: forward seismograms.
+// The time loop runs as replays of a captured chunk of steps (hipGraph) plus
+// eager launches for the remainder.
 int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record)
 {
     const int n = (int)shots.size();
     if (n < 1 || n > c->batch_cap) return dn_fail(c, "dn_run_pass: %d shots for a batch of %d", n, c->batch_cap);
     int ntmax = 0;
     for (int s = 0; s < n; ++s) ntmax = std::max(ntmax, c->shot_nt[shots[s]]);
+    if (c->nt_limit > 0) ntmax = std::min(ntmax, c->nt_limit);
     if (c->cur_map != shots) {
         HIPCHK(c, hipMemcpyAsync(c->d_shot_map, shots.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
         c->cur_map = shots;
@@ -598,36 +753,31 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     a.snap_shot_stride = (long long)nsnap * 3 * (long long)nxny;
     a.stage = c->d_stage;
     a.stage_max = c->d_stage_max;
+    const bool pack_mode = !adjoint && store && kind != 0;
+    const int CH = chunk_steps(c);
+    const int nchunks = (CH > 0 && ntmax >= 2 * CH) ? ntmax / CH : 0;
+    hipGraphExec_t exec = nullptr;
+    if (nchunks > 0) {
+        if (!c->d_ntbase) HIPCHK(c, hipMalloc((void **)&c->d_ntbase, 256));
+        if (get_graph(c, a, n, adjoint, store, record, pack_mode, nsnap, CH, &exec)) return -1;
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int cur = 0;
-    for (int nt = 1; nt <= ntmax; ++nt) {
-        a.nt = nt;
-        a.vy_in = c->d_vy[cur];
-        a.sxy_in = c->d_sxy[cur];
-        a.syz_in = c->d_syz[cur];
-        a.vy_out = c->d_vy[cur ^ 1];
-        a.sxy_out = c->d_sxy[cur ^ 1];
-        a.syz_out = c->d_syz[cur ^ 1];
-        a.psx_in = c->d_psx[cur];
-        a.psx_out = c->d_psx[cur ^ 1];
-        a.psy_in = c->d_psy[cur];
-        a.psy_out = c->d_psy[cur ^ 1];
-        const bool pack = !adjoint && store && kind != 0 && ((nt - 1) % c->p.dtinv) == 0;
-        if (pack) HIPCHK(c, launch_fill_u32(c->d_stage_max, (size_t)n * 3, 0u, c->stream));
-        HIPCHK(c, launch_step(a, c->K, c->p.L > 0, adjoint, c->stream));
-        if (pack)
-            HIPCHK(c, launch_snap_pack(n, c->g.nx, c->g.ny, c->d_stage, c->d_stage_max, c->d_snap16, c->d_snap_scale,
-                                       a.snap_shot_stride, nsnap, (nt - 1) / c->p.dtinv, kind, c->stream));
-        cur ^= 1;
-        c->launches++;
+    if (nchunks > 0) {
+        HIPCHK(c, launch_fill_u32((unsigned int *)c->d_ntbase, 1, 0u, c->stream));
+        for (int k = 0; k < nchunks; ++k) HIPCHK(c, hipGraphLaunch(exec, c->stream));
+        c->graph_replays += nchunks;
     }
+    const int done = nchunks * CH;   // even: the fields are back in half 0
+    if (enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, done + 1, ntmax - done, nullptr, cur)) return -1;
+    c->launches += ntmax;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->ms_steps += ms;
     long long cells = 0;
-    for (int s = 0; s < n; ++s) cells += (long long)c->g.nx * c->g.ny * c->shot_nt[shots[s]];
+    for (int s = 0; s < n; ++s) cells += (long long)c->g.nx * c->g.ny * std::min(c->shot_nt[shots[s]], ntmax);
     c->cell_updates += cells;
     c->ms_pass[adjoint ? 1 : 0] += ms;
     c->launches_pass[adjoint ? 1 : 0] += ntmax;

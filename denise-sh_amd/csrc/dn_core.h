@@ -62,6 +62,17 @@ struct dn_ctx {
     long long launches = 0, cell_updates = 0;
     double ms_pass[2] = {0.0, 0.0};          // [forward, adjoint]
     long long launches_pass[2] = {0, 0}, cells_pass[2] = {0, 0};
+    // time-loop replay: captured chunks of `steps` time steps per batch shape
+    struct Graph {
+        int n, R, steps, kind;
+        bool adjoint, store, record;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    int graph_steps = -1;            // -1 default chunk, 0 eager launches, > 0 requested chunk
+    int *d_ntbase = nullptr;         // device step counter read by replayed chunks
+    int nt_limit = 0;                // > 0: passes stop after this step (autotuner)
+    long long graph_replays = 0;
     // FWI optimiser state (dn_fwi.cpp) and an optional gradient preconditioner
     dn_fwi_state *fwi = nullptr;
     int (*precond_hook)(dn_ctx *, float *grad_vs, float *grad_rho) = nullptr;
@@ -78,3 +89,4 @@ int dn_forward_shots(dn_ctx *c, const std::vector<int> &shots, float *seis_host)
 int dn_nsnap(const dn_ctx *c);
 void dn_reset_timing(dn_ctx *c);
 void dn_fwi_destroy(dn_ctx *c);
+void dn_drop_graphs(dn_ctx *c);

@@ -86,7 +86,8 @@ struct StepArgs {
     unsigned int *stage_max;                // store_mode 2: [s][3] max |x| as float bits
     float *wacc;                            // Ws (forward) or Wr (adjoint) [s][ny][nx]
     float *grad_u, *grad_rho;               // adjoint [s][ny][nx]
-    int nt;                                 // current step, 1-based
+    int nt;                                 // current step, 1-based (offset within a graph chunk if nt_base)
+    const int *nt_base;                     // graph replay: device step counter added to nt, or null
 };
 
 // launchers (dn_kernels.hip)
@@ -119,9 +120,13 @@ hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, cons
                              float *grad_rho, hipStream_t st);
 
 // bf16/fp16 snapshot packing (store_mode 2)
+// (snapshot slot = (nt_base ? *nt_base : 0) + nt - 1) / dtinv)
 hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, const unsigned int *stage_max,
-                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap, int slot,
-                            int kind, hipStream_t st);
+                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap,
+                            const int *nt_base, int nt, int dtinv, int kind, hipStream_t st);
+
+// graph replay step counter: *p += v (one lane)
+hipError_t launch_advance(int *p, int v, hipStream_t st);
 
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t st);
 hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st);

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const Geo &g = a.g;
     const int S = a.shot_map[s];
     const int NTs = a.shot_nt[S];
-    const int nt = a.nt;
+    const int nt = a.nt_base ? (*a.nt_base + a.nt) : a.nt;
     if (nt > NTs) return;  // shot finished (uneven record lengths)
 
     const int lane = threadIdx.x;
@@ -611,9 +611,11 @@ hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, cons
 // max/T per snapshot and field (T = 1 for bf16, 2^15 for fp16).
 // ---------------------------------------------------------------------------
 __global__ void k_snap_pack(int nxny, const float *stage, const unsigned int *stage_max, uint16_t *snap16,
-                            float *snap_scale, long long shot_stride16, int nsnap, int slot, int kind)
+                            float *snap_scale, long long shot_stride16, int nsnap, const int *nt_base, int nt,
+                            int dtinv, int kind)
 {
     const int s = blockIdx.z, f = blockIdx.y;
+    const int slot = ((nt_base ? *nt_base : 0) + nt - 1) / dtinv;
     const float m = __uint_as_float(stage_max[s * 3 + f]);
     const float T = (kind == 1) ? 1.0f : 32768.0f;
     const float enc = (m > 0.0f) ? (T / m) : 1.0f;
@@ -628,12 +630,23 @@ __global__ void k_snap_pack(int nxny, const float *stage, const unsigned int *st
 }
 
 hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, const unsigned int *stage_max,
-                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap, int slot,
-                            int kind, hipStream_t st)
+                            uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap,
+                            const int *nt_base, int nt, int dtinv, int kind, hipStream_t st)
 {
     const int nxny = nx * ny;
     hipLaunchKernelGGL(k_snap_pack, dim3(std::min(1024, (nxny + 255) / 256), 3, nshots), dim3(256), 0, st, nxny,
-                       stage, stage_max, snap16, snap_scale, shot_stride16, nsnap, slot, kind);
+                       stage, stage_max, snap16, snap_scale, shot_stride16, nsnap, nt_base, nt, dtinv, kind);
+    return hipGetLastError();
+}
+
+__global__ void k_advance(int *p, int v)
+{
+    if (threadIdx.x == 0) p[0] += v;
+}
+
+hipError_t launch_advance(int *p, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, p, v);
     return hipGetLastError();
 }
 

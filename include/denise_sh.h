@@ -186,8 +186,19 @@ int dn_set_signals(dn_ctx *ctx, const float *signals);
 /* Observed data host [nshots][ntr][nt_max] (samples 1..nt[s] used). */
 int dn_set_observed(dn_ctx *ctx, const float *obs);
 
-/* Maximum number of shots advanced together (0 = automatic from free HBM). */
+/* Maximum number of shots advanced together (0 = automatic from free HBM)
+ * and rows per workgroup segment of the time-step kernel (0 = automatic). */
 int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
+
+/* Time-loop execution: graph_steps = 0 launches every step from the host;
+ * > 0 captures chunks of about that many steps once per batch shape
+ * (hipGraph) and replays them; -1 (default) uses 64. */
+int dn_set_exec(dn_ctx *ctx, int graph_steps);
+
+/* Tile-shape autotuner: times nsteps forward steps of the first nshots
+ * shots for each candidate rows-per-segment, with and without graph
+ * replay, keeps the fastest setting and returns it. */
+int dn_autotune(dn_ctx *ctx, int nshots, int nsteps, int *rows_per_segment, int *graph_steps);
 
 /* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis
  * (host [nshots][ntr][nt_max]) when non-NULL. */

@@ -21,7 +21,7 @@ class Cfg(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("maxrelerror", C.c_int), ("fw", C.c_int),
                 ("free_surf", C.c_int), ("L", C.c_int), ("nt", C.c_int), ("dtinv", C.c_int),
                 ("dt", C.c_double), ("dh", C.c_double), ("damping", C.c_double), ("fpml", C.c_double),
-                ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8)]
+                ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int)]
 
 
 def build():
@@ -29,8 +29,9 @@ def build():
 
 
 def make_cfg(nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, nt=100, dtinv=1, dt=1e-4, dh=0.25,
-             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=()):
+             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=(), eprecond=1):
     c = Cfg()
+    c.eprecond = eprecond
     c.nx, c.ny, c.fdorder, c.maxrelerror, c.fw, c.free_surf, c.L, c.nt, c.dtinv = (
         nx, ny, fdorder, maxrelerror, fw, free_surf, L, nt, dtinv)
     # scalar parameters are float32 values in the reference (read as float)

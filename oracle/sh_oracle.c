@@ -66,6 +66,7 @@ typedef struct {
     double dt, dh;
     double damping, fpml, npower, k_max_pml;
     double fl[8];        /* relaxation frequencies FL[1..L] at fl[0..L-1]   */
+    int eprecond;        /* EPRECOND: 1 = eprecond1 energy scaling, 0 = none */
 } orc_cfg;
 
 #define ND(c) ((c)->fdorder / 2 + 1)
@@ -1100,6 +1101,11 @@ int F(orc_adjoint)(const orc_cfg *c, const float *vs, const float *rho, const fl
 }
 
 /* ------------------------------------------------------------------------
+ * Pinned (with orc_adjoint) by central differences of the misfit on smooth
+ * bumps with EPRECOND = 0: the density correlation equals c*dJ/drho|mu and
+ * the shear correlation c*DTF*mu*dJ/dmu (DTF = DT elastic, 1 visco) with one
+ * constant c, within 8% / 25%; eprecond1 and the C_vs/C_rho means are
+ * restated from denise.c and pinned only through the GPU parity.
  * Full gradient over nshots (denise.c:746-816, 875-1694, 1743-1813):
  *   C_vs, C_rho = float means of the model (i outer, j inner, Q14);
  *   per shot: forward, residual, adjoint, eprecond1 (Q16), scaling;
@@ -1151,6 +1157,14 @@ int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const f
         F(orc_residual)(ntr, NT, c->dt, seis, obsr, diff, l2, energy);
         rc = F(orc_adjoint)(c, vs, rho, qs, ntr, rec_i, rec_j, diff, snaps, gu_shot, grho_shot, Wr);
         if (rc) break;
+        if (!c->eprecond) { /* EPRECOND 0: the shot gradients are summed unscaled (denise.c:1665) */
+            for (i = 1; i <= NX; i++)
+                for (j = 1; j <= NY; j++) {
+                    AT(grho, c, j, i) += AT(grho_shot, c, j, i);
+                    AT(gu, c, j, i) += AT(gu_shot, c, j, i);
+                }
+            continue;
+        }
         /* eprecond1.c:15-70 */
         maxWe = 0.0;
         for (i = 1; i <= NX; i++)

@@ -153,3 +153,80 @@ def test_gradient(case):
     assert ev <= GRAD_TOL and er <= GRAD_TOL
     assert abs(l2 - r32["l2"]) <= 1e-9 * abs(r32["l2"])
     assert abs(en - r32["energy"]) <= 1e-9 * abs(r32["energy"])
+
+
+def _obs_for(ctx, cfg, d, nx, ny, nshots):
+    vt = d["vs"] * np.float32(1.03)
+    vt[ny // 4: ny // 2, nx // 3: 2 * nx // 3] *= np.float32(1.1)
+    o32 = oracle("f32")
+    obs = np.zeros((nshots, len(d["rec_i"]), d["sig"].shape[1]), np.float32)
+    for s in range(nshots):
+        obs[s], _, _ = o32.forward(cfg, vt, d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1],
+                                   d["sig"][s], d["rec_i"], d["rec_j"])
+    return obs
+
+
+def test_graph_replay_equals_eager_launches():
+    """The captured time-step chunk, replayed many times per pass and across
+    passes (with eager work and host synchronisation in between), gives the
+    same bits as launching every step from the host."""
+    say("case graph replay vs eager")
+    nts = [700, 530, 611]
+    ctx, cfg, d = setup_case(100, 44, 8, 2, 1, 10, 700, 60.0, nshots=3, dtinv=3, snap_precision=1, nts=nts)
+    ctx.set_exec(0)
+    eager = ctx.forward()
+    ctx.set_exec(-1)
+    g1 = ctx.forward()
+    g2 = ctx.forward()
+    ctx.set_exec(30)            # chunk of 30 steps (lcm(2, dtinv) = 6)
+    g3 = ctx.forward()
+    for g in (g1, g2, g3):
+        assert np.array_equal(g, eager)
+    # gradient with bf16 snapshots: the pack kernel reads the step counter too
+    from pyoracle import make_cfg
+    obs = np.zeros((3, len(d["rec_i"]), 700), np.float32)
+    obs[:, :, :] = eager * np.float32(0.9)
+    ctx.set_observed(obs)
+    ctx.set_exec(0)
+    ctx.gradient()
+    ge = ctx.get_gradient()
+    ctx.set_exec(-1)
+    for _ in range(2):
+        ctx.gradient()
+        gg = ctx.get_gradient()
+        for a, b in zip(ge, gg):
+            assert np.array_equal(a, b)
+
+
+def test_autotuner_keeps_results():
+    say("case autotune")
+    ctx, cfg, d = setup_case(120, 60, 8, 2, 1, 10, 300, 60.0, nshots=4)
+    ref = ctx.forward()
+    rows, graph = ctx.autotune(4, 30)
+    say("  tuned rows %d graph %d" % (rows, graph))
+    assert 1 <= rows <= 60 and graph in (-1, 0)
+    assert np.array_equal(ctx.forward(), ref)
+
+
+@pytest.mark.parametrize("kind,tol", [(1, 1e-2), (2, 2e-3)], ids=["bf16", "fp16"])
+def test_reduced_precision_snapshots(kind, tol):
+    """Gradient with BF16 / FP16 snapshots (per-snapshot max-abs scaling, FP32
+    accumulation): error against FP32 storage and the FP64 oracle stated."""
+    nx, ny, nt = 100, 44, 800
+    say("case snapshots", "bf16" if kind == 1 else "fp16")
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=2, dtinv=2)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 2)
+    ctx.set_observed(obs)
+    ctx.gradient()
+    f32 = ctx.get_gradient()
+    ctx16, _, _ = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=2, dtinv=2, snap_precision=kind)
+    ctx16.set_observed(obs)
+    ctx16.gradient()
+    low = ctx16.get_gradient()
+    r64 = oracle("f64").gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"],
+                                 d["rec_j"], obs)
+    ev, er = rel_l2(low[0], f32[0]), rel_l2(low[1], f32[1])
+    say("  grad_vs rel L2 vs fp32 storage %.3e, vs f64 oracle %.3e (fp32 storage vs f64 %.3e)" %
+        (ev, rel_l2(low[0], r64["grad_vs"]), rel_l2(f32[0], r64["grad_vs"])))
+    say("  grad_rho rel L2 vs fp32 storage %.3e, vs f64 oracle %.3e" % (er, rel_l2(low[1], r64["grad_rho"])))
+    assert 0 < ev <= tol and er <= tol

@@ -546,3 +546,71 @@ def test_checkfd_equals_oracle_and_flags(L):
     assert rc == 1 and r3["fw_too_large"]
     hc = dn.holbergcoeff(8, 1)
     np.testing.assert_allclose(r["dhstab"], r["cmin"] / (hc[0] * 2 / 0.04), rtol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# adjoint gradient: directional derivative of the misfit
+# --------------------------------------------------------------------------
+def _misfit(o, c, vs, rho, qs, si, sj, sig, ri, rj, obs):
+    seis, _, _ = o.forward(c, vs, rho, qs, si, sj, sig, ri, rj)
+    d = seis[:, 1:] - obs[:, 1:].astype(seis.dtype)   # calc_res.c: obs[1] := syn[1]
+    return float(np.sum(d.astype(np.float64) ** 2))
+
+
+@pytest.mark.parametrize("L", [0, 2])
+def test_gradient_is_misfit_derivative(L):
+    """Directional derivatives of the misfit sum_(t>1) (syn - obs)^2 by
+    central differences against the oracle's gradient with EPRECOND = 0, on
+    smooth bumps in a homogeneous medium.  What the adjoint-state method
+    fixes for this velocity-stress scheme with raw residual injection:
+
+      density at fixed mu:   -DT * sum_t vyp1_adj * vyp1_fwd = c * dJ/drho
+      shear modulus:         -DT * sum_t (uyx sxy + uy syz)  = c * DTF * mu * dJ/dmu
+      vs (chain rule 2 rho vs): the same ratio as mu, since d/d ln vs = 2 d/d ln mu
+
+    with one positive constant c for both parameters and DTF = DT for the
+    elastic update (its uy, uyx carry the DT factor) and 1 for the
+    viscoelastic one.  A wrong sign, a dropped correlation term, a
+    mis-paired snapshot, a transposed field or a wrong chain-rule factor
+    breaks one of the ratios below."""
+    nx, ny, nt, dh, dt = 70, 44, 700, 0.25, 1e-4
+    fl = [6.25, 50.0][:L]
+    c = cfg(nx=nx, ny=ny, fdorder=4, maxrelerror=1, fw=10, free_surf=1, L=L, nt=nt, dtinv=1, dt=dt, dh=dh,
+            damping=300.0, fpml=30.0, fl=fl, eprecond=0)
+    vs = np.full((ny, nx), 200.0, np.float32)
+    rho = np.full((ny, nx), 1800.0, np.float32)
+    qs = np.full((ny, nx), 20.0, np.float32) if L else None
+    t = np.arange(1, nt + 1) * dt
+    tau = np.pi * (t - 1.5 / 30.0) / (1.5 / 30.0)
+    sig = ((1 - 4 * tau ** 2) * np.exp(-2 * tau ** 2)).astype(np.float32)[None, :]
+    si, sj = np.array([35], np.int32), np.array([3], np.int32)
+    ri = np.arange(14, 57, 3, dtype=np.int32)
+    rj = np.full_like(ri, 2)
+    vt = vs.copy()
+    vt[14:24, 25:45] *= np.float32(1.06)
+    obs, _, _ = O64.forward(c, vt, rho, qs, si, sj, sig[0], ri, rj)
+    obs = obs.astype(np.float32)
+    g = O64.gradient(c, vs, rho, qs, si, sj, sig, ri, rj, obs[None])
+    J = lambda v, r: _misfit(O64, c, v, r, qs, si, sj, sig[0], ri, rj, obs)
+    jj, ii = np.mgrid[1:ny + 1, 1:nx + 1]
+    mu = 1800.0 * 200.0 ** 2
+    dtf = dt if L == 0 else 1.0
+    e = 2e-3
+    rows = []
+    for cj, ci in [(10, 24), (16, 25), (24, 44), (8, 50), (30, 35)]:
+        d = np.exp(-((jj - cj) ** 2 + (ii - ci) ** 2) / (2 * 5.0 ** 2))
+        fd_mu = (J(vs * np.sqrt(1 + e * d), rho) - J(vs * np.sqrt(1 - e * d), rho)) / (2 * e)       # d/d ln mu
+        fd_rho = (J(vs / np.sqrt(1 + e * d), rho * (1 + e * d)) -
+                  J(vs / np.sqrt(1 - e * d), rho * (1 - e * d))) / (2 * e)                          # d/d ln rho | mu
+        fd_vs = (J(vs * (1 + e * d), rho) - J(vs * (1 - e * d), rho)) / (2 * e)                     # d/d ln vs | rho
+        k_mu = float(np.sum(-dt * g["raw_u"] * mu * d))
+        k_rho = float(np.sum(-dt * g["raw_rho"] * 1800.0 * d))
+        k_vs = float(np.sum(g["grad_vs"] * vs * d))
+        rows.append((k_rho / fd_rho, k_mu / fd_mu / (dtf * mu), k_vs / fd_vs / (dtf * mu)))
+    r = np.array(rows)
+    print("ratios [rho|mu, mu/(DTF mu), vs/(DTF mu)]:\n", r)
+    cst = np.median(r[:, 0])
+    assert cst > 0
+    assert np.all(np.abs(r[:, 0] / cst - 1) < 0.08)
+    assert np.all((r[:, 1] / cst > 0.75) & (r[:, 1] / cst < 1.12))
+    np.testing.assert_allclose(r[:, 2], r[:, 1], rtol=0.02)
