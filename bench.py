@@ -16,7 +16,8 @@ correlation, energy preconditioning and the chain rule.  The metric counts
 cell-updates of the forward and the adjoint sweeps (NX*NY*NT*2 per shot).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT]
-       [--shots S] [--snap fp32|bf16|fp16] [--no-cpu]
+       [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu] [--autotune]
+       python bench.py --copy-probe      (copy kernel that scales HBM counters)
 """
 import argparse
 import json
@@ -69,24 +70,46 @@ def ricker(dn, nt, dt, fc):
     return dn.wavelet(1, nt, dt, 0.0, fc, 1.0)
 
 
-def cpu_baseline(W, nt_sample):
-    """Oracle (FP32 restatement of the reference) on the host cores: one shot,
-    forward + adjoint over nt_sample steps of the same grid."""
+def cpu_baseline(W, seconds):
+    """Oracle (FP32 restatement of the reference, OpenMP over rows) on the host
+    cores: one shot of the same grid, forward + adjoint gradient.  A short
+    probe sizes the sample to about `seconds` of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle, make_cfg
     o = Oracle("f32")
-    cfg = make_cfg(W["nx"], W["ny"], fdorder=W["fdorder"], fw=W["fw"], free_surf=1, L=W["L"], nt=nt_sample,
-                   dtinv=W["dtinv"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], fl=W["fl"])
-    sig = o.wavelet(1, nt_sample, W["dt"], 0.0, W["fc"], 1.0).astype(np.float32)[None]
-    obs = np.zeros((1, len(W["rec_i"]), nt_sample), np.float32)
-    t0 = time.perf_counter()
-    o.gradient(cfg, W["vs_start"], W["rho"], W["qs"], W["src_i"][:1], W["src_j"][:1], sig, W["rec_i"], W["rec_j"],
-               obs)
-    sec = time.perf_counter() - t0
+
+    def run(nt_sample):
+        cfg = make_cfg(W["nx"], W["ny"], fdorder=W["fdorder"], fw=W["fw"], free_surf=1, L=W["L"], nt=nt_sample,
+                       dtinv=W["dtinv"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], fl=W["fl"])
+        sig = o.wavelet(1, nt_sample, W["dt"], 0.0, W["fc"], 1.0).astype(np.float32)[None]
+        obs = np.zeros((1, len(W["rec_i"]), nt_sample), np.float32)
+        t0 = time.perf_counter()
+        o.gradient(cfg, W["vs_start"], W["rho"], W["qs"], W["src_i"][:1], W["src_j"][:1], sig, W["rec_i"],
+                   W["rec_j"], obs)
+        return time.perf_counter() - t0
+
+    probe = 40
+    t = run(probe)
+    nt_sample = int(min(W["nt"], max(probe, probe * seconds / max(t, 1e-3))))
+    sec = run(nt_sample)
     cells = 2.0 * W["nx"] * W["ny"] * nt_sample
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=cells / sec / 1e9, unit="Gcell-updates/s", cores=threads, kind="port",
-                sample="1 shot of the same grid, forward+adjoint gradient over %d steps (%.1f s)" % (nt_sample, sec))
+                sample="1 of %d shots, forward+adjoint gradient over %d of %d steps on the same grid (%.1f s)"
+                       % (len(W["src_i"]), nt_sample, W["nt"], sec))
+
+
+def copy_probe(nbytes=1 << 30, reps=4):
+    """Plain device-to-device copy of known size (torch's copy kernel): the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
+    profiling run to bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "reps": reps}))
 
 
 def main():
@@ -97,9 +120,14 @@ def main():
     ap.add_argument("--nt", type=int, default=10000)
     ap.add_argument("--shots", type=int, default=40)
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"])
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to scale HBM counters")
+    ap.add_argument("--autotune", action="store_true", help="tune rows per segment and graph replay first")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs (--gpus %d) are not provided" % args.gpus,
               file=sys.stderr)
@@ -119,6 +147,9 @@ def main():
     obs = ctx.forward()
     ctx.set_observed(obs)
     ctx.set_model(W["vs_start"], W["rho"], W["qs"])
+    tuned = None
+    if args.autotune:
+        tuned = ctx.autotune(args.shots, min(args.nt, 256))
     for _ in range(args.warmup):
         ctx.gradient()
     ms = {0: 0.0, 1: 0.0}
@@ -166,8 +197,10 @@ def main():
                      "cell_updates_per_launch": units_per_launch},
         "misfit_l2": l2,
     }
+    if tuned is not None:
+        out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     ctx.close()
     print(json.dumps(out))
 

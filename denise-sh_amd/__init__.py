@@ -23,7 +23,7 @@ EXPORTS = [
     "dn_workflow_read", "dn_workflow_free", "dn_workflow_nstages", "dn_workflow_get", "dn_read_sources",
     "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
     "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction", "dn_checkfd",
-    "dn_set_exec", "dn_autotune",
+    "dn_set_exec", "dn_autotune", "dn_pseudo_hessian",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
                                      C.POINTER(C.c_longlong)]
         L.dn_set_globals.argtypes = [C.POINTER(Globals)]
         L.dn_set_exec.argtypes = [C.c_void_p, C.c_int]
+        L.dn_pseudo_hessian.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp]
         L.dn_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip]
         L.dn_set_shots_multi.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, ip, fp]
         L.dn_set_signals.argtypes = [C.c_void_p, fp]
@@ -194,6 +195,13 @@ class Context:
         en = C.c_double()
         self._chk(lib().dn_gradient(self._h, shot0, nshots, 1 if accumulate else 0, C.byref(l2), C.byref(en)))
         return l2.value, en.value
+
+    def pseudo_hessian(self, shot0=0, nshots=None):
+        nshots = self.nshots - shot0 if nshots is None else nshots
+        hv = np.zeros((self.ny, self.nx), np.float32)
+        hr = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(lib().dn_pseudo_hessian(self._h, shot0, nshots, _f(hv), _f(hr)))
+        return hv, hr
 
     def get_gradient(self):
         out = [np.zeros((self.ny, self.nx), np.float32) for _ in range(4)]

@@ -212,6 +212,8 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
     dfree_t(c->d_obs);
     dfree_t(c->d_ntbase);
+    dfree_t(c->d_hu);
+    dfree_t(c->d_hr);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -683,9 +685,10 @@ static int get_graph(dn_ctx *c, StepArgs a, int n, bool adjoint, bool store, boo
                      int nsnap, int steps, hipGraphExec_t *out)
 {
     const int kind = c->p.snap_precision;
+    const bool hess = a.hess != 0;
     for (auto &gr : c->graphs)
         if (gr.n == n && gr.R == a.R && gr.steps == steps && gr.kind == kind && gr.adjoint == adjoint &&
-            gr.store == store && gr.record == record) {
+            gr.store == store && gr.record == record && gr.hess == hess) {
             *out = gr.exec;
             return 0;
         }
@@ -705,7 +708,7 @@ static int get_graph(dn_ctx *c, StepArgs a, int n, bool adjoint, bool store, boo
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return dn_fail(c, "graph instantiate failed: %s", hipGetErrorString(ei));
-    c->graphs.push_back({n, a.R, steps, kind, adjoint, store, record, exec});
+    c->graphs.push_back({n, a.R, steps, kind, adjoint, store, record, hess, exec});
     *out = exec;
     return 0;
 }
@@ -737,6 +740,13 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
         a.store_mode = store ? (kind == 0 ? 1 : 2) : 0;
         a.wacc = c->d_Ws;
         if (store) HIPCHK(c, launch_fill(c->d_Ws, (size_t)n * nxny, 0.0f, c->stream));
+        if (c->hess_mode) {
+            a.hess = 1;
+            a.grad_u = c->d_gu_shot;
+            a.grad_rho = c->d_grho_shot;
+            HIPCHK(c, launch_fill(c->d_gu_shot, (size_t)n * nxny, 0.0f, c->stream));
+            HIPCHK(c, launch_fill(c->d_grho_shot, (size_t)n * nxny, 0.0f, c->stream));
+        }
     } else {
         a.resid = c->d_resid;
         a.wacc = c->d_Wr;
@@ -894,6 +904,35 @@ extern "C" int dn_gradient(dn_ctx *c, int shot0, int nshots, int accumulate, dou
     }
     if (l2) *l2 = L2;
     if (energy) *energy = EN;
+    return 0;
+}
+
+extern "C" int dn_pseudo_hessian(dn_ctx *c, int shot0, int nshots, float *h_vs, float *h_rho)
+{
+    if (!c) return -1;
+    if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_pseudo_hessian: bad shot range");
+    if (!c->have_C) return dn_fail(c, "dn_pseudo_hessian: model not set");
+    const size_t nxny = (size_t)c->g.nx * c->g.ny;
+    if (!c->d_hu) HIPCHK(c, dalloc(&c->d_hu, nxny));
+    if (!c->d_hr) HIPCHK(c, dalloc(&c->d_hr, nxny));
+    dn_reset_timing(c);
+    std::vector<int> shots(nshots);
+    for (int s = 0; s < nshots; ++s) shots[s] = shot0 + s;
+    for (size_t b0 = 0; b0 < shots.size();) {
+        if (dn_ensure_batch(c, (int)(shots.size() - b0), false)) return -1;
+        const size_t n = std::min((size_t)c->batch_cap, shots.size() - b0);
+        std::vector<int> part(shots.begin() + b0, shots.begin() + b0 + n);
+        c->hess_mode = true;
+        const int rc = dn_run_pass(c, part, false, false, false);
+        c->hess_mode = false;
+        if (rc) return rc;
+        HIPCHK(c, launch_hessian_finish((int)n, c->g.nx, c->g.ny, c->g, c->d_gu_shot, c->d_grho_shot, c->d_vs,
+                                        c->d_rho, c->d_hu, c->d_hr, b0 == 0, b0 + n == shots.size(), c->stream));
+        b0 += n;
+    }
+    if (h_vs) HIPCHK(c, hipMemcpyAsync(h_vs, c->d_hu, nxny * 4, hipMemcpyDeviceToHost, c->stream));
+    if (h_rho) HIPCHK(c, hipMemcpyAsync(h_rho, c->d_hr, nxny * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

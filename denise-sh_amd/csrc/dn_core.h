@@ -65,7 +65,7 @@ struct dn_ctx {
     // time-loop replay: captured chunks of `steps` time steps per batch shape
     struct Graph {
         int n, R, steps, kind;
-        bool adjoint, store, record;
+        bool adjoint, store, record, hess;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -73,6 +73,9 @@ struct dn_ctx {
     int *d_ntbase = nullptr;         // device step counter read by replayed chunks
     int nt_limit = 0;                // > 0: passes stop after this step (autotuner)
     long long graph_replays = 0;
+    // pseudo-Hessian (forward passes accumulate it when hess_mode is set)
+    bool hess_mode = false;
+    float *d_hu = nullptr, *d_hr = nullptr;
     // FWI optimiser state (dn_fwi.cpp) and an optional gradient preconditioner
     dn_fwi_state *fwi = nullptr;
     int (*precond_hook)(dn_ctx *, float *grad_vs, float *grad_rho) = nullptr;

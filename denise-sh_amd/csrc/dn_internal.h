@@ -85,7 +85,8 @@ struct StepArgs {
     float *stage;                           // store_mode 2: [s][3][ny][nx]
     unsigned int *stage_max;                // store_mode 2: [s][3] max |x| as float bits
     float *wacc;                            // Ws (forward) or Wr (adjoint) [s][ny][nx]
-    float *grad_u, *grad_rho;               // adjoint [s][ny][nx]
+    float *grad_u, *grad_rho;               // adjoint [s][ny][nx]; forward with hess: H_mu, H_rho sums
+    int hess;                               // forward: accumulate the pseudo-Hessian (hess_save.c, hessian.c)
     int nt;                                 // current step, 1-based (offset within a graph chunk if nt_base)
     const int *nt_base;                     // graph replay: device step counter added to nt, or null
 };
@@ -104,6 +105,12 @@ struct ModelPrepArgs {
     float eta[kMaxL];                       // readmod.c:33 (host)
 };
 hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st);
+
+// pseudo-Hessian: per-shot sums added in shot order, then the chain rule of
+// hessian_out.c:25-31 (INVMAT1 1)
+hipError_t launch_hessian_finish(int nshots, int nx, int ny, const Geo &g, const float *hu_shot,
+                                 const float *hr_shot, const float *vs_pad, const float *rho_pad, float *hu,
+                                 float *hr, int first, int last, hipStream_t st);
 
 // per-shot residual (calc_res / calc_energy): one thread per trace
 hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,

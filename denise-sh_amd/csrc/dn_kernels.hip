@@ -95,13 +95,16 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const long long nxny = (long long)g.nx * g.ny;
     const int fw = g.fw, fw2 = 2 * g.fw;
 
-    bool fstore = false, corr = false;
+    bool fstore = false, corr = false, hsum = false;
     int slot = 0;
     if (!ADJ) {
         if (a.store_mode != 0 && ((nt - 1) % a.dtinv) == 0) {
             fstore = true;
             slot = (nt - 1) / a.dtinv;
         }
+        // hessian.c:91: stored steps 1..NTDTINV-1 of this shot
+        if (a.hess && ((nt - 1) % a.dtinv) == 0 && (nt - 1) / a.dtinv < (NTs + a.dtinv - 1) / a.dtinv - 1)
+            hsum = true;
     } else {
         const int sf = NTs - nt + 1;  // forward step paired with this adjoint step
         if (((sf - 1) % a.dtinv) == 0) {
@@ -186,6 +189,11 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             const float bb = a.bfield[gi];
             const float tsum = sxy_x + syy_y;
             const long long ci = (long long)(jv - 1) * g.nx + (c - 1);
+            if (!ADJ && hsum && own_v) {
+                const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
+                float *hp = a.grad_rho + (long long)s * nxny + ci;
+                *hp = *hp + (acc * acc);
+            }
             if (!ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
                 if (a.store_mode == 1)
@@ -336,6 +344,11 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                     float *wp = a.wacc + (long long)s * nxny + ci;
                     *wp = *wp + vc * vc;       // eprecond.c:11
                 }
+                if (!ADJ && hsum) {
+                    const float eyx = 0.5f * vyx;     // hess_save.c:13 (exact in float)
+                    float *hp = a.grad_u + (long long)s * nxny + ci;
+                    *hp = *hp + ((vyy * vyy) + (eyx * eyx));
+                }
                 if (ADJ && corr) {
                     float fy, fu;
                     const long long sidx = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
@@ -410,6 +423,42 @@ hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipSt
     case 6: return launch_step_k<6>(a, visco, adjoint, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Pseudo-Hessian: add the per-shot sums in shot order (the reference's
+// accumulator runs on across shots) and, after the last batch, the chain
+// rule of hessian_out.c:25-31 with the reference's double promotion.
+__global__ void k_hessian_finish(int nshots, int nx, int ny, Geo g, const float *hu_shot, const float *hr_shot,
+                                 const float *vs_pad, const float *rho_pad, float *hu, float *hr, int first,
+                                 int last)
+{
+    const int nxny = nx * ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x) {
+        float a = first ? 0.0f : hu[t];
+        float b = first ? 0.0f : hr[t];
+        for (int s = 0; s < nshots; ++s) {
+            a = a + hu_shot[(long long)s * nxny + t];
+            b = b + hr_shot[(long long)s * nxny + t];
+        }
+        if (last) {
+            const int j = t / nx + 1, i = t % nx + 1;
+            const float u = vs_pad[gidx(g, j, i)], r = rho_pad[gidx(g, j, i)];
+            b = b + u * u * u * u * a;
+            a = (float)(4.0 * r * r * u * u * a);
+        }
+        hu[t] = a;
+        hr[t] = b;
+    }
+}
+
+hipError_t launch_hessian_finish(int nshots, int nx, int ny, const Geo &g, const float *hu_shot,
+                                 const float *hr_shot, const float *vs_pad, const float *rho_pad, float *hu,
+                                 float *hr, int first, int last, hipStream_t st)
+{
+    const int nxny = nx * ny;
+    hipLaunchKernelGGL(k_hessian_finish, dim3(std::min(2048, (nxny + 255) / 256)), dim3(256), 0, st, nshots, nx, ny,
+                       g, hu_shot, hr_shot, vs_pad, rho_pad, hu, hr, first, last);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -190,6 +190,14 @@ int dn_set_observed(dn_ctx *ctx, const float *obs);
  * and rows per workgroup segment of the time-step kernel (0 = automatic). */
 int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
 
+/* Pseudo-Hessian diagonal (HESSIAN = 1, hess_save.c:2-17, hessian.c:91-94,
+ * hessian_out.c:25-31): forward modelling of shots [shot0, shot0+nshots)
+ * with the signals set (the reference uses the QUELLART 6 spike low-passed
+ * at FC_HESSIAN), summing uy^2 + (uyx/2)^2 and vyp1^2 over stored steps
+ * 1..NTDTINV-1 and applying the chain rule to vs and rho.  Outputs host
+ * [ny][nx]; either may be NULL. */
+int dn_pseudo_hessian(dn_ctx *ctx, int shot0, int nshots, float *h_vs, float *h_rho);
+
 /* Time-loop execution: graph_steps = 0 launches every step from the host;
  * > 0 captures chunks of about that many steps once per batch shape
  * (hipGraph) and replays them; -1 (default) uses 64. */

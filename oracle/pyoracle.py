@@ -163,6 +163,22 @@ class Oracle:
             raise RuntimeError("oracle gradient failed (%d)" % rc)
         return dict(grad_vs=gvs, grad_rho=grho, raw_u=wu, raw_rho=wr, l2=l2.value, energy=en.value)
 
+    def pseudo_hessian(self, cfg, vs, rho, qs, src_i, src_j, signals):
+        vs = np.ascontiguousarray(vs, np.float32)
+        rho = np.ascontiguousarray(rho, np.float32)
+        qs = None if qs is None else np.ascontiguousarray(qs, np.float32)
+        src_i = np.ascontiguousarray(src_i, np.int32)
+        src_j = np.ascontiguousarray(src_j, np.int32)
+        sig = np.ascontiguousarray(signals, np.float32)
+        hv = np.zeros((cfg.ny, cfg.nx), self.dt)
+        hr = np.zeros((cfg.ny, cfg.nx), self.dt)
+        rc = self.fn("orc_pseudo_hessian")(C.byref(cfg), self._f32(vs), self._f32(rho), self._f32(qs), len(src_i),
+                                           self._i32(src_i), self._i32(src_j), self._f32(sig), self._r(hv),
+                                           self._r(hr))
+        if rc:
+            raise RuntimeError("oracle pseudo-Hessian failed (%d)" % rc)
+        return hv, hr
+
     # ---- single sweeps (padded arrays, shapes from padded_shape) ----
     def update_v(self, cfg, sw, vy, vyp1, syz, sxy, rho, hc, pml, psi_syy_y, psi_sxy_x):
         self.fn("orc_update_v")(C.byref(cfg), sw, self._r(vy), self._r(vyp1), self._r(syz), self._r(sxy),

@@ -1200,6 +1200,50 @@ int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const f
     return rc;
 }
 
+/* ------------------------------------------------------------------------
+ * Pseudo-Hessian diagonal, HESSIAN = 1: hess_save.c:2-17 stores uy,
+ * 0.5*uyx and vyp1 at every stored forward step; hessian.c:91-94 sums their
+ * squares over stored steps 1..NTDTINV-1 (the last stored step is left
+ * out) into one accumulator that runs on across shots; hessian_out.c:25-36
+ * applies the chain rule to vs and rho (INVMAT1 = 1, rho first).  Pinned
+ * against sums of squares of orc_forward's snapshots in the tests.
+ * Outputs [NY][NX].
+ * ---------------------------------------------------------------------- */
+int F(orc_pseudo_hessian)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
+                          const int *src_i, const int *src_j, const float *signals, real *hess_vs, real *hess_rho)
+{
+    const int NT = c->nt, NX = c->nx, NY = c->ny;
+    const int NTDTINV = F(orc_ntdtinv)(NT, c->dtinv);
+    const size_t nxny = (size_t)NX * NY;
+    real *snaps = zalloc((size_t)NTDTINV * 3 * nxny);
+    real *hu = zalloc(nxny), *hr = zalloc(nxny);
+    int ishot, k, rc = 0;
+    size_t o;
+    for (ishot = 0; ishot < nshots && rc == 0; ishot++) {
+        rc = F(orc_forward)(c, vs, rho, qs, 1, &src_i[ishot], &src_j[ishot], signals + (size_t)ishot * NT, 0, NULL,
+                            NULL, NULL, snaps, NULL);
+        for (k = 0; k < NTDTINV - 1; k++) {
+            const real *base = snaps + (size_t)k * 3 * nxny;
+            for (o = 0; o < nxny; o++) {
+                const real eyy = base[nxny + o];
+                const real eyx = 0.5 * base[2 * nxny + o];
+                const real utty = base[o];
+                hu[o] += (eyy * eyy) + (eyx * eyx);
+                hr[o] += (utty * utty);
+            }
+        }
+    }
+    for (o = 0; o < nxny; o++) {
+        const real u = (real)vs[o], r = (real)rho[o];
+        hr[o] += u * u * u * u * hu[o];
+        hu[o] = 4.0 * r * r * u * u * hu[o];
+        hess_vs[o] = hu[o];
+        hess_rho[o] = hr[o];
+    }
+    free(snaps); free(hu); free(hr);
+    return rc;
+}
+
 /* ========================================================================
  * Optimiser pieces (host-side in the reference), restated for the tests.
  * ====================================================================== */

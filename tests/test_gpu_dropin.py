@@ -59,8 +59,9 @@ def test_update_v_PML(fdorder, free_surf, sw):
     dn.set_globals(NX=nx, NY=ny, FDORDER=fdorder, FW=fw, FREE_SURF=free_surf, L=0, DT=1e-4, DH=0.25, NT=10)
     vy, vyp1 = _nr(dn, F["vy"], nd, nx, ny), _nr(dn, np.zeros_like(F["vy"]), nd, nx, ny)
     syz, sxy, rho = _nr(dn, F["syz"], nd, nx, ny), _nr(dn, F["sxy"], nd, nx, ny), _nr(dn, P["rho"].astype(np.float32), nd, nx, ny)
-    psy = dn.NRMatrix(1, 2 * fw, 1, nx, F["psi_a"].reshape(2 * fw, nx))
-    psx = dn.NRMatrix(1, ny, 1, 2 * fw, F["psi_b"].reshape(ny, 2 * fw))
+    # the drop-in updates the CPML memory in place: hand it copies
+    psy = dn.NRMatrix(1, 2 * fw, 1, nx, F["psi_a"].reshape(2 * fw, nx).copy())
+    psx = dn.NRMatrix(1, ny, 1, 2 * fw, F["psi_b"].reshape(ny, 2 * fw).copy())
     hcv = dn.NRVector(0, 6, hc.astype(np.float32))
     vecs = [_vec(dn, pml[k].astype(np.float32), fw) for k in
             ("K_x", "a_x", "b_x", "K_x_half", "a_x_half", "b_x_half", "K_y", "a_y", "b_y", "K_y_half", "a_y_half",
@@ -99,8 +100,8 @@ def test_update_s_PML(fdorder, free_surf, L):
     vy = _nr(dn, F["vy"], nd, nx, ny)
     syz, sxy = _nr(dn, F["syz"], nd, nx, ny), _nr(dn, F["sxy"], nd, nx, ny)
     uy, uyx = _nr(dn, np.zeros_like(F["vy"]), nd, nx, ny), _nr(dn, np.zeros_like(F["vy"]), nd, nx, ny)
-    psy = dn.NRMatrix(1, 2 * fw, 1, nx, F["psi_a"].reshape(2 * fw, nx))
-    psx = dn.NRMatrix(1, ny, 1, 2 * fw, F["psi_b"].reshape(ny, 2 * fw))
+    psy = dn.NRMatrix(1, 2 * fw, 1, nx, F["psi_a"].reshape(2 * fw, nx).copy())
+    psx = dn.NRMatrix(1, ny, 1, 2 * fw, F["psi_b"].reshape(ny, 2 * fw).copy())
     hcv = dn.NRVector(0, 6, hc.astype(np.float32))
     vecs = [_vec(dn, pml[k].astype(np.float32), fw) for k in
             ("K_x", "a_x", "b_x", "K_x_half", "a_x_half", "b_x_half", "K_y", "a_y", "b_y", "K_y_half", "a_y_half",

@@ -230,3 +230,26 @@ def test_reduced_precision_snapshots(kind, tol):
         (ev, rel_l2(low[0], r64["grad_vs"]), rel_l2(f32[0], r64["grad_vs"])))
     say("  grad_rho rel L2 vs fp32 storage %.3e, vs f64 oracle %.3e" % (er, rel_l2(low[1], r64["grad_rho"])))
     assert 0 < ev <= tol and er <= tol
+
+
+def test_pseudo_hessian():
+    """HESSIAN = 1 path: band-limited spike (QUELLART 6 low-passed at
+    FC_HESSIAN), sums of squared forward fields, chain rule.  The reference's
+    accumulator runs on across shots in FP32; the batched sum adds per-shot
+    sums, so the bar is 1e-5 relative L2 (FP32 summation order)."""
+    say("case pseudo-Hessian")
+    nx, ny, nt = 100, 44, 500
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, dtinv=3)
+    dn = product()
+    spike = np.zeros((3, nt), np.float32)
+    spike[:, 0] = 1.0
+    sig = dn.timedomain_filt(spike, DT, 80.0, 4, 1)
+    ctx.set_signals(sig)
+    hv, hr = ctx.pseudo_hessian()
+    r32 = oracle("f32").pseudo_hessian(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], sig)
+    r64 = oracle("f64").pseudo_hessian(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], sig)
+    ev, er = rel_l2(hv, r32[0]), rel_l2(hr, r32[1])
+    say("  H_vs rel L2 vs f32 %.3e vs f64 %.3e; H_rho rel L2 vs f32 %.3e vs f64 %.3e" %
+        (ev, rel_l2(hv, r64[0]), er, rel_l2(hr, r64[1])))
+    assert np.max(r32[0]) > 0
+    assert ev <= 1e-5 and er <= 1e-5

@@ -614,3 +614,35 @@ def test_gradient_is_misfit_derivative(L):
     assert np.all(np.abs(r[:, 0] / cst - 1) < 0.08)
     assert np.all((r[:, 1] / cst > 0.75) & (r[:, 1] / cst < 1.12))
     np.testing.assert_allclose(r[:, 2], r[:, 1], rtol=0.02)
+
+
+def test_pseudo_hessian_is_sum_of_squared_snapshots():
+    """hessian.c: H_mu = sum over stored steps 1..NTDTINV-1 of uy^2 + (uyx/2)^2,
+    H_rho = sum vyp1^2, running on across shots; hessian_out.c chain rule.
+    Checked against numpy sums of the forward snapshots of orc_forward."""
+    nx, ny, nt, dtinv = 60, 40, 300, 3
+    c = cfg(nx=nx, ny=ny, fdorder=6, maxrelerror=1, fw=8, free_surf=1, L=2, nt=nt, dtinv=dtinv, dt=1e-4, dh=0.25,
+            damping=300.0, fpml=60.0, fl=[6.25, 50.0])
+    vs, rho, qs = model(nx, ny, 0.25, seed=7, qs=20.0)
+    sig = np.zeros((2, nt), np.float32)
+    sig[:, 0] = 1.0                       # QUELLART 6 spike ...
+    sig = O64.timedomain_filt(sig, 1e-4, 80.0, 4, 1).astype(np.float32)   # ... low-passed at FC_HESSIAN
+    si = np.array([20, 40], np.int32)
+    sj = np.array([3, 3], np.int32)
+    hv, hr = O64.pseudo_hessian(c, vs, rho, qs, si, sj, sig)
+    hu = np.zeros((ny, nx))
+    hrr = np.zeros((ny, nx))
+    last = np.zeros((ny, nx))
+    for s in range(2):
+        _, snaps, _ = O64.forward(c, vs, rho, qs, si[s:s + 1], sj[s:s + 1], sig[s], np.zeros(0, np.int32),
+                                  np.zeros(0, np.int32), want_snaps=True)
+        n = snaps.shape[0]
+        hu += np.sum(snaps[:n - 1, 1] ** 2 + (0.5 * snaps[:n - 1, 2]) ** 2, axis=0)
+        hrr += np.sum(snaps[:n - 1, 0] ** 2, axis=0)
+        last += snaps[n - 1, 1] ** 2
+    v, r = vs.astype(np.float64), rho.astype(np.float64)
+    np.testing.assert_allclose(hr, hrr + v ** 4 * hu, rtol=1e-11)
+    np.testing.assert_allclose(hv, 4 * r * r * v * v * hu, rtol=1e-11)
+    assert np.max(last) > 0          # the left-out last stored step carries energy (quirk matters)
+    h32v, h32r = O32.pseudo_hessian(c, vs, rho, qs, si, sj, sig)
+    assert rel_l2(h32v, hv) < 1e-5 and rel_l2(h32r, hr) < 1e-5
