@@ -23,7 +23,7 @@ EXPORTS = [
     "dn_workflow_read", "dn_workflow_free", "dn_workflow_nstages", "dn_workflow_get", "dn_read_sources",
     "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
     "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction", "dn_checkfd",
-    "dn_set_exec", "dn_autotune", "dn_pseudo_hessian",
+    "dn_set_exec", "dn_autotune", "dn_pseudo_hessian", "dn_get_energy_maps", "dn_hessian_probe",
 ]
 
 
@@ -37,7 +37,7 @@ class Params(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("maxrelerror", C.c_int), ("fw", C.c_int),
                 ("free_surf", C.c_int), ("L", C.c_int), ("fl", C.c_float * 4), ("dt", C.c_float), ("dh", C.c_float),
                 ("damping", C.c_float), ("fpml", C.c_float), ("npower", C.c_float), ("k_max_pml", C.c_float),
-                ("dtinv", C.c_int), ("snap_precision", C.c_int)]
+                ("dtinv", C.c_int), ("snap_precision", C.c_int), ("eprecond", C.c_int)]
 
 
 _lib = None
@@ -73,6 +73,8 @@ def lib():
         L.dn_set_globals.argtypes = [C.POINTER(Globals)]
         L.dn_set_exec.argtypes = [C.c_void_p, C.c_int]
         L.dn_pseudo_hessian.argtypes = [C.c_void_p, C.c_int, C.c_int, fp, fp]
+        L.dn_get_energy_maps.argtypes = [C.c_void_p, fp, fp]
+        L.dn_hessian_probe.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
         L.dn_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip]
         L.dn_set_shots_multi.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, ip, fp]
         L.dn_set_signals.argtypes = [C.c_void_p, fp]
@@ -96,13 +98,13 @@ class Context:
     """Handle API wrapper: one dn_ctx on the current HIP device."""
 
     def __init__(self, nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, fl=(), dt=1e-4, dh=0.25,
-                 damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, dtinv=1, snap_precision=0):
+                 damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, dtinv=1, snap_precision=0, eprecond=1):
         p = Params()
         p.nx, p.ny, p.fdorder, p.maxrelerror, p.fw, p.free_surf, p.L = nx, ny, fdorder, maxrelerror, fw, free_surf, L
         for k, v in enumerate(fl):
             p.fl[k] = v
         p.dt, p.dh, p.damping, p.fpml, p.npower, p.k_max_pml = dt, dh, damping, fpml, npower, k_max_pml
-        p.dtinv, p.snap_precision = dtinv, snap_precision
+        p.dtinv, p.snap_precision, p.eprecond = dtinv, snap_precision, eprecond
         self.p = p
         self.nx, self.ny = nx, ny
         self._h = lib().dn_create(C.byref(p))
@@ -195,6 +197,18 @@ class Context:
         en = C.c_double()
         self._chk(lib().dn_gradient(self._h, shot0, nshots, 1 if accumulate else 0, C.byref(l2), C.byref(en)))
         return l2.value, en.value
+
+    def energy_maps(self):
+        ws = np.zeros((self.ny, self.nx), np.float32)
+        wr = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(lib().dn_get_energy_maps(self._h, _f(ws), _f(wr)))
+        return ws, wr
+
+    def hessian_probe(self, nprobes=8, eps=1e-3, seed=1, shot0=0, nshots=None):
+        nshots = self.nshots - shot0 if nshots is None else nshots
+        d = np.zeros((self.ny, self.nx), np.float32)
+        self._chk(lib().dn_hessian_probe(self._h, shot0, nshots, nprobes, eps, seed, _f(d)))
+        return d
 
     def pseudo_hessian(self, shot0=0, nshots=None):
         nshots = self.nshots - shot0 if nshots is None else nshots

@@ -89,6 +89,10 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
         dn_fail(nullptr, "dn_create: snap_precision must be 0, 1 or 2");
         return nullptr;
     }
+    if (p->eprecond != 0 && p->eprecond != 1) {
+        dn_fail(nullptr, "dn_create: eprecond must be 0 or 1");
+        return nullptr;
+    }
     dn_ctx *c = new dn_ctx();
     c->p = *p;
     if (fd_coefficients(p->fdorder, p->maxrelerror, c->hc) != 0) {
@@ -150,6 +154,8 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
               dalloc(&c->d_grho, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_gvs_out, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_grho_out, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_ws_sum, (size_t)p->nx * p->ny) == hipSuccess &&
+              dalloc(&c->d_wr_sum, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_maxbits, 4096) == hipSuccess;
     if (!ok) {
         dn_fail(nullptr, "dn_create: out of device memory for the model");
@@ -214,6 +220,8 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_ntbase);
     dfree_t(c->d_hu);
     dfree_t(c->d_hr);
+    dfree_t(c->d_ws_sum);
+    dfree_t(c->d_wr_sum);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -880,6 +888,7 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
         if (dn_run_pass(c, part, true, false, false)) return -1;
         HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->d_Ws, c->d_Wr, c->d_gu_shot,
                                              c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits,
+                                             c->p.eprecond, c->d_ws_sum, c->d_wr_sum, b0 == 0 && !accumulate,
                                              c->stream));
         b0 += n;
     }
@@ -904,6 +913,61 @@ extern "C" int dn_gradient(dn_ctx *c, int shot0, int nshots, int accumulate, dou
     }
     if (l2) *l2 = L2;
     if (energy) *energy = EN;
+    return 0;
+}
+
+extern "C" int dn_get_energy_maps(dn_ctx *c, float *ws, float *wr)
+{
+    if (!c) return -1;
+    const size_t n = (size_t)c->g.nx * c->g.ny * sizeof(float);
+    if (ws) HIPCHK(c, hipMemcpy(ws, c->d_ws_sum, n, hipMemcpyDeviceToHost));
+    if (wr) HIPCHK(c, hipMemcpy(wr, c->d_wr_sum, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int dn_hessian_probe(dn_ctx *c, int shot0, int nshots, int nprobes, float eps, unsigned int seed,
+                                float *diag_vs)
+{
+    if (!c) return -1;
+    if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots || nprobes < 1 || !(eps > 0.0f) || !diag_vs)
+        return dn_fail(c, "dn_hessian_probe: bad arguments");
+    if (!c->have_C || !c->d_obs) return dn_fail(c, "dn_hessian_probe: model and observed data must be set");
+    const size_t P = (size_t)c->g.plane, nxny = (size_t)c->g.nx * c->g.ny;
+    float *vs0 = nullptr, *g0 = nullptr, *diag = nullptr;
+    auto cleanup = [&]() {
+        dfree_t(vs0);
+        dfree_t(g0);
+        dfree_t(diag);
+    };
+    if (dalloc(&vs0, P) != hipSuccess || dalloc(&g0, nxny) != hipSuccess || dalloc(&diag, nxny) != hipSuccess) {
+        cleanup();
+        return dn_fail(c, "dn_hessian_probe: out of device memory");
+    }
+    std::vector<int> shots(nshots);
+    for (int s = 0; s < nshots; ++s) shots[s] = shot0 + s;
+    int rc = 0;
+    auto step = [&](hipError_t e) {
+        if (rc == 0 && e != hipSuccess) rc = dn_fail(c, "dn_hessian_probe: %s", hipGetErrorString(e));
+    };
+    step(hipMemcpyAsync(vs0, c->d_vs, P * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (rc == 0) rc = dn_gradient_shots(c, shots, false);
+    step(hipMemcpyAsync(g0, c->d_gvs_out, nxny * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    step(launch_fill(diag, nxny, 0.0f, c->stream));
+    for (int k = 0; k < nprobes && rc == 0; ++k) {
+        step(launch_probe_perturb(c->g, vs0, c->d_vs, eps, seed, k, c->stream));
+        if (rc == 0) rc = dn_prepare_model(c);
+        if (rc == 0) rc = dn_gradient_shots(c, shots, false);
+        step(launch_probe_accumulate(c->g, vs0, g0, c->d_gvs_out, eps, seed, k, diag, c->stream));
+    }
+    // restore the model (also after a failure)
+    (void)hipMemcpyAsync(c->d_vs, vs0, P * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+    if (dn_prepare_model(c) != 0 && rc == 0) rc = -1;
+    std::vector<float> h(nxny);
+    if (rc == 0) step(hipMemcpyAsync(h.data(), diag, nxny * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    step(hipStreamSynchronize(c->stream));
+    cleanup();
+    if (rc) return rc;
+    for (size_t t = 0; t < nxny; ++t) diag_vs[t] = h[t] / (float)nprobes;
     return 0;
 }
 

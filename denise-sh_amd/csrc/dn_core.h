@@ -76,6 +76,7 @@ struct dn_ctx {
     // pseudo-Hessian (forward passes accumulate it when hess_mode is set)
     bool hess_mode = false;
     float *d_hu = nullptr, *d_hr = nullptr;
+    float *d_ws_sum = nullptr, *d_wr_sum = nullptr;   // energy maps of the last gradient
     // FWI optimiser state (dn_fwi.cpp) and an optional gradient preconditioner
     dn_fwi_state *fwi = nullptr;
     int (*precond_hook)(dn_ctx *, float *grad_vs, float *grad_rho) = nullptr;

@@ -117,9 +117,18 @@ hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map,
                            const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st);
 
 // energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)
+// (precond 0: EPRECOND 0, plain sum).  ws_sum/wr_sum collect the energy maps
+// over shots (reset when first).
 hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
                                       const float *gu_shot, const float *grho_shot, float *gu, float *grho,
-                                      float C_vs, float C_rho, unsigned int *maxbits, hipStream_t st);
+                                      float C_vs, float C_rho, unsigned int *maxbits, int precond, float *ws_sum,
+                                      float *wr_sum, int first, hipStream_t st);
+
+// finite-difference Hessian probes: Rademacher +-1 of (seed, probe, cell)
+hipError_t launch_probe_perturb(const Geo &g, const float *vs0_pad, float *vs_pad, float eps, unsigned int seed,
+                                int probe, hipStream_t st);
+hipError_t launch_probe_accumulate(const Geo &g, const float *vs0_pad, const float *g0, const float *g1, float eps,
+                                   unsigned int seed, int probe, float *diag, hipStream_t st);
 
 // chain rule (denise.c:1756-1811)
 hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, const float *grho,

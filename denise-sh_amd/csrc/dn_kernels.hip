@@ -595,37 +595,100 @@ __global__ void k_we_max(int nxny, const float *Ws, const float *Wr, unsigned in
 
 __global__ void k_eprecond_acc(int nshots, int nxny, const float *Ws, const float *Wr, const float *gu_shot,
                                const float *grho_shot, float *gu, float *grho, float C_vs, float C_rho,
-                               const unsigned int *maxbits)
+                               const unsigned int *maxbits, int precond, float *ws_sum, float *wr_sum, int first)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nxny) return;
     const float EPSILON_WE = 0.005f;
     float au = gu[t], ar = grho[t];
+    float es = first ? 0.0f : ws_sum[t], er = first ? 0.0f : wr_sum[t];
     for (int s = 0; s < nshots; ++s) {
         const long long o = (long long)s * nxny + t;
-        float we = (float)sqrt((double)(Ws[o] * Wr[o]));
-        we = we + (EPSILON_WE * __uint_as_float(maxbits[s]));
-        const float su = gu_shot[o] / (we * C_vs * C_vs);
-        const float sr = grho_shot[o] / (we * C_rho * C_rho);
-        ar = ar + sr;
-        au = au + su;
+        es = es + Ws[o];
+        er = er + Wr[o];
+        if (precond) {
+            float we = (float)sqrt((double)(Ws[o] * Wr[o]));
+            we = we + (EPSILON_WE * __uint_as_float(maxbits[s]));
+            const float su = gu_shot[o] / (we * C_vs * C_vs);
+            const float sr = grho_shot[o] / (we * C_rho * C_rho);
+            ar = ar + sr;
+            au = au + su;
+        } else {
+            ar = ar + grho_shot[o];
+            au = au + gu_shot[o];
+        }
     }
     gu[t] = au;
     grho[t] = ar;
+    ws_sum[t] = es;
+    wr_sum[t] = er;
 }
 
 hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
                                       const float *gu_shot, const float *grho_shot, float *gu, float *grho,
-                                      float C_vs, float C_rho, unsigned int *maxbits, hipStream_t st)
+                                      float C_vs, float C_rho, unsigned int *maxbits, int precond, float *ws_sum,
+                                      float *wr_sum, int first, hipStream_t st)
 {
     const int nxny = nx * ny;
-    hipError_t e = launch_fill_u32(maxbits, (size_t)nshots, 0u, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, Ws, Wr, maxbits);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (precond) {
+        hipError_t e = launch_fill_u32(maxbits, (size_t)nshots, 0u, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, Ws, Wr, maxbits);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_eprecond_acc, dim3((nxny + 255) / 256), dim3(256), 0, st, nshots, nxny, Ws, Wr, gu_shot,
-                       grho_shot, gu, grho, C_vs, C_rho, maxbits);
+                       grho_shot, gu, grho, C_vs, C_rho, maxbits, precond, ws_sum, wr_sum, first);
+    return hipGetLastError();
+}
+
+// Rademacher sign of (seed, probe, cell): one round of a 32-bit integer
+// hash (splitmix-style finaliser) of the combined key, top bit.
+__device__ inline float rademacher(unsigned int seed, int probe, int cell)
+{
+    unsigned int x = seed * 0x9E3779B9u ^ ((unsigned int)probe * 0x85EBCA6Bu) ^ ((unsigned int)cell * 0xC2B2AE35u);
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return (x & 0x80000000u) ? -1.0f : 1.0f;
+}
+
+__global__ void k_probe_perturb(Geo g, const float *vs0, float *vs, float eps, unsigned int seed, int probe)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int j = t / g.nx + 1, i = t % g.nx + 1;
+    const long long o = gidx(g, j, i);
+    vs[o] = vs0[o] * (1.0f + eps * rademacher(seed, probe, t));
+}
+
+__global__ void k_probe_acc(Geo g, const float *vs0, const float *g0, const float *g1, float eps, unsigned int seed,
+                            int probe, float *diag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int j = t / g.nx + 1, i = t % g.nx + 1;
+    const float v = rademacher(seed, probe, t);
+    const float dm = eps * vs0[gidx(g, j, i)];
+    diag[t] = diag[t] + v * (g1[t] - g0[t]) / dm;
+}
+
+hipError_t launch_probe_perturb(const Geo &g, const float *vs0_pad, float *vs_pad, float eps, unsigned int seed,
+                                int probe, hipStream_t st)
+{
+    const int n = g.nx * g.ny;
+    hipLaunchKernelGGL(k_probe_perturb, dim3((n + 255) / 256), dim3(256), 0, st, g, vs0_pad, vs_pad, eps, seed, probe);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_accumulate(const Geo &g, const float *vs0_pad, const float *g0, const float *g1, float eps,
+                                   unsigned int seed, int probe, float *diag, hipStream_t st)
+{
+    const int n = g.nx * g.ny;
+    hipLaunchKernelGGL(k_probe_acc, dim3((n + 255) / 256), dim3(256), 0, st, g, vs0_pad, g0, g1, eps, seed, probe,
+                       diag);
     return hipGetLastError();
 }
 

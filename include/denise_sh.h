@@ -148,6 +148,7 @@ typedef struct dn_params {
     float npower, k_max_pml;
     int dtinv;           /* DTINV: snapshot every dtinv-th step            */
     int snap_precision;  /* 0: fp32, 1: bf16, 2: fp16 (per-snapshot scale) */
+    int eprecond;        /* EPRECOND: 1 energy preconditioning (eprecond1.c), 0 none */
 } dn_params;
 
 /* Context on the current HIP device; NULL on failure (dn_last_error(NULL)). */
@@ -189,6 +190,18 @@ int dn_set_observed(dn_ctx *ctx, const float *obs);
 /* Maximum number of shots advanced together (0 = automatic from free HBM)
  * and rows per workgroup segment of the time-step kernel (0 = automatic). */
 int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
+
+/* Source- and receiver-side energy maps of the last gradient, summed over
+ * its shots: sum_s Ws, sum_s Wr (eprecond.c), host [ny][nx]. */
+int dn_get_energy_maps(dn_ctx *ctx, float *ws, float *wr);
+
+/* Diagonal of the misfit Hessian w.r.t. vs estimated with nprobes
+ * finite-difference Hessian-vector probes (the idea of the reference's
+ * unused hessian_FD.c): diag ~ mean_k v_k * (g(m + eps*vs*v_k) - g(m)) /
+ * (eps*vs), Rademacher v_k from seed, g = the vs gradient of shots
+ * [shot0, shot0+nshots) with the context's EPRECOND.  The model is restored
+ * afterwards.  Output host [ny][nx]. */
+int dn_hessian_probe(dn_ctx *ctx, int shot0, int nshots, int nprobes, float eps, unsigned int seed, float *diag_vs);
 
 /* Pseudo-Hessian diagonal (HESSIAN = 1, hess_save.c:2-17, hessian.c:91-94,
  * hessian_out.c:25-31): forward modelling of shots [shot0, shot0+nshots)

@@ -43,13 +43,13 @@ DH = 0.25
 
 
 def setup_case(nx, ny, fdorder, L, free_surf, fw, nt, fc, nshots=2, dtinv=1, snap_precision=0, seed=3,
-               nts=None):
+               nts=None, eprecond=1):
     qs = 20.0 if L > 0 else None
     vs, rho, q = model(nx, ny, DH, seed=seed, qs=qs)
     fl = [6.25, 50.0, 12.0, 30.0][:L]
     dn = product()
     ctx = dn.Context(nx, ny, fdorder=fdorder, maxrelerror=1, fw=fw, free_surf=free_surf, L=L, fl=fl, dt=DT, dh=DH,
-                     damping=300.0, fpml=fc, dtinv=dtinv, snap_precision=snap_precision)
+                     damping=300.0, fpml=fc, dtinv=dtinv, snap_precision=snap_precision, eprecond=eprecond)
     ctx.set_model(vs, rho, q)
     rec_i = np.arange(fw + 3, nx - fw - 2, 3, dtype=np.int32)
     rec_j = np.full_like(rec_i, 2)
@@ -61,7 +61,7 @@ def setup_case(nx, ny, fdorder, L, free_surf, fw, nt, fc, nshots=2, dtinv=1, sna
     ctx.set_shots(src_i, src_j, sig, ntv)
     from pyoracle import make_cfg
     cfg = make_cfg(nx, ny, fdorder=fdorder, maxrelerror=1, fw=fw, free_surf=free_surf, L=L, nt=nt, dtinv=dtinv,
-                   dt=DT, dh=DH, damping=300.0, fpml=fc, fl=fl)
+                   dt=DT, dh=DH, damping=300.0, fpml=fc, fl=fl, eprecond=eprecond)
     return ctx, cfg, dict(vs=vs, rho=rho, qs=q, rec_i=rec_i, rec_j=rec_j, src_i=src_i, src_j=src_j, sig=sig)
 
 
@@ -253,3 +253,56 @@ def test_pseudo_hessian():
         (ev, rel_l2(hv, r64[0]), er, rel_l2(hr, r64[1])))
     assert np.max(r32[0]) > 0
     assert ev <= 1e-5 and er <= 1e-5
+
+
+def _rademacher(seed, probe, n):
+    """numpy restatement of the device's Rademacher signs (32-bit hash)."""
+    m = np.uint64(0xFFFFFFFF)
+    cell = np.arange(n, dtype=np.uint64)
+    x = ((np.uint64(seed) * np.uint64(0x9E3779B9)) & m) ^ ((np.uint64(probe) * np.uint64(0x85EBCA6B)) & m) ^ \
+        ((cell * np.uint64(0xC2B2AE35)) & m)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & m
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & m
+    x ^= x >> np.uint64(16)
+    return np.where(x & np.uint64(0x80000000), -1.0, 1.0).astype(np.float32)
+
+
+def test_gradient_without_eprecond_and_hessian_probe():
+    """EPRECOND 0 gradient against the oracle, energy maps against the
+    oracle's Ws/Wr sums, and one finite-difference Hessian-vector probe
+    against the same probe formed from oracle gradients."""
+    nx, ny, nt = 90, 40, 700
+    say("case eprecond 0 + Hessian probe")
+    ctx, cfg, d = setup_case(nx, ny, 4, 0, 1, 10, nt, 60.0, nshots=2, dtinv=1, eprecond=0)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 2)
+    ctx.set_observed(obs)
+    ctx.gradient()
+    g = ctx.get_gradient()
+    o32 = oracle("f32")
+    r = o32.gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    ev = rel_l2(g[0], r["grad_vs"])
+    say("  EPRECOND 0 grad_vs rel L2 vs f32 %.3e" % ev)
+    assert ev <= GRAD_TOL and rel_l2(g[1], r["grad_rho"]) <= GRAD_TOL
+    ws, wr = ctx.energy_maps()
+    ws_ref = np.zeros((ny, nx))
+    for s in range(2):
+        _, _, W = o32.forward(cfg, d["vs"], d["rho"], d["qs"], d["src_i"][s:s + 1], d["src_j"][s:s + 1], d["sig"][s],
+                              d["rec_i"], d["rec_j"], want_snaps=True)
+        nd = cfg.fdorder // 2 + 1
+        ws_ref += W[nd:nd + ny, nd:nd + nx]
+    say("  Ws map rel L2 %.3e" % rel_l2(ws, ws_ref))
+    assert rel_l2(ws, ws_ref) <= 1e-5 and np.all(wr >= 0) and np.max(wr) > 0
+    eps, seed = 1e-2, 7
+    diag = ctx.hessian_probe(nprobes=1, eps=eps, seed=seed)
+    v = _rademacher(seed, 0, nx * ny).reshape(ny, nx)
+    vs1 = (d["vs"] * (np.float32(1.0) + np.float32(eps) * v)).astype(np.float32)
+    r1 = o32.gradient(cfg, vs1, d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    ref = v * (r1["grad_vs"] - r["grad_vs"]) / (np.float32(eps) * d["vs"])
+    e = rel_l2(diag, ref)
+    say("  Hessian probe rel L2 vs oracle-built probe %.3e" % e)
+    assert e <= 1e-4
+    # the model is restored: the gradient is unchanged
+    ctx.gradient()
+    assert np.array_equal(ctx.get_gradient()[0], g[0])
