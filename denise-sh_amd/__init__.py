@@ -24,6 +24,8 @@ EXPORTS = [
     "dn_read_receivers", "dn_read_model_file", "dn_write_model_file", "dn_read_model", "dn_write_su", "dn_read_su",
     "dn_fwi_start", "dn_fwi_iteration", "dn_fwi_get_model", "dn_fwi_get_direction", "dn_checkfd",
     "dn_set_exec", "dn_autotune", "dn_pseudo_hessian", "dn_get_energy_maps", "dn_hessian_probe",
+    "dn_unet_create", "dn_unet_destroy", "dn_unet_num_params", "dn_unet_get_params", "dn_unet_set_params",
+    "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
 ]
 
 
@@ -39,6 +41,26 @@ class Params(C.Structure):
                 ("damping", C.c_float), ("fpml", C.c_float), ("npower", C.c_float), ("k_max_pml", C.c_float),
                 ("dtinv", C.c_int), ("snap_precision", C.c_int), ("eprecond", C.c_int)]
 
+
+class FwiOpts(C.Structure):
+    _fields_ = [("grad_method", C.c_int), ("nlbfgs", C.c_int), ("eps_scale", C.c_float), ("stepmax", C.c_int),
+                ("scalefac", C.c_float), ("testshot_start", C.c_int), ("testshot_end", C.c_int),
+                ("testshot_incr", C.c_int), ("vs_lower", C.c_float), ("vs_upper", C.c_float),
+                ("rho_lower", C.c_float), ("rho_upper", C.c_float), ("inv_vs_iter", C.c_int),
+                ("inv_rho_iter", C.c_int), ("scalerho", C.c_float), ("min_iter", C.c_int), ("pro", C.c_float)]
+
+
+class FwiRecord(C.Structure):
+    _fields_ = [("stage", C.c_int), ("iter", C.c_int), ("iter_true", C.c_int), ("opteps_vs", C.c_float),
+                ("epst1", C.c_float * 3), ("L2t", C.c_float * 4), ("M2_vs", C.c_float), ("M2_rho", C.c_float),
+                ("eps_u", C.c_float), ("eps_rho", C.c_float), ("l2", C.c_double), ("energy", C.c_double),
+                ("step1", C.c_int), ("step3", C.c_int), ("test_forwards", C.c_int), ("next_stage", C.c_int),
+                ("precond", C.c_int)]
+
+
+FWI_DEFAULTS = dict(grad_method=2, nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1,
+                    testshot_end=1, testshot_incr=1, vs_lower=0.0, vs_upper=1e9, rho_lower=0.0, rho_upper=1e9,
+                    inv_vs_iter=0, inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0)
 
 _lib = None
 
@@ -78,6 +100,17 @@ def lib():
         L.dn_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip]
         L.dn_set_shots_multi.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, ip, fp]
         L.dn_set_signals.argtypes = [C.c_void_p, fp]
+        L.dn_unet_create.restype = C.c_void_p
+        L.dn_unet_create.argtypes = [C.c_uint]
+        L.dn_unet_destroy.argtypes = [C.c_void_p]
+        L.dn_unet_num_params.restype = C.c_longlong
+        L.dn_unet_num_params.argtypes = [C.c_void_p]
+        L.dn_unet_get_params.argtypes = [C.c_void_p, fp]
+        L.dn_unet_set_params.argtypes = [C.c_void_p, fp]
+        L.dn_unet_forward.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, fp, fp]
+        L.dn_unet_loss_grad.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, fp, fp, C.POINTER(C.c_double), fp]
+        L.dn_unet_train_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, fp, fp, C.c_float,
+                                         C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -173,6 +206,42 @@ class Context:
         obs = np.ascontiguousarray(obs, np.float32)
         assert obs.shape == (self.nshots, self.ntr, self.nt_max)
         self._chk(lib().dn_set_observed(self._h, _f(obs)))
+
+    # ---- FWI (dn_fwi_*) ----
+    def fwi_start(self, **opts):
+        o = FwiOpts()
+        for k, v in dict(FWI_DEFAULTS, **opts).items():
+            setattr(o, k, v)
+        lib().dn_fwi_start.argtypes = [C.c_void_p, C.POINTER(FwiOpts)]
+        self._chk(lib().dn_fwi_start(self._h, C.byref(o)))
+
+    def fwi_iteration(self):
+        r = FwiRecord()
+        lib().dn_fwi_iteration.argtypes = [C.c_void_p, C.POINTER(FwiRecord)]
+        self._chk(lib().dn_fwi_iteration(self._h, C.byref(r)))
+        out = {k: getattr(r, k) for k, _ in FwiRecord._fields_}
+        out["epst1"] = list(r.epst1)
+        out["L2t"] = list(r.L2t)
+        return out
+
+    def fwi_model(self):
+        vs = np.zeros((self.ny, self.nx), np.float32)
+        rho = np.zeros((self.ny, self.nx), np.float32)
+        lib().dn_fwi_get_model.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        self._chk(lib().dn_fwi_get_model(self._h, _f(vs), _f(rho)))
+        return vs, rho
+
+    def set_learned(self, net, judge_every=5):
+        lib().dn_fwi_set_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        self._chk(lib().dn_fwi_set_learned(self._h, net._h if net is not None else None, judge_every))
+        self._net = net      # keep the network alive while the context uses it
+
+    def precond_features(self):
+        h8, w8 = (self.ny + 7) // 8 * 8, (self.nx + 7) // 8 * 8
+        x = np.zeros((3, h8, w8), np.float32)
+        lib().dn_precond_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        self._chk(lib().dn_precond_features(self._h, _f(x)))
+        return x
 
     def set_exec(self, graph_steps=-1):
         self._chk(lib().dn_set_exec(self._h, graph_steps))
@@ -395,3 +464,60 @@ def checkfd(vs, rho, qs, fdorder, maxrelerror, fl, dt, dh, ts, fw):
     if rc < 0:
         raise DeniseError("dn_checkfd: bad arguments")
     return rc, {k: getattr(out, k) for k, _ in CheckFD._fields_}
+
+
+class UNet:
+    """The learned diagonal-Hessian preconditioner network (dn_unet_*)."""
+
+    def __init__(self, seed=1):
+        self._h = lib().dn_unet_create(seed)
+        if not self._h:
+            raise DeniseError(lib().dn_last_error(None).decode())
+        self.nparams = lib().dn_unet_num_params(self._h)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise DeniseError(lib().dn_last_error(None).decode())
+
+    def close(self):
+        if self._h:
+            lib().dn_unet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self._chk(lib().dn_unet_get_params(self._h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        self._chk(lib().dn_unet_set_params(self._h, _f(np.ascontiguousarray(p, np.float32))))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n, c, h, w = x.shape
+        y = np.zeros((n, h, w), np.float32)
+        self._chk(lib().dn_unet_forward(self._h, n, h, w, _f(x), _f(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, c, h, w = x.shape
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double()
+        self._chk(lib().dn_unet_loss_grad(self._h, n, h, w, _f(x), _f(t), C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, c, h, w = x.shape
+        loss = C.c_double()
+        self._chk(lib().dn_unet_train_step(self._h, n, h, w, _f(x), _f(t), lr, C.byref(loss)))
+        return loss.value

@@ -33,6 +33,8 @@ SOURCES = [
     ("dn_dropin.cpp", False),
     ("dn_fwi.cpp", False),
     ("dn_io.cpp", False),
+    ("dn_unet.hip", True),
+    ("dn_unet.cpp", False),
 ]
 
 

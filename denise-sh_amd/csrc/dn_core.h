@@ -9,6 +9,9 @@
 #include <vector>
 
 struct dn_fwi_state;
+struct dn_unet;
+// U-Net forward on device buffers (dn_unet.cpp): x [n][3][H][W] -> y [n][H][W]
+int dn_unet_forward_device(dn_unet *u, int n, int H, int W, const float *x_dev, float *y_dev);
 
 struct dn_ctx {
     dn_params p{};
@@ -80,6 +83,9 @@ struct dn_ctx {
     // FWI optimiser state (dn_fwi.cpp) and an optional gradient preconditioner
     dn_fwi_state *fwi = nullptr;
     int (*precond_hook)(dn_ctx *, float *grad_vs, float *grad_rho) = nullptr;
+    // learned preconditioner (owned by the caller) and the judge interval
+    dn_unet *unet = nullptr;
+    int judge_every = 5;
 };
 
 int dn_fail(dn_ctx *c, const char *fmt, ...);

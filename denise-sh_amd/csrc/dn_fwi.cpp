@@ -98,6 +98,12 @@ __global__ void k_axpy(float *y, const float *x, double c, size_t n)
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
         y[t] = (float)((double)y[t] + c * (double)x[t]);
 }
+// y = d * x (learned diagonal scaling)
+__global__ void k_mul(float *y, const float *d, const float *x, size_t n)
+{
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        y[t] = d[t] * x[t];
+}
 // PCG: dir = new + beta * dir
 __global__ void k_pcg_dir(float *dir, const float *nw, float beta, size_t n)
 {
@@ -162,13 +168,19 @@ struct dn_fwi_state {
     float *pcg_new = nullptr, *pcg_old = nullptr, *pcg_dir = nullptr;              // [u; rho]
     double *part = nullptr, *scalar = nullptr;
     unsigned int *maxbits = nullptr;
+    // learned preconditioner: diagonal D of the initial L-BFGS Hessian for
+    // the vs part, the learned direction and a spare copy of the plain one
+    float *D = nullptr, *r2 = nullptr, *gu_l = nullptr, *grho_l = nullptr, *gu_p = nullptr, *grho_p = nullptr;
+    float *feat = nullptr, *pred = nullptr;
+    double last_plain_drop = -1.0;
 };
 
 static void fwi_free(dn_fwi_state *f)
 {
     if (!f) return;
-    void *ps[] = {f->m_vs, f->m_rho, f->m0_vs, f->m0_rho, f->t_vs, f->t_rho, f->gu, f->grho, f->y, f->s, f->q,
-                  f->r, f->gold, f->mold, f->pcg_new, f->pcg_old, f->pcg_dir, f->part, f->scalar, f->maxbits};
+    void *ps[] = {f->m_vs, f->m_rho, f->m0_vs, f->m0_rho, f->t_vs,   f->t_rho,  f->gu,   f->grho,   f->y,
+                  f->s,    f->q,     f->r,    f->gold,   f->mold, f->pcg_new, f->pcg_old, f->pcg_dir, f->part,
+                  f->scalar, f->maxbits, f->D, f->r2, f->gu_l, f->grho_l, f->gu_p, f->grho_p, f->feat, f->pred};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete f;
@@ -310,8 +322,11 @@ extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
     return 0;
 }
 
-// L-BFGS (store_LBFGS_SH.c, LBFGS.c, extract_LBFGS_SH.c) on [rho; u] vectors
-static int lbfgs_direction(dn_ctx *c)
+// L-BFGS (store_LBFGS_SH.c, LBFGS.c, extract_LBFGS_SH.c) on [rho; u] vectors.
+// With D (learned preconditioner) a second direction with the initial
+// Hessian gamma * diag(1, D) is formed from the same history into
+// f->grho_l / f->gu_l.
+static int lbfgs_direction(dn_ctx *c, const float *D)
 {
     dn_fwi_state *f = c->fwi;
     const size_t n = f->nxny, v2 = 2 * n;
@@ -357,16 +372,41 @@ static int lbfgs_direction(dn_ctx *c)
                                alpha[k] - beta, v2);
         }
         HIPCHK(c, hipGetLastError());
+        if (D) {
+            // r2 = gamma * [q_rho; D q_u], then the same second loop
+            HIPCHK(c, launch_fill(f->r2, v2, 0.0f, c->stream));
+            hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, c->stream, f->r2 + n, D, f->q + n, n);
+            HIPCHK(c, hipMemcpyAsync(f->r2, f->q, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c->stream, f->r2, f->r2, (float)gamma, v2);
+            for (int k = 1; k <= NL; ++k) {
+                double d;
+                if (dot(c, f->y + (size_t)(k - 1) * v2, f->r2, v2, &d)) return -1;
+                const double beta = rho[k] * d;
+                hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, c->stream, f->r2, f->s + (size_t)(k - 1) * v2,
+                                   alpha[k] - beta, v2);
+            }
+            HIPCHK(c, hipGetLastError());
+        }
     }
     // save old model and normalised gradient (extract_LBFGS_SH.c:68-178)
     HIPCHK(c, hipMemcpyAsync(f->gold, f->grho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(f->gold + n, f->gu, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(f->mold, f->m_rho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(f->mold + n, f->m_vs, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (D && iter == 1) {
+        // first iteration: the (normalised) gradient, scaled by D for vs
+        HIPCHK(c, hipMemcpyAsync(f->grho_l, f->grho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, c->stream, f->gu_l, D, f->gu, n);
+        HIPCHK(c, hipGetLastError());
+    }
     if (iter > 1) {
         // extract and denormalise (extract_LBFGS_SH.c:31-64)
         hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, f->grho, f->r, c->C_rho, n);
         hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, f->gu, f->r + n, c->C_vs, n);
+        if (D) {
+            hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, f->grho_l, f->r2, c->C_rho, n);
+            hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, c->stream, f->gu_l, f->r2 + n, c->C_vs, n);
+        }
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -489,6 +529,99 @@ static int line_search(dn_ctx *c, double energy_sum, float *L2t, float *epst1, i
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Learned preconditioner: input maps and the diagonal D
+// ---------------------------------------------------------------------------
+static int features_host(dn_ctx *c, std::vector<float> &x, std::vector<float> &ws, std::vector<float> &wr, int &H8,
+                         int &W8)
+{
+    const Geo &g = c->g;
+    const int nx = g.nx, ny = g.ny;
+    const size_t nxny = (size_t)nx * ny;
+    H8 = (ny + 7) / 8 * 8;
+    W8 = (nx + 7) / 8 * 8;
+    ws.resize(nxny);
+    wr.resize(nxny);
+    std::vector<float> vs(nxny);
+    HIPCHK(c, hipMemcpy(ws.data(), c->d_ws_sum, nxny * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(wr.data(), c->d_wr_sum, nxny * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy2D(vs.data(), nx * sizeof(float), c->d_vs + gidx(g, 1, 1), g.pitch * sizeof(float),
+                          nx * sizeof(float), ny, hipMemcpyDeviceToHost));
+    double mws = 0.0, mwr = 0.0, svs = 0.0;
+    for (size_t t = 0; t < nxny; ++t) {
+        mws = std::max(mws, (double)ws[t]);
+        mwr = std::max(mwr, (double)wr[t]);
+        svs += vs[t];
+    }
+    if (!(mws > 0.0) || !(mwr > 0.0)) return dn_fail(c, "learned preconditioner: energy maps are empty (run a gradient)");
+    const double mvs = svs / (double)nxny;
+    x.assign((size_t)3 * H8 * W8, 0.0f);
+    for (int j = 0; j < H8; ++j)
+        for (int i = 0; i < W8; ++i) {
+            const size_t o = (size_t)std::min(j, ny - 1) * nx + std::min(i, nx - 1);
+            const size_t d = (size_t)j * W8 + i;
+            x[d] = (float)(std::log10(std::max((double)ws[o] / mws, 1e-12)) / 12.0 + 1.0);
+            x[(size_t)H8 * W8 + d] = (float)(std::log10(std::max((double)wr[o] / mwr, 1e-12)) / 12.0 + 1.0);
+            x[(size_t)2 * H8 * W8 + d] = (float)(vs[o] / mvs - 1.0);
+        }
+    return 0;
+}
+
+extern "C" int dn_precond_features(dn_ctx *c, float *x)
+{
+    if (!c || !x) return dn_fail(c, "dn_precond_features: NULL argument");
+    std::vector<float> h, ws, wr;
+    int H8, W8;
+    if (features_host(c, h, ws, wr, H8, W8)) return -1;
+    std::memcpy(x, h.data(), h.size() * sizeof(float));
+    return 0;
+}
+
+extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
+{
+    if (!c) return -1;
+    if (judge_every < 0) return dn_fail(c, "dn_fwi_set_learned: judge_every must be >= 0");
+    c->unet = net;
+    c->judge_every = judge_every;
+    return 0;
+}
+
+// D = We / H_pred (We = sqrt(sum Ws * sum Wr) + 0.005 max, the energy
+// weight of eprecond1.c; H_pred = 10^net), normalised to mean 1 and kept in
+// [1e-2, 1e2]: the learned diagonal relative to the plain preconditioning
+static int learned_diagonal(dn_ctx *c)
+{
+    dn_fwi_state *f = c->fwi;
+    const Geo &g = c->g;
+    const size_t nxny = f->nxny;
+    std::vector<float> x, ws, wr;
+    int H8, W8;
+    if (features_host(c, x, ws, wr, H8, W8)) return -1;
+    const size_t P8 = (size_t)H8 * W8;
+    if (!f->feat) HIPCHK(c, hipMalloc((void **)&f->feat, 3 * P8 * sizeof(float)));
+    if (!f->pred) HIPCHK(c, hipMalloc((void **)&f->pred, P8 * sizeof(float)));
+    HIPCHK(c, hipMemcpy(f->feat, x.data(), 3 * P8 * sizeof(float), hipMemcpyHostToDevice));
+    if (dn_unet_forward_device(c->unet, 1, H8, W8, f->feat, f->pred)) return dn_fail(c, "%s", dn_last_error(nullptr));
+    std::vector<float> pred(P8), D(nxny);
+    HIPCHK(c, hipMemcpy(pred.data(), f->pred, P8 * sizeof(float), hipMemcpyDeviceToHost));
+    double wmax = 0.0;
+    for (size_t t = 0; t < nxny; ++t) wmax = std::max(wmax, std::sqrt((double)ws[t] * (double)wr[t]));
+    double sum = 0.0;
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+            const size_t t = (size_t)j * g.nx + i;
+            const double we = std::sqrt((double)ws[t] * (double)wr[t]) + 0.005 * wmax;
+            const double h = std::pow(10.0, std::min(30.0, std::max(-30.0, (double)pred[(size_t)j * W8 + i])));
+            D[t] = (float)(we / h);
+            sum += D[t];
+        }
+    const double mean = sum / (double)nxny;
+    for (float &d : D) d = (float)std::min(1e2, std::max(1e-2, (double)d / mean));
+    if (!f->D) HIPCHK(c, hipMalloc((void **)&f->D, nxny * sizeof(float)));
+    HIPCHK(c, hipMemcpy(f->D, D.data(), nxny * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
 {
     if (!c || !c->fwi) return dn_fail(c, "dn_fwi_iteration: call dn_fwi_start first");
@@ -545,15 +678,80 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
     HIPCHK(c, hipMemcpyAsync(f->gu, c->d_gvs_out, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(f->grho, c->d_grho_out, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     if (c->precond_hook && c->precond_hook(c, f->gu, f->grho)) return -1;
+    const bool learned = (c->unet != nullptr) && (f->o.grad_method == 2);
+    if (learned) {
+        if (!f->r2) {
+            HIPCHK(c, hipMalloc((void **)&f->r2, 2 * n * sizeof(float)));
+            float **bs[] = {&f->gu_l, &f->grho_l, &f->gu_p, &f->grho_p};
+            for (float **b : bs) HIPCHK(c, hipMalloc((void **)b, n * sizeof(float)));
+        }
+        if (learned_diagonal(c)) return -1;
+    }
     if (f->o.grad_method == 1) {
         if (pcg_direction(c)) return -1;
     } else {
-        if (lbfgs_direction(c)) return -1;
+        if (lbfgs_direction(c, learned ? f->D : nullptr)) return -1;
     }
     // line search on the test shots
     int step1 = 0, step3 = 0, nfwd = 0;
     float eps_scale = 0.0f;
-    if (line_search(c, energy, L2t, epst1, &step1, &step3, &eps_scale, &nfwd)) return -1;
+    R.precond = 0;
+    if (!learned) {
+        if (line_search(c, energy, L2t, epst1, &step1, &step3, &eps_scale, &nfwd)) return -1;
+    } else {
+        // learned direction first; the plain (eprecond) direction judges it
+        // every judge_every iterations and replaces it when its line search
+        // fails or its misfit drop falls below half the last plain drop
+        auto drop_of = [](const float *L, int st1) {
+            return st1 ? (double)(L[1] - std::min(L[2], L[3])) / (double)L[1] : 0.0;
+        };
+        HIPCHK(c, hipMemcpyAsync(f->gu_p, f->gu, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(f->grho_p, f->grho, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(f->gu, f->gu_l, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(f->grho, f->grho_l, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        float L2l[5], epl[4];
+        std::memcpy(L2l, L2t, sizeof(L2l));
+        std::memcpy(epl, epst1, sizeof(epl));
+        int s1l = 0, s3l = 0;
+        float epsl = 0.0f;
+        if (line_search(c, energy, L2l, epl, &s1l, &s3l, &epsl, &nfwd)) return -1;
+        const double dl = drop_of(L2l, s1l);
+        const bool judge = (f->last_plain_drop < 0.0) || (c->judge_every > 0 && (f->iter - 1) % c->judge_every == 0);
+        const bool fail = (s1l == 0) || (f->last_plain_drop > 0.0 && dl < 0.5 * f->last_plain_drop);
+        bool take_learned = true;
+        if (judge || fail) {
+            HIPCHK(c, hipMemcpyAsync(f->gu, f->gu_p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(f->grho, f->grho_p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            float L2p[5], epp[4];
+            std::memcpy(L2p, L2t, sizeof(L2p));
+            std::memcpy(epp, epst1, sizeof(epp));
+            int s1p = 0, s3p = 0;
+            float epsp = 0.0f;
+            if (line_search(c, energy, L2p, epp, &s1p, &s3p, &epsp, &nfwd)) return -1;
+            const double dp = drop_of(L2p, s1p);
+            f->last_plain_drop = dp;
+            take_learned = !fail && dl >= dp;
+            if (!take_learned) {
+                std::memcpy(L2t, L2p, sizeof(L2p));
+                std::memcpy(epst1, epp, sizeof(epp));
+                step1 = s1p;
+                step3 = s3p;
+                eps_scale = epsp;
+            } else {
+                HIPCHK(c, hipMemcpyAsync(f->gu, f->gu_l, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(f->grho, f->grho_l, n * sizeof(float), hipMemcpyDeviceToDevice,
+                                         c->stream));
+            }
+        }
+        if (take_learned) {
+            std::memcpy(L2t, L2l, sizeof(L2l));
+            std::memcpy(epst1, epl, sizeof(epl));
+            step1 = s1l;
+            step3 = s3l;
+            eps_scale = epsl;
+            R.precond = 1;
+        }
+    }
     float opteps_vs = eps_scale;
     if ((f->iter > f->min_iter_help) && (step1 == 0)) {
         eps_scale = 0.0f;

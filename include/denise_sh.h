@@ -134,6 +134,7 @@ int dn_checkfd(int nx, int ny, int fdorder, int maxrelerror, int L, const float 
  * ---------------------------------------------------------------------- */
 
 typedef struct dn_ctx dn_ctx;
+typedef struct dn_unet dn_unet;   /* learned preconditioner network (below) */
 
 typedef struct dn_params {
     int nx, ny;          /* NX, NY                                         */
@@ -322,6 +323,7 @@ typedef struct dn_fwi_record {
     int step1, step3;     /* line-search flags                                     */
     int test_forwards;    /* shot simulations spent in the line search             */
     int next_stage;       /* 1: the abort criterion ended the stage                */
+    int precond;          /* direction taken: 0 eprecond (plain), 1 learned         */
 } dn_fwi_record;
 
 /* Starts FWI on the current model, or a new stage (optimiser reset) when
@@ -330,9 +332,46 @@ int dn_fwi_start(dn_ctx *ctx, const dn_fwi_opts *opts);
 /* One iteration: gradient over all shots, direction, line search on the
  * test shots, model update; fills rec. */
 int dn_fwi_iteration(dn_ctx *ctx, dn_fwi_record *rec);
+/* Learned preconditioner for L-BFGS (NULL: off).  The network's
+ * prediction H of the vs diagonal Hessian gives the initial L-BFGS Hessian
+ * gamma * diag(We / H) for vs (We the eprecond1.c energy weight, normalised
+ * to mean 1); the plain direction (the reference's) judges it on the first
+ * iteration and every judge_every-th (0: never again), and replaces it when
+ * its line search fails or its misfit drop is below half the plain one.
+ * The network stays owned by the caller.  Not in the reference. */
+int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
+/* The network's input maps from the last gradient and the current model,
+ * host [3][H8][W8] with H8, W8 = ny, nx rounded up to multiples of 8 (edge
+ * rows/columns repeated): log10 of sum Ws and sum Wr relative to their
+ * maxima (floored at 1e-12) / 12 + 1, and vs / mean(vs) - 1. */
+int dn_precond_features(dn_ctx *ctx, float *x);
+
 /* Current FWI model / last search direction, host [ny][nx]. */
 int dn_fwi_get_model(dn_ctx *ctx, float *vs, float *rho);
 int dn_fwi_get_direction(dn_ctx *ctx, float *grad_vs, float *grad_rho);
+
+/* ------------------------------------------------------------------------
+ * Learned diagonal-Hessian preconditioner.  Not in the reference: a 4-level
+ * U-Net (3x3 convolutions, 16/32/48/64 channels, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores) mapping
+ * [source energy, receiver energy, vs] maps to the log of a per-cell
+ * diagonal-Hessian scaling.  Images are host [n][3][h][w] (inputs) and
+ * [n][h][w] (outputs, targets) float32, h and w multiples of 8.  Errors:
+ * dn_last_error(NULL).
+ * ---------------------------------------------------------------------- */
+dn_unet *dn_unet_create(unsigned int seed);
+void dn_unet_destroy(dn_unet *u);
+long long dn_unet_num_params(const dn_unet *u);
+/* FP32 master parameters, per layer W[3][3][co][ci] then b[co]. */
+int dn_unet_get_params(dn_unet *u, float *out);
+int dn_unet_set_params(dn_unet *u, const float *in);
+int dn_unet_forward(dn_unet *u, int n, int h, int w, const float *x, float *y);
+/* Mean squared error against t and its gradient w.r.t. the parameters
+ * (grad may be NULL); no update. */
+int dn_unet_loss_grad(dn_unet *u, int n, int h, int w, const float *x, const float *t, double *loss, float *grad);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr; loss
+ * is the value before the step. */
+int dn_unet_train_step(dn_unet *u, int n, int h, int w, const float *x, const float *t, float lr, double *loss);
 
 #ifdef __cplusplus
 }
