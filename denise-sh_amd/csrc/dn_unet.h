@@ -8,7 +8,7 @@
 namespace dn {
 
 struct ConvArgs {
-    int n, H, W, Ci, Co;       // Ci, Co multiples of 16, Co <= 64
+    int n, H, W, Ci, Co;       // Ci, Co multiples of 16
     const uint16_t *x;         // [n][H][W][Ci]
     const uint16_t *w;         // [3][3][Co][Ci]
     const float *bias;         // [Co] or null

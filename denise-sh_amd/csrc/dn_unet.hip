@@ -28,8 +28,9 @@ __device__ __forceinline__ float u_bf2f(uint16_t h) { return __uint_as_float(((u
 // channels of one tap, B = 16 input channels x 16 pixels of one row.
 // Operand layout (16x16x16): lane l supplies A[l%16][4(l/16)+j] and
 // B[4(l/16)+j][l%16], j = 0..3, and receives D[4(l/16)+j][l%16].
-// Workgroup: 4 waves, 64 consecutive pixels of one image row; each wave
-// keeps all Co/16 (<= 4) accumulator tiles of its 16 pixels.
+// Workgroup: 4 waves, 64 consecutive pixels of one image row and one group
+// of up to 64 output channels (blockIdx.z); each wave keeps the group's
+// (<= 4) accumulator tiles of its 16 pixels.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 {
@@ -38,7 +39,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
     const int img = row / a.H, y = row - img * a.H;
     const int xb = blockIdx.x * 64 + wave * 16;
     const int nl = lane & 15, kq = lane >> 4;
-    const int nmb = a.Co >> 4;
+    const int cg = blockIdx.z * 64;                 // first output channel of the group
+    const int nmb = min(4, (a.Co - cg) >> 4);
     f4v acc[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[m] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 #pragma unroll
                 for (int m = 0; m < 4; ++m)
                     if (m < nmb) {
-                        const s4v av = *(const s4v *)(wp + (long long)(m * 16 + nl) * a.Ci + c0 + 4 * kq);
+                        const s4v av = *(const s4v *)(wp + (long long)(cg + m * 16 + nl) * a.Ci + c0 + 4 * kq);
                         acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[m], 0, 0, 0);
                     }
             }
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
         if (m < nmb)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int co = m * 16 + 4 * kq + j;
+                const int co = cg + m * 16 + 4 * kq + j;
                 float v = acc[m][j];
                 if (a.bias) v += a.bias[co];
                 if (a.relu) v = fmaxf(v, 0.0f);
@@ -82,8 +84,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t st)
 {
-    if (a.Ci % 16 || a.Co % 16 || a.Co > 64 || a.Ci < 16) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((a.W + 63) / 64), (unsigned)(a.n * a.H));
+    if (a.Ci % 16 || a.Co % 16 || a.Ci < 16 || a.Co < 16) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.W + 63) / 64), (unsigned)(a.n * a.H), (unsigned)((a.Co + 63) / 64));
     hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -1,0 +1,57 @@
+"""The learned preconditioner end to end on the GPU: training pairs from
+finite-difference Hessian probes on randomised models, training of the MFMA
+U-Net, and FWI of a held-out model with and without it (the plain
+energy-preconditioned L-BFGS judges the learned direction).  Small sizes;
+the numbers are printed."""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from common import ROOT, product
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a)
+    sys.stdout.flush()
+
+
+def _pt():
+    spec = importlib.util.spec_from_file_location("precond_train", os.path.join(ROOT, "denise-sh_amd",
+                                                                                 "precond_train.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_learned_preconditioner_trains_and_is_judged():
+    dn = product()
+    pt = _pt()
+    nx, ny = 128, 48
+    geom = dict(nshots=6, nt=700, fc=40.0)
+    say("training pairs 16 x (1 + 4 probes) gradients at %dx%d" % (nx, ny))
+    X, T, sec = pt.make_pairs(dn, 16, nx, ny, nprobes=4, **geom)
+    say("  %.1f s" % sec)
+    assert np.all(np.isfinite(X)) and np.all(np.isfinite(T))
+    net = dn.UNet(seed=1)
+    hist = pt.train(net, X[:12], T[:12], steps=150, lr=2e-3, batch=6)
+    val0 = float(np.mean((0.0 * T[12:] + np.mean(T[:12]) - T[12:]) ** 2))
+    val = net.loss_grad(X[12:], T[12:])[0]
+    say("  training loss %.4f -> %.4f; held-out %.4f (constant predictor %.4f)" % (hist[0], hist[-1], val, val0))
+    assert hist[-1] < hist[0]
+    vt, v0, rho, qs = pt.random_model(nx, ny, 0.25, 999)
+    out = {}
+    for mode in ("plain", "learned"):
+        sv = pt.Survey(dn, nx, ny, **geom)
+        recs, vs = pt.run_fwi(sv, vt, v0, rho, qs, 6, net=net if mode == "learned" else None)
+        out[mode] = recs
+        sv.ctx.close()
+        say("  %-7s misfit %s  precond %s  %.2f s" % (mode, " ".join("%.4f" % r["L2t"][3] for r in recs),
+                                                     "".join(str(r["precond"]) for r in recs), recs[-1]["seconds"]))
+        assert recs[-1]["L2t"][3] < recs[0]["L2t"][3]
+    # the judge keeps the learned run at least as good as the plain one, up to 10 %
+    assert out["learned"][-1]["L2t"][3] <= 1.10 * out["plain"][-1]["L2t"][3]
