@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to scale HBM counters")
     ap.add_argument("--autotune", action="store_true", help="tune rows per segment and graph replay first")
+    ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
+                    help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -139,6 +141,7 @@ def main():
     ctx = dn.Context(W["nx"], W["ny"], fdorder=W["fdorder"], maxrelerror=1, fw=W["fw"], free_surf=1, L=W["L"],
                      fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], dtinv=W["dtinv"],
                      snap_precision=prec)
+    ctx.set_exec(-1 if args.exec == "graph" else 0)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
     ctx.set_shots(W["src_i"][:args.shots], W["src_j"][:args.shots], sig)
@@ -199,6 +202,7 @@ def main():
     }
     if tuned is not None:
         out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
+    out["config"]["exec"] = args.exec
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     ctx.close()

@@ -103,16 +103,7 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
     }
     c->K = p->fdorder / 2;
     Geo &g = c->g;
-    g.nx = p->nx;
-    g.ny = p->ny;
-    g.xoff = kPad;
-    g.yoff = kPad;
-    g.pitch = ((p->nx + 2 * kPad) + 31) / 32 * 32;
-    g.rows = p->ny + 2 * kPad;
-    g.plane = (long long)g.pitch * g.rows;
-    g.fw = p->fw;
-    g.free_surf = p->free_surf ? 1 : 0;
-    g.L = p->L;
+    g = make_geo(p->nx, p->ny, p->fw, p->free_surf, p->L);
 
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         dn_fail(nullptr, "dn_create: hipStreamCreate failed (no HIP device?)");
@@ -511,13 +502,18 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
     int cap = want;
     if (c->batch_req > 0) cap = std::min(cap, c->batch_req);
     const size_t per = per_shot_bytes(c, gradient || c->batch_has_snap);
-    if (c->batch_cap >= cap && c->batch_nt_max == c->nt_max && (!gradient || c->batch_has_snap)) return 0;
+    // reuse the workspace when it holds the request, or as many shots as
+    // fitted into HBM when it was sized
+    if (c->batch_cap > 0 && c->batch_cap >= std::min(cap, c->batch_fit) && c->batch_nt_max == c->nt_max &&
+        (!gradient || c->batch_has_snap))
+        return 0;
     free_batch(c);
     size_t freeb = 0, total = 0;
     HIPCHK(c, hipMemGetInfo(&freeb, &total));
     const size_t budget = (size_t)((double)freeb * 0.85);
     const int fit = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     cap = std::max(1, std::min(std::min(cap, fit), 4096));
+    c->batch_fit = std::min(fit, 4096);
     if (per > budget) return dn_fail(c, "one shot needs %zu MB but only %zu MB of HBM are free", per >> 20, freeb >> 20);
     const Geo &g = c->g;
     const size_t P = (size_t)g.plane, nxny = (size_t)g.nx * g.ny;

@@ -45,7 +45,7 @@ struct dn_ctx {
     int *d_shot_nt = nullptr, *d_src_off = nullptr, *d_src_i = nullptr, *d_src_j = nullptr;
     float *d_sig = nullptr, *d_obs = nullptr;
     // batch workspace
-    int batch_req = 0, rows_req = 0, batch_cap = 0, batch_nt_max = 0;
+    int batch_req = 0, rows_req = 0, batch_cap = 0, batch_nt_max = 0, batch_fit = 0;
     bool batch_has_snap = false;
     float *d_vy[2] = {nullptr, nullptr}, *d_sxy[2] = {nullptr, nullptr}, *d_syz[2] = {nullptr, nullptr};
     float *d_psx[2] = {nullptr, nullptr}, *d_psy[2] = {nullptr, nullptr};

@@ -64,16 +64,7 @@ bool ensure(DropIn &d)
     if (d.stream == nullptr && hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess)
         return fail("no HIP device");
     Geo &g = d.g;
-    g.nx = G.NX;
-    g.ny = G.NY;
-    g.xoff = kPad;
-    g.yoff = kPad;
-    g.pitch = ((G.NX + 2 * kPad) + 31) / 32 * 32;
-    g.rows = G.NY + 2 * kPad;
-    g.plane = (long long)g.pitch * g.rows;
-    g.fw = G.FW;
-    g.free_surf = G.FREE_SURF ? 1 : 0;
-    g.L = G.L;
+    g = make_geo(G.NX, G.NY, G.FW, G.FREE_SURF, G.L);
     if (d.cap_plane == (size_t)g.plane && d.cap_L == G.L && d.cap_fw == G.FW && d.cap_nx == G.NX &&
         d.cap_ny == G.NY)
         return true;

@@ -9,7 +9,13 @@ namespace dn {
 
 constexpr int kWave = 64;      // CDNA wavefront width: one wave per strip
 constexpr int kMaxL = 4;       // relaxation mechanisms supported
-constexpr int kPad = 8;        // halo columns/rows kept around the interior (>= FDORDER/2)
+// Zero frame around the interior of every padded field.  The fused step
+// reads up to 2K-1 rows above/below the interior, 2K-1 columns left of it
+// and 63 right of it (the last strip's wave), K <= 6.
+constexpr int kPadX = 16;      // columns left of i = 1 (64-byte aligned rows)
+constexpr int kPadR = 64;      // columns right of i = nx (at least)
+constexpr int kPadY = 12;      // rows above j = 1 and below j = ny
+
 
 // Padded, x-fastest layout of one 2-D field.  Element (j,i) of the
 // reference's index space (1-based interior) lives at
@@ -24,6 +30,22 @@ struct Geo {
     long long plane;    // pitch * rows
     int fw, free_surf, L;
 };
+
+inline Geo make_geo(int nx, int ny, int fw, int free_surf, int L)
+{
+    Geo g{};
+    g.nx = nx;
+    g.ny = ny;
+    g.xoff = kPadX;
+    g.yoff = kPadY;
+    g.pitch = (nx + kPadX + kPadR + 31) / 32 * 32;
+    g.rows = ny + 2 * kPadY;
+    g.plane = (long long)g.pitch * g.rows;
+    g.fw = fw;
+    g.free_surf = free_surf ? 1 : 0;
+    g.L = L;
+    return g;
+}
 
 __host__ __device__ inline long long gidx(const Geo &g, int j, int i)
 {

@@ -11,16 +11,6 @@
 
 namespace dn {
 
-// Load of a padded field with the zero frame made explicit: cells further
-// than K from the interior are never stored (they are outside the padded
-// array for strips/segments at the edges) and read as 0.
-template <int K>
-__device__ __forceinline__ float ldz(const float *__restrict__ p, const Geo &g, int j, int i)
-{
-    if (j < 1 - K || j > g.ny + K || i < 1 - K || i > g.nx + K) return 0.0f;
-    return p[gidx(g, j, i)];
-}
-
 hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st);
 
 __device__ __forceinline__ float wave_max(float v)
@@ -68,9 +58,10 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 // range of (tile, shot) pairs with the shot index fastest: the shots of one
 // tile share the static model fields through that XCD's L2.
 // ---------------------------------------------------------------------------
-template <int K, bool VISCO, bool ADJ>
+template <int K, int L, bool ADJ>
 __global__ __launch_bounds__(64) void k_step(const StepArgs a)
 {
+    constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
     const int lin = (w & 7) * a.nblocks8 + (w >> 3);
     if (lin >= a.nblocks) return;
@@ -125,25 +116,119 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     __shared__ float lsx[kWave + 2 * K];
     __shared__ float lvy[kWave];
 
+    // The padded planes hold a zero frame of at least 2K-1 rows above and
+    // below the interior, 2K-1 columns left and 64 right (dn_internal.h), so
+    // every row and column this wave touches is inside the plane: loads are
+    // unconditional and cells outside the interior read as 0 (wavefields)
+    // or are not used (model fields).  Lanes 0..K-1 also fetch the K sxy
+    // columns left of the wave, lanes K..2K-2 the K-1 columns right of it.
+    const int hcol = (lane < K) ? (c0 - 2 * K + 1 + lane) : ((lane < 2 * K - 1) ? (c0 + 65 - 2 * K + lane) : c);
+    const int hslot = (lane < K) ? lane : (kWave + lane);
+    const float *__restrict__ rbase = VISCO ? a.r + (long long)s * L * g.plane : nullptr;
+    const float *__restrict__ qbase = VISCO ? a.q + (long long)s * L * g.plane : nullptr;
+
     float zw[2 * K];      // old syz, slot t <-> row jv-K+t (after the head load)
     float vw[2 * K];      // new vy, slot t <-> row jv-2K+1+t (after the push)
     float sq[K + 1];      // old sxy at the lane's column, slot t <-> row jv-K+t
     const int jstart = j0 - K + 1;
     const int jend = j1 + K;
 #pragma unroll
-    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = ldz<K>(syz_in, g, jstart - K + t, c);
+    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = syz_in[gidx(g, jstart - K + t, c)];
 #pragma unroll
     for (int t = 0; t < 2 * K; ++t) vw[t] = 0.0f;
 #pragma unroll
     for (int t = 0; t <= K; ++t) sq[t] = 0.0f;
 
+    // one row of inputs, fetched one iteration ahead: velocity row jv and
+    // the static fields / memory variables of stress row jv - K
+    struct Row {
+        float syz, sc, halo, vy, b, s1, s2;
+        float dp[L > 0 ? L : 1], dd[L > 0 ? L : 1], r[L > 0 ? L : 1], q[L > 0 ? L : 1];
+        int rrow;              // receiver row table entry of row jv (-1: none)
+        float gr, f0;          // velocity row: grad_rho / H_rho sum, stored vyp1 (fp32 snapshots)
+        float wa, gu, fy, fu;  // stress row: Ws / Wr, grad_u / H_mu sum, stored uy, uyx (fp32)
+        uint16_t h0, hy, hu;   // the stored fields of 16-bit snapshots (decoded where used)
+    };
+    // per-shot snapshot slot of this step and its decode scales
+    const long long sbase = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny;
+    float dsc[3] = {1.0f, 1.0f, 1.0f};
+    if (ADJ && corr && a.snap_kind != 0) {
+        const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
+        dsc[0] = scp[0];
+        dsc[1] = scp[1];
+        dsc[2] = scp[2];
+    }
+    auto decode = [&](uint16_t h, float scale) -> float {
+        return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
+    };
+    const long long shot_off = (long long)s * nxny;
+    auto fetch = [&](int jv, Row &o) {
+        const long long gv = gidx(g, jv, c);
+        o.syz = syz_in[gidx(g, jv + K - 1, c)];
+        o.sc = sxy_in[gv];
+        o.halo = sxy_in[gidx(g, jv, hcol)];
+        o.vy = vy_in[gv];
+        o.b = a.bfield[gv];
+        o.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
+        const bool vin = (jv >= j0 && jv <= j1) && own_x;
+        const long long cv = (long long)(jv - 1) * g.nx + (c - 1);
+        if ((ADJ && corr) || (!ADJ && hsum)) {
+            o.gr = vin ? a.grad_rho[shot_off + cv] : 0.0f;
+            if (ADJ && vin) {
+                if (a.snap_kind == 0)
+                    o.f0 = a.snap[sbase + cv];
+                else
+                    o.h0 = a.snap16[sbase + cv];
+            }
+        }
+        const int jsn = jv - K;
+        const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
+        const long long cs = (long long)(jsn - 1) * g.nx + (c - 1);
+        if ((ADJ && corr) || (!ADJ && fstore)) o.wa = sin_ ? a.wacc[shot_off + cs] : 0.0f;
+        if ((ADJ && corr) || (!ADJ && hsum)) o.gu = sin_ ? a.grad_u[shot_off + cs] : 0.0f;
+        if (ADJ && corr && sin_) {
+            if (a.snap_kind == 0) {
+                o.fy = a.snap[sbase + nxny + cs];
+                o.fu = a.snap[sbase + 2 * nxny + cs];
+            } else {
+                o.hy = a.snap16[sbase + nxny + cs];
+                o.hu = a.snap16[sbase + 2 * nxny + cs];
+            }
+        }
+        const long long gs = gidx(g, jv - K, c);
+        if (!VISCO) {
+            o.s1 = a.uip[gs];
+            o.s2 = a.ujp[gs];
+        } else {
+            o.s1 = a.fipjp[gs];
+            o.s2 = a.fcoef[gs];
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                o.dp[l] = a.dip[(long long)l * g.plane + gs];
+                o.dd[l] = a.dcoef[(long long)l * g.plane + gs];
+                o.r[l] = rbase[(long long)l * g.plane + gs];
+                o.q[l] = qbase[(long long)l * g.plane + gs];
+            }
+        }
+    };
+    // does a source of this shot fall into the wave's columns and rows?
+    bool has_src = false;
+    if (!ADJ)
+        for (int l = sb; l < se; ++l) {
+            const int si = a.src_i[l], sj = a.src_j[l];
+            if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src = true;
+        }
+    Row nx_;
+    fetch(jstart, nx_);
+
     for (int jv = jstart; jv <= jend; ++jv) {
-        zw[2 * K - 1] = ldz<K>(syz_in, g, jv + K - 1, c);
-        const float sc = ldz<K>(sxy_in, g, jv, c);
+        const Row cr = nx_;
+        if (jv < jend) fetch(jv + 1, nx_);
+        zw[2 * K - 1] = cr.syz;
+        const float sc = cr.sc;
         lsx[K + lane] = sc;
-        if (lane < K) lsx[lane] = ldz<K>(sxy_in, g, jv, c0 - 2 * K + 1 + lane);
-        if (lane < K - 1) lsx[kWave + K + lane] = ldz<K>(sxy_in, g, jv, c0 + 65 - K + lane);
-        __syncthreads();
+        if (lane < 2 * K - 1) lsx[hslot] = cr.halo;
+        __builtin_amdgcn_wave_barrier();
 
         float vnew = 0.0f;
         if (jv >= 1 && jv <= g.ny && in_x) {
@@ -185,14 +270,13 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 }
             }
             const long long gi = gidx(g, jv, c);
-            const float vold = vy_in[gi];
-            const float bb = a.bfield[gi];
+            const float vold = cr.vy;
+            const float bb = cr.b;
             const float tsum = sxy_x + syy_y;
             const long long ci = (long long)(jv - 1) * g.nx + (c - 1);
             if (!ADJ && hsum && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
-                float *hp = a.grad_rho + (long long)s * nxny + ci;
-                *hp = *hp + (acc * acc);
+                a.grad_rho[shot_off + ci] = cr.gr + (acc * acc);
             }
             if (!ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
@@ -203,23 +287,17 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             }
             if (ADJ && corr && own_v) {
                 // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
-                float f0;
-                const long long sidx = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
-                if (a.snap_kind == 0)
-                    f0 = a.snap[sidx];
-                else {
-                    const float sc0 = a.snap_scale[((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3 + 0];
-                    f0 = (a.snap_kind == 1 ? bf162f(a.snap16[sidx]) : h2f(a.snap16[sidx])) * sc0;
-                }
-                float *gr = a.grad_rho + (long long)s * nxny + ci;
-                *gr = *gr + (vold * f0);
+                const float f0 = (a.snap_kind == 0) ? cr.f0 : decode(cr.h0, dsc[0]);
+                a.grad_rho[shot_off + ci] = cr.gr + (vold * f0);
             }
             vnew = vold + ((bb * a.DT) * tsum) / a.DH;
             if (!ADJ) {
-                for (int l = sb; l < se; ++l)
-                    if (a.src_j[l] == jv && a.src_i[l] == c) vnew = vnew + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
+                if (has_src)
+                    for (int l = sb; l < se; ++l)
+                        if (a.src_j[l] == jv && a.src_i[l] == c)
+                            vnew = vnew + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
             } else {
-                const int off = a.rec_row[jv];
+                const int off = cr.rrow;
                 if (off >= 0) {
                     const int r = a.rec_tab[off + c];
                     if (r >= 0) vnew = vnew + a.resid[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)];
@@ -228,7 +306,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             if (own_v) {
                 vy_out[gi] = vnew;
                 if (!ADJ && a.seis != nullptr) {
-                    const int off = a.rec_row[jv];
+                    const int off = cr.rrow;
                     if (off >= 0) {
                         const int r = a.rec_tab[off + c];
                         if (r >= 0) a.seis[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)] = vnew;
@@ -247,7 +325,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
         const int js = jv - K;  // stress row of this iteration
         if (js >= j0) {
             lvy[lane] = vw[K - 1];
-            __syncthreads();
+            __builtin_amdgcn_wave_barrier();
             if (own_x) {
                 float vyx = a.hc[1] * (lvy[lane + 1] - lvy[lane]);
                 float vyy = a.hc[1] * (vw[K] - vw[K - 1]);
@@ -295,35 +373,33 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 const float syo = zw[0];
                 float sxn, syn;
                 if (!VISCO) {
-                    sxn = sxo + a.uip[gs] * vyx;
-                    syn = syo + a.ujp[gs] * vyy;
+                    sxn = sxo + cr.s1 * vyx;
+                    syn = syo + cr.s2 * vyy;
                 } else {
-                    float rr[kMaxL], qq[kMaxL];
+                    float rr[L > 0 ? L : 1], qq[L > 0 ? L : 1];
                     float sumr = 0.0f, sumq = 0.0f;
-                    float *rp = a.r + (long long)s * g.L * g.plane + gs;
-                    float *qp = a.q + (long long)s * g.L * g.plane + gs;
+                    float *rp = a.r + (long long)s * L * g.plane + gs;
+                    float *qp = a.q + (long long)s * L * g.plane + gs;
 #pragma unroll
-                    for (int l = 0; l < kMaxL; ++l)
-                        if (l < g.L) {
-                            rr[l] = rp[(long long)l * g.plane];
-                            qq[l] = qp[(long long)l * g.plane];
-                            sumr += rr[l];
-                            sumq += qq[l];
-                        }
-                    sxn = sxo + ((a.fipjp[gs] * vyx) + (a.dthalbe * sumr));
-                    syn = syo + ((a.fcoef[gs] * vyy) + (a.dthalbe * sumq));
+                    for (int l = 0; l < L; ++l) {
+                        rr[l] = cr.r[l];
+                        qq[l] = cr.q[l];
+                        sumr += rr[l];
+                        sumq += qq[l];
+                    }
+                    sxn = sxo + ((cr.s1 * vyx) + (a.dthalbe * sumr));
+                    syn = syo + ((cr.s2 * vyy) + (a.dthalbe * sumq));
                     sumr = 0.0f;
                     sumq = 0.0f;
 #pragma unroll
-                    for (int l = 0; l < kMaxL; ++l)
-                        if (l < g.L) {
-                            rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (a.dip[(long long)l * g.plane + gs] * vyx));
-                            qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (a.dcoef[(long long)l * g.plane + gs] * vyy));
-                            rp[(long long)l * g.plane] = rr[l];
-                            qp[(long long)l * g.plane] = qq[l];
-                            sumr += rr[l];
-                            sumq += qq[l];
-                        }
+                    for (int l = 0; l < L; ++l) {
+                        rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cr.dp[l] * vyx));
+                        qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cr.dd[l] * vyy));
+                        rp[(long long)l * g.plane] = rr[l];
+                        qp[(long long)l * g.plane] = qq[l];
+                        sumr += rr[l];
+                        sumq += qq[l];
+                    }
                     sxn = sxn + (a.dthalbe * sumr);
                     syn = syn + (a.dthalbe * sumq);
                 }
@@ -341,38 +417,21 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                         b0[nxny] = vyy;
                         b0[2 * nxny] = vyx;
                     }
-                    float *wp = a.wacc + (long long)s * nxny + ci;
-                    *wp = *wp + vc * vc;       // eprecond.c:11
+                    a.wacc[shot_off + ci] = cr.wa + vc * vc;   // eprecond.c:11
                 }
                 if (!ADJ && hsum) {
                     const float eyx = 0.5f * vyx;     // hess_save.c:13 (exact in float)
-                    float *hp = a.grad_u + (long long)s * nxny + ci;
-                    *hp = *hp + ((vyy * vyy) + (eyx * eyx));
+                    a.grad_u[shot_off + ci] = cr.gu + ((vyy * vyy) + (eyx * eyx));
                 }
                 if (ADJ && corr) {
-                    float fy, fu;
-                    const long long sidx = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
-                    if (a.snap_kind == 0) {
-                        fy = a.snap[sidx + nxny];
-                        fu = a.snap[sidx + 2 * nxny];
-                    } else {
-                        const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
-                        if (a.snap_kind == 1) {
-                            fy = bf162f(a.snap16[sidx + nxny]) * scp[1];
-                            fu = bf162f(a.snap16[sidx + 2 * nxny]) * scp[2];
-                        } else {
-                            fy = h2f(a.snap16[sidx + nxny]) * scp[1];
-                            fu = h2f(a.snap16[sidx + 2 * nxny]) * scp[2];
-                        }
-                    }
-                    float *gu = a.grad_u + (long long)s * nxny + ci;
-                    *gu = *gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
-                    float *wp = a.wacc + (long long)s * nxny + ci;
-                    *wp = *wp + vc * vc;
+                    const float fy = (a.snap_kind == 0) ? cr.fy : decode(cr.hy, dsc[1]);
+                    const float fu = (a.snap_kind == 0) ? cr.fu : decode(cr.hu, dsc[2]);
+                    a.grad_u[shot_off + ci] = cr.gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
+                    a.wacc[shot_off + ci] = cr.wa + vc * vc;
                 }
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int t = 0; t < 2 * K - 1; ++t) zw[t] = zw[t + 1];
     }
@@ -401,26 +460,41 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     }
 }
 
-template <int K>
-static hipError_t launch_step_k(const StepArgs &a, bool visco, bool adjoint, hipStream_t st)
+template <int K, int L>
+static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, hipStream_t st)
 {
     dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave);
-    if (!visco && !adjoint) hipLaunchKernelGGL((k_step<K, false, false>), grid, block, 0, st, a);
-    if (!visco && adjoint) hipLaunchKernelGGL((k_step<K, false, true>), grid, block, 0, st, a);
-    if (visco && !adjoint) hipLaunchKernelGGL((k_step<K, true, false>), grid, block, 0, st, a);
-    if (visco && adjoint) hipLaunchKernelGGL((k_step<K, true, true>), grid, block, 0, st, a);
+    if (adjoint)
+        hipLaunchKernelGGL((k_step<K, L, true>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_step<K, L, false>), grid, block, 0, st, a);
     return hipGetLastError();
+}
+
+template <int K>
+static hipError_t launch_step_k(const StepArgs &a, int L, bool adjoint, hipStream_t st)
+{
+    switch (L) {
+    case 0: return launch_step_kl<K, 0>(a, adjoint, st);
+    case 1: return launch_step_kl<K, 1>(a, adjoint, st);
+    case 2: return launch_step_kl<K, 2>(a, adjoint, st);
+    case 3: return launch_step_kl<K, 3>(a, adjoint, st);
+    case 4: return launch_step_kl<K, 4>(a, adjoint, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipStream_t st)
 {
+    const int L = visco ? a.g.L : 0;
+    if (visco != (a.g.L > 0)) return hipErrorInvalidValue;
     switch (K) {
-    case 1: return launch_step_k<1>(a, visco, adjoint, st);
-    case 2: return launch_step_k<2>(a, visco, adjoint, st);
-    case 3: return launch_step_k<3>(a, visco, adjoint, st);
-    case 4: return launch_step_k<4>(a, visco, adjoint, st);
-    case 5: return launch_step_k<5>(a, visco, adjoint, st);
-    case 6: return launch_step_k<6>(a, visco, adjoint, st);
+    case 1: return launch_step_k<1>(a, L, adjoint, st);
+    case 2: return launch_step_k<2>(a, L, adjoint, st);
+    case 3: return launch_step_k<3>(a, L, adjoint, st);
+    case 4: return launch_step_k<4>(a, L, adjoint, st);
+    case 5: return launch_step_k<5>(a, L, adjoint, st);
+    case 6: return launch_step_k<6>(a, L, adjoint, st);
     default: return hipErrorInvalidValue;
     }
 }
