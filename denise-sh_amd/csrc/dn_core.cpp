@@ -190,6 +190,7 @@ static void free_batch(dn_ctx *c)
     dfree_t(c->d_stage); dfree_t(c->d_stage_max);
     dfree_t(c->d_Ws); dfree_t(c->d_Wr); dfree_t(c->d_gu_shot); dfree_t(c->d_grho_shot); dfree_t(c->d_part);
     dfree_t(c->d_shot_map);
+    c->cur_map.clear();   // the shot map must be uploaded again into the new workspace
     c->batch_cap = 0;
     c->batch_nt_max = 0;
 }
@@ -577,6 +578,7 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a = StepArgs{};
     a.g = g;
     a.nshots = n;
+    a.nshots_all = c->nshots;
     a.ow = kWave + 1 - 2 * c->K;
     a.nstrips = (g.nx + a.ow - 1) / a.ow;
     a.R = choose_rows(c, n, a.nstrips);
@@ -811,7 +813,7 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
 int dn_batch_residual(dn_ctx *c, const std::vector<int> &shots)
 {
     const int n = (int)shots.size();
-    HIPCHK(c, launch_residual(n, c->ntr, c->nt_max, c->d_shot_map, c->d_shot_nt, c->d_seis, c->d_obs, c->d_resid,
+    HIPCHK(c, launch_residual(n, c->nshots, c->ntr, c->nt_max, c->d_shot_map, c->d_shot_nt, c->d_seis, c->d_obs, c->d_resid,
                               c->d_part, c->p.dt, c->stream));
     std::vector<double> part((size_t)n * c->ntr * 2);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -62,6 +62,7 @@ struct PmlVec {
 struct StepArgs {
     Geo g;
     int nshots;                    // shots in this launch
+    int nshots_all;                // shots of the context (bound of shot_map entries)
     int ow, nstrips, R, nsegs;     // strip width (owned columns), strips, rows per segment, segments
     int nblocks, nblocks8;         // real blocks and blocks per XCD slice
     float DT, DH, dhi, dthalbe;
@@ -135,7 +136,7 @@ hipError_t launch_hessian_finish(int nshots, int nx, int ny, const Geo &g, const
                                  float *hr, int first, int last, hipStream_t st);
 
 // per-shot residual (calc_res / calc_energy): one thread per trace
-hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
                            const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st);
 
 // energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)

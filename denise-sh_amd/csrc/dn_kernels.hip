@@ -71,6 +71,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const int seg = tile / a.nstrips;
     const Geo &g = a.g;
     const int S = a.shot_map[s];
+    if (S < 0 || S >= a.nshots_all) return;   // a stale or foreign shot map: touch nothing
     const int NTs = a.shot_nt[S];
     const int nt = a.nt_base ? (*a.nt_base + a.nt) : a.nt;
     if (nt > NTs) return;  // shot finished (uneven record lengths)
@@ -614,13 +615,18 @@ hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st)
 // calc_energy.c:53-61).  One thread per (shot, trace); per-trace FP64 partial
 // sums are written in place and reduced on the host in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+__global__ void k_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
                            const float *syn, const float *obs, float *resid, double *part, float DT)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nshots * ntr) return;
     const int s = t / ntr, r = t % ntr;
     const int S = shot_map[s];
+    if (S < 0 || S >= nshots_all) {
+        part[2 * (long long)t] = 0.0;
+        part[2 * (long long)t + 1] = 0.0;
+        return;
+    }
     const int ns = shot_nt[S];
     const float *sy = syn + (long long)t * nt_max;
     const float *ob = obs + ((long long)S * ntr + r) * nt_max;
@@ -640,12 +646,12 @@ __global__ void k_residual(int nshots, int ntr, int nt_max, const int *shot_map,
     part[2 * (long long)t + 1] = en;
 }
 
-hipError_t launch_residual(int nshots, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
+hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
                            const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st)
 {
     const int n = nshots * ntr;
-    hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, ntr, nt_max, shot_map, shot_nt, syn,
-                       obs, resid, part, DT);
+    hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, nshots_all, ntr, nt_max, shot_map,
+                       shot_nt, syn, obs, resid, part, DT);
     return hipGetLastError();
 }
 

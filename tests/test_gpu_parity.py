@@ -306,3 +306,23 @@ def test_gradient_without_eprecond_and_hessian_probe():
     # the model is restored: the gradient is unchanged
     ctx.gradient()
     assert np.array_equal(ctx.get_gradient()[0], g[0])
+
+
+def test_gradient_after_forward_on_same_shots():
+    """Regression: a forward pass followed by a gradient over the same shots
+    (the workspace is re-sized for snapshots in between) must give the
+    gradient of a fresh context."""
+    say("case gradient after forward, same shots")
+    nx, ny, nt = 96, 40, 600
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, dtinv=2)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 3)
+    ctx.forward()
+    ctx.set_observed(obs)
+    ctx.gradient()
+    g1 = ctx.get_gradient()
+    fresh, _, _ = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, dtinv=2)
+    fresh.set_observed(obs)
+    fresh.gradient()
+    g2 = fresh.get_gradient()
+    for a, b in zip(g1, g2):
+        assert np.array_equal(a, b)
