@@ -700,8 +700,9 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
         if (line_search(c, energy, L2t, epst1, &step1, &step3, &eps_scale, &nfwd)) return -1;
     } else {
         // learned direction first; the plain (eprecond) direction judges it
-        // every judge_every iterations and replaces it when its line search
-        // fails or its misfit drop falls below half the last plain drop
+        // on the first iteration of a stage and every judge_every-th one
+        // (the better misfit drop of the two is taken), and replaces it
+        // whenever its line search finds no decrease
         auto drop_of = [](const float *L, int st1) {
             return st1 ? (double)(L[1] - std::min(L[2], L[3])) / (double)L[1] : 0.0;
         };
@@ -716,8 +717,8 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
         float epsl = 0.0f;
         if (line_search(c, energy, L2l, epl, &s1l, &s3l, &epsl, &nfwd)) return -1;
         const double dl = drop_of(L2l, s1l);
-        const bool judge = (f->last_plain_drop < 0.0) || (c->judge_every > 0 && (f->iter - 1) % c->judge_every == 0);
-        const bool fail = (s1l == 0) || (f->last_plain_drop > 0.0 && dl < 0.5 * f->last_plain_drop);
+        const bool judge = (f->iter == 1) || (c->judge_every > 0 && (f->iter - 1) % c->judge_every == 0);
+        const bool fail = (s1l == 0);
         bool take_learned = true;
         if (judge || fail) {
             HIPCHK(c, hipMemcpyAsync(f->gu, f->gu_p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));

@@ -336,8 +336,9 @@ int dn_fwi_iteration(dn_ctx *ctx, dn_fwi_record *rec);
  * prediction H of the vs diagonal Hessian gives the initial L-BFGS Hessian
  * gamma * diag(We / H) for vs (We the eprecond1.c energy weight, normalised
  * to mean 1); the plain direction (the reference's) judges it on the first
- * iteration and every judge_every-th (0: never again), and replaces it when
- * its line search fails or its misfit drop is below half the plain one.
+ * iteration of a stage and every judge_every-th (0: only the first), the
+ * larger misfit drop of the two being taken, and replaces it whenever its
+ * line search finds no decrease.
  * The network stays owned by the caller.  Not in the reference. */
 int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
 /* The network's input maps from the last gradient and the current model,
