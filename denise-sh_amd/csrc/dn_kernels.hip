@@ -20,6 +20,12 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+// 32-bit offset of (j, i) inside one padded plane (plane < 2^31 elements)
+__device__ __forceinline__ int gi32(const Geo &g, int j, int i)
+{
+    return (j - 1 + g.yoff) * g.pitch + (i - 1 + g.xoff);
+}
+
 __device__ __forceinline__ uint16_t f2bf16(float x)
 {
     // round-to-nearest-even truncation of the fp32 pattern (NaN not expected)
@@ -84,7 +90,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const int j0 = 1 + seg * a.R;
     const int j1 = min(j0 + a.R - 1, g.ny);
     const long long so = (long long)s * g.plane;
-    const long long nxny = (long long)g.nx * g.ny;
+    const int nxny = g.nx * g.ny;                 // per-lane offsets are 32-bit; bases are per shot
+    const int plane = (int)g.plane;
     const int fw = g.fw, fw2 = 2 * g.fw;
 
     bool fstore = false, corr = false, hsum = false;
@@ -125,8 +132,18 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     // columns left of the wave, lanes K..2K-2 the K-1 columns right of it.
     const int hcol = (lane < K) ? (c0 - 2 * K + 1 + lane) : ((lane < 2 * K - 1) ? (c0 + 65 - 2 * K + lane) : c);
     const int hslot = (lane < K) ? lane : (kWave + lane);
-    const float *__restrict__ rbase = VISCO ? a.r + (long long)s * L * g.plane : nullptr;
-    const float *__restrict__ qbase = VISCO ? a.q + (long long)s * L * g.plane : nullptr;
+    float *__restrict__ rbase = VISCO ? a.r + (long long)s * L * g.plane : nullptr;
+    float *__restrict__ qbase = VISCO ? a.q + (long long)s * L * g.plane : nullptr;
+    const long long shot_off = (long long)s * nxny;
+    float *__restrict__ grb = a.grad_rho ? a.grad_rho + shot_off : nullptr;
+    float *__restrict__ gub = a.grad_u ? a.grad_u + shot_off : nullptr;
+    float *__restrict__ wab = a.wacc ? a.wacc + shot_off : nullptr;
+    const float *__restrict__ psxi = a.psx_in + (long long)s * g.ny * fw2;
+    float *__restrict__ psxo = a.psx_out + (long long)s * g.ny * fw2;
+    const float *__restrict__ psyi = a.psy_in + (long long)s * fw2 * g.nx;
+    float *__restrict__ psyo = a.psy_out + (long long)s * fw2 * g.nx;
+    float *__restrict__ pvxb = a.pvx + (long long)s * g.ny * fw2;
+    float *__restrict__ pvyb = a.pvy + (long long)s * fw2 * g.nx;
 
     float zw[2 * K];      // old syz, slot t <-> row jv-K+t (after the head load)
     float vw[2 * K];      // new vy, slot t <-> row jv-2K+1+t (after the push)
@@ -134,7 +151,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const int jstart = j0 - K + 1;
     const int jend = j1 + K;
 #pragma unroll
-    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = syz_in[gidx(g, jstart - K + t, c)];
+    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = syz_in[gi32(g, jstart - K + t, c)];
 #pragma unroll
     for (int t = 0; t < 2 * K; ++t) vw[t] = 0.0f;
 #pragma unroll
@@ -152,6 +169,10 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     };
     // per-shot snapshot slot of this step and its decode scales
     const long long sbase = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny;
+    const float *__restrict__ snb = a.snap ? a.snap + sbase : nullptr;
+    float *__restrict__ snw = a.snap ? a.snap + sbase : nullptr;
+    const uint16_t *__restrict__ snb16 = a.snap16 ? a.snap16 + sbase : nullptr;
+    float *__restrict__ stg = a.stage ? a.stage + (long long)s * 3 * nxny : nullptr;
     float dsc[3] = {1.0f, 1.0f, 1.0f};
     if (ADJ && corr && a.snap_kind != 0) {
         const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
@@ -162,41 +183,40 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     auto decode = [&](uint16_t h, float scale) -> float {
         return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
     };
-    const long long shot_off = (long long)s * nxny;
     auto fetch = [&](int jv, Row &o) {
-        const long long gv = gidx(g, jv, c);
-        o.syz = syz_in[gidx(g, jv + K - 1, c)];
+        const int gv = gi32(g, jv, c);
+        o.syz = syz_in[gi32(g, jv + K - 1, c)];
         o.sc = sxy_in[gv];
-        o.halo = sxy_in[gidx(g, jv, hcol)];
+        o.halo = sxy_in[gi32(g, jv, hcol)];
         o.vy = vy_in[gv];
         o.b = a.bfield[gv];
         o.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
         const bool vin = (jv >= j0 && jv <= j1) && own_x;
-        const long long cv = (long long)(jv - 1) * g.nx + (c - 1);
+        const int cv = (jv - 1) * g.nx + (c - 1);
         if ((ADJ && corr) || (!ADJ && hsum)) {
-            o.gr = vin ? a.grad_rho[shot_off + cv] : 0.0f;
+            o.gr = vin ? grb[cv] : 0.0f;
             if (ADJ && vin) {
                 if (a.snap_kind == 0)
-                    o.f0 = a.snap[sbase + cv];
+                    o.f0 = snb[cv];
                 else
-                    o.h0 = a.snap16[sbase + cv];
+                    o.h0 = snb16[cv];
             }
         }
         const int jsn = jv - K;
         const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
-        const long long cs = (long long)(jsn - 1) * g.nx + (c - 1);
-        if ((ADJ && corr) || (!ADJ && fstore)) o.wa = sin_ ? a.wacc[shot_off + cs] : 0.0f;
-        if ((ADJ && corr) || (!ADJ && hsum)) o.gu = sin_ ? a.grad_u[shot_off + cs] : 0.0f;
+        const int cs = (jsn - 1) * g.nx + (c - 1);
+        if ((ADJ && corr) || (!ADJ && fstore)) o.wa = sin_ ? wab[cs] : 0.0f;
+        if ((ADJ && corr) || (!ADJ && hsum)) o.gu = sin_ ? gub[cs] : 0.0f;
         if (ADJ && corr && sin_) {
             if (a.snap_kind == 0) {
-                o.fy = a.snap[sbase + nxny + cs];
-                o.fu = a.snap[sbase + 2 * nxny + cs];
+                o.fy = snb[nxny + cs];
+                o.fu = snb[2 * nxny + cs];
             } else {
-                o.hy = a.snap16[sbase + nxny + cs];
-                o.hu = a.snap16[sbase + 2 * nxny + cs];
+                o.hy = snb16[nxny + cs];
+                o.hu = snb16[2 * nxny + cs];
             }
         }
-        const long long gs = gidx(g, jv - K, c);
+        const int gs = gi32(g, jv - K, c);
         if (!VISCO) {
             o.s1 = a.uip[gs];
             o.s2 = a.ujp[gs];
@@ -205,10 +225,10 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             o.s2 = a.fcoef[gs];
 #pragma unroll
             for (int l = 0; l < L; ++l) {
-                o.dp[l] = a.dip[(long long)l * g.plane + gs];
-                o.dd[l] = a.dcoef[(long long)l * g.plane + gs];
-                o.r[l] = rbase[(long long)l * g.plane + gs];
-                o.q[l] = qbase[(long long)l * g.plane + gs];
+                o.dp[l] = a.dip[l * plane + gs];
+                o.dd[l] = a.dcoef[l * plane + gs];
+                o.r[l] = rbase[l * plane + gs];
+                o.q[l] = qbase[l * plane + gs];
             }
         }
     };
@@ -244,52 +264,52 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             }
             if (fw > 0) {
                 if (c <= fw) {
-                    const long long pi = ((long long)s * g.ny + (jv - 1)) * fw2 + (c - 1);
-                    const float p = a.pml.bx[c] * a.psx_in[pi] + a.pml.ax[c] * sxy_x;
-                    if (own_v) a.psx_out[pi] = p;
+                    const int pi = (jv - 1) * fw2 + (c - 1);
+                    const float p = a.pml.bx[c] * psxi[pi] + a.pml.ax[c] * sxy_x;
+                    if (own_v) psxo[pi] = p;
                     sxy_x = sxy_x / a.pml.Kx[c] + p;
                 }
                 if (c >= g.nx - fw + 1) {
                     const int h1 = c - g.nx + fw2;
-                    const long long pi = ((long long)s * g.ny + (jv - 1)) * fw2 + (h1 - 1);
-                    const float p = a.pml.bx[h1] * a.psx_in[pi] + a.pml.ax[h1] * sxy_x;
-                    if (own_v) a.psx_out[pi] = p;
+                    const int pi = (jv - 1) * fw2 + (h1 - 1);
+                    const float p = a.pml.bx[h1] * psxi[pi] + a.pml.ax[h1] * sxy_x;
+                    if (own_v) psxo[pi] = p;
                     sxy_x = sxy_x / a.pml.Kx[h1] + p;
                 }
                 if (!g.free_surf && jv <= fw) {
-                    const long long pi = ((long long)s * fw2 + (jv - 1)) * g.nx + (c - 1);
-                    const float p = a.pml.by[jv] * a.psy_in[pi] + a.pml.ay[jv] * syy_y;
-                    if (own_v) a.psy_out[pi] = p;
+                    const int pi = (jv - 1) * g.nx + (c - 1);
+                    const float p = a.pml.by[jv] * psyi[pi] + a.pml.ay[jv] * syy_y;
+                    if (own_v) psyo[pi] = p;
                     syy_y = syy_y / a.pml.Ky[jv] + p;
                 }
                 if (jv >= g.ny - fw + 1) {
                     const int h1 = jv - g.ny + fw2;
-                    const long long pi = ((long long)s * fw2 + (h1 - 1)) * g.nx + (c - 1);
-                    const float p = a.pml.by[h1] * a.psy_in[pi] + a.pml.ay[h1] * syy_y;
-                    if (own_v) a.psy_out[pi] = p;
+                    const int pi = (h1 - 1) * g.nx + (c - 1);
+                    const float p = a.pml.by[h1] * psyi[pi] + a.pml.ay[h1] * syy_y;
+                    if (own_v) psyo[pi] = p;
                     syy_y = syy_y / a.pml.Ky[h1] + p;
                 }
             }
-            const long long gi = gidx(g, jv, c);
+            const int gi = gi32(g, jv, c);
             const float vold = cr.vy;
             const float bb = cr.b;
             const float tsum = sxy_x + syy_y;
-            const long long ci = (long long)(jv - 1) * g.nx + (c - 1);
+            const int ci = (jv - 1) * g.nx + (c - 1);
             if (!ADJ && hsum && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
-                a.grad_rho[shot_off + ci] = cr.gr + (acc * acc);
+                grb[ci] = cr.gr + (acc * acc);
             }
             if (!ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
                 if (a.store_mode == 1)
-                    a.snap[(long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci] = acc;
+                    snw[ci] = acc;
                 else
-                    a.stage[(long long)s * 3 * nxny + ci] = acc;
+                    stg[ci] = acc;
             }
             if (ADJ && corr && own_v) {
                 // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
                 const float f0 = (a.snap_kind == 0) ? cr.f0 : decode(cr.h0, dsc[0]);
-                a.grad_rho[shot_off + ci] = cr.gr + (vold * f0);
+                grb[ci] = cr.gr + (vold * f0);
             }
             vnew = vold + ((bb * a.DT) * tsum) / a.DH;
             if (!ADJ) {
@@ -339,37 +359,37 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 vyy = vyy * a.dhi;
                 if (fw > 0) {
                     if (c <= fw) {
-                        const long long pi = ((long long)s * g.ny + (js - 1)) * fw2 + (c - 1);
-                        const float p = a.pml.bxh[c] * a.pvx[pi] + a.pml.axh[c] * vyx;
-                        a.pvx[pi] = p;
+                        const int pi = (js - 1) * fw2 + (c - 1);
+                        const float p = a.pml.bxh[c] * pvxb[pi] + a.pml.axh[c] * vyx;
+                        pvxb[pi] = p;
                         vyx = vyx / a.pml.Kxh[c] + p;
                     }
                     if (c >= g.nx - fw + 1) {
                         const int h1 = c - g.nx + fw2;
-                        const long long pi = ((long long)s * g.ny + (js - 1)) * fw2 + (h1 - 1);
-                        const float p = a.pml.bxh[h1] * a.pvx[pi] + a.pml.axh[h1] * vyx;
-                        a.pvx[pi] = p;
+                        const int pi = (js - 1) * fw2 + (h1 - 1);
+                        const float p = a.pml.bxh[h1] * pvxb[pi] + a.pml.axh[h1] * vyx;
+                        pvxb[pi] = p;
                         vyx = vyx / a.pml.Kxh[h1] + p;
                     }
                     if (!g.free_surf && js <= fw) {
                         // Q2: the viscoelastic kernel uses the full-grid y profile on top
-                        const long long pi = ((long long)s * fw2 + (js - 1)) * g.nx + (c - 1);
+                        const int pi = (js - 1) * g.nx + (c - 1);
                         const float bY = VISCO ? a.pml.by[js] : a.pml.byh[js];
                         const float aY = VISCO ? a.pml.ay[js] : a.pml.ayh[js];
                         const float KY = VISCO ? a.pml.Ky[js] : a.pml.Kyh[js];
-                        const float p = bY * a.pvy[pi] + aY * vyy;
-                        a.pvy[pi] = p;
+                        const float p = bY * pvyb[pi] + aY * vyy;
+                        pvyb[pi] = p;
                         vyy = vyy / KY + p;
                     }
                     if (js >= g.ny - fw + 1) {
                         const int h1 = js - g.ny + fw2;
-                        const long long pi = ((long long)s * fw2 + (h1 - 1)) * g.nx + (c - 1);
-                        const float p = a.pml.byh[h1] * a.pvy[pi] + a.pml.ayh[h1] * vyy;
-                        a.pvy[pi] = p;
+                        const int pi = (h1 - 1) * g.nx + (c - 1);
+                        const float p = a.pml.byh[h1] * pvyb[pi] + a.pml.ayh[h1] * vyy;
+                        pvyb[pi] = p;
                         vyy = vyy / a.pml.Kyh[h1] + p;
                     }
                 }
-                const long long gs = gidx(g, js, c);
+                const int gs = gi32(g, js, c);
                 const float sxo = sq[0];
                 const float syo = zw[0];
                 float sxn, syn;
@@ -379,8 +399,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 } else {
                     float rr[L > 0 ? L : 1], qq[L > 0 ? L : 1];
                     float sumr = 0.0f, sumq = 0.0f;
-                    float *rp = a.r + (long long)s * L * g.plane + gs;
-                    float *qp = a.q + (long long)s * L * g.plane + gs;
+                    float *rp = rbase + gs;
+                    float *qp = qbase + gs;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
                         rr[l] = cr.r[l];
@@ -396,8 +416,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                     for (int l = 0; l < L; ++l) {
                         rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cr.dp[l] * vyx));
                         qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cr.dd[l] * vyy));
-                        rp[(long long)l * g.plane] = rr[l];
-                        qp[(long long)l * g.plane] = qq[l];
+                        rp[l * plane] = rr[l];
+                        qp[l * plane] = qq[l];
                         sumr += rr[l];
                         sumq += qq[l];
                     }
@@ -406,29 +426,29 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 }
                 sxy_out[gs] = sxn;
                 syz_out[gs] = syn;
-                const long long ci = (long long)(js - 1) * g.nx + (c - 1);
+                const int ci = (js - 1) * g.nx + (c - 1);
                 const float vc = vw[K - 1];
                 if (!ADJ && fstore) {
                     if (a.store_mode == 1) {
-                        float *b0 = a.snap + (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny + ci;
+                        float *b0 = snw + ci;
                         b0[nxny] = vyy;        // uy  (forward_prop_y)
                         b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
                     } else {
-                        float *b0 = a.stage + (long long)s * 3 * nxny + ci;
+                        float *b0 = stg + ci;
                         b0[nxny] = vyy;
                         b0[2 * nxny] = vyx;
                     }
-                    a.wacc[shot_off + ci] = cr.wa + vc * vc;   // eprecond.c:11
+                    wab[ci] = cr.wa + vc * vc;   // eprecond.c:11
                 }
                 if (!ADJ && hsum) {
                     const float eyx = 0.5f * vyx;     // hess_save.c:13 (exact in float)
-                    a.grad_u[shot_off + ci] = cr.gu + ((vyy * vyy) + (eyx * eyx));
+                    gub[ci] = cr.gu + ((vyy * vyy) + (eyx * eyx));
                 }
                 if (ADJ && corr) {
                     const float fy = (a.snap_kind == 0) ? cr.fy : decode(cr.hy, dsc[1]);
                     const float fu = (a.snap_kind == 0) ? cr.fu : decode(cr.hu, dsc[2]);
-                    a.grad_u[shot_off + ci] = cr.gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
-                    a.wacc[shot_off + ci] = cr.wa + vc * vc;
+                    gub[ci] = cr.gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
+                    wab[ci] = cr.wa + vc * vc;
                 }
             }
         }
