@@ -99,6 +99,22 @@ def cpu_baseline(W, seconds):
                        % (len(W["src_i"]), nt_sample, W["nt"], sec))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_visco_gradient_1200x300_hbm_traffic.md")
+
+
+def measured_traffic(cells_per_launch):
+    """HBM bytes per launch of the adjoint step kernel on this workload, from
+    the FETCH_SIZE / WRITE_SIZE profile (scaled by the copy-probe factors)
+    kept in profiles/: bytes per cell-update x cell-updates per launch."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_cell_update_adjoint:"):
+                return round(float(line.split(":")[1]) * cells_per_launch)
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """Plain device-to-device copy of known size (torch's copy kernel): the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
@@ -177,6 +193,7 @@ def main():
     if prec != 0:
         bytes_per_cell -= (6.0 / W["dtinv"])
     achieved = units_per_launch * bytes_per_cell / (per_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(units_per_launch) if (dom == 1 and prec == 0 and args.nt >= 1000) else None
     out = {
         "metric": "Gcell-updates/s fwd+adj",
         "value": round(value, 3),
@@ -194,7 +211,7 @@ def main():
                                % (args.shots, args.nt, W["dtinv"], args.snap),
                    "nx": W["nx"], "ny": W["ny"], "shots": args.shots, "nt": args.nt, "fdorder": 8, "L": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step<4,visco,%s>" % ("adjoint" if dom else "forward"),
                      "ms_per_launch": round(per_launch_ms, 4), "bytes_per_cell_update": bytes_per_cell,
                      "cell_updates_per_launch": units_per_launch},
