@@ -646,7 +646,7 @@ int dn_zero_state(dn_ctx *c, int n)
 // the device counter nt_base inside a captured chunk).  cur selects the
 // ping-pong half holding the current fields and is advanced per step.
 static int enqueue_steps(dn_ctx *c, StepArgs &a, int n, bool adjoint, bool pack_mode, int nsnap, int nt0, int count,
-                         const int *nt_base, int &cur)
+                         const int *nt_base, int &cur, const std::vector<char> &corr_res, bool fwd_extra)
 {
     const int kind = c->p.snap_precision;
     for (int k = 0; k < count; ++k) {
@@ -667,7 +667,11 @@ static int enqueue_steps(dn_ctx *c, StepArgs &a, int n, bool adjoint, bool pack_
         // pattern depends on the offset only
         const bool pack = pack_mode && ((nt - 1) % c->p.dtinv) == 0;
         if (pack) HIPCHK(c, launch_fill_u32(c->d_stage_max, (size_t)n * 3, 0u, c->stream));
-        HIPCHK(c, launch_step(a, c->K, c->p.L > 0, adjoint, c->stream));
+        // adjoint: shot s correlates at step nt when nt = NT_s (mod dtinv);
+        // inside a chunk the counter is a multiple of dtinv
+        const int dt = c->p.dtinv;
+        const bool extra = adjoint ? corr_res[((nt % dt) + dt) % dt] != 0 : (fwd_extra && ((nt - 1) % dt) == 0);
+        HIPCHK(c, launch_step(a, c->K, c->p.L > 0, adjoint, extra, c->stream));
         if (pack)
             HIPCHK(c, launch_snap_pack(n, c->g.nx, c->g.ny, c->d_stage, c->d_stage_max, c->d_snap16, c->d_snap_scale,
                                        a.snap_shot_stride, nsnap, nt_base, nt, c->p.dtinv, kind, c->stream));
@@ -688,20 +692,20 @@ static int chunk_steps(const dn_ctx *c)
 // Captured chunk for this batch shape (built on first use): `steps` time
 // steps reading the device counter, followed by the counter advance.
 static int get_graph(dn_ctx *c, StepArgs a, int n, bool adjoint, bool store, bool record, bool pack_mode,
-                     int nsnap, int steps, hipGraphExec_t *out)
+                     int nsnap, int steps, const std::vector<char> &corr_res, bool fwd_extra, hipGraphExec_t *out)
 {
     const int kind = c->p.snap_precision;
     const bool hess = a.hess != 0;
     for (auto &gr : c->graphs)
         if (gr.n == n && gr.R == a.R && gr.steps == steps && gr.kind == kind && gr.adjoint == adjoint &&
-            gr.store == store && gr.record == record && gr.hess == hess) {
+            gr.store == store && gr.record == record && gr.hess == hess && gr.corr_res == corr_res) {
             *out = gr.exec;
             return 0;
         }
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int cur = 0;
-    int rc = enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, 1, steps, c->d_ntbase, cur);
+    int rc = enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, 1, steps, c->d_ntbase, cur, corr_res, fwd_extra);
     if (rc == 0 && launch_advance(c->d_ntbase, steps, c->stream) != hipSuccess)
         rc = dn_fail(c, "graph capture: counter advance failed");
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
@@ -714,7 +718,7 @@ static int get_graph(dn_ctx *c, StepArgs a, int n, bool adjoint, bool store, boo
     const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return dn_fail(c, "graph instantiate failed: %s", hipGetErrorString(ei));
-    c->graphs.push_back({n, a.R, steps, kind, adjoint, store, record, hess, exec});
+    c->graphs.push_back({n, a.R, steps, kind, adjoint, store, record, hess, corr_res, exec});
     *out = exec;
     return 0;
 }
@@ -770,12 +774,15 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     a.stage = c->d_stage;
     a.stage_max = c->d_stage_max;
     const bool pack_mode = !adjoint && store && kind != 0;
+    std::vector<char> corr_res(c->p.dtinv, 0);
+    for (int sh : shots) corr_res[c->shot_nt[sh] % c->p.dtinv] = 1;
+    const bool fwd_extra = store || c->hess_mode;
     const int CH = chunk_steps(c);
     const int nchunks = (CH > 0 && ntmax >= 2 * CH) ? ntmax / CH : 0;
     hipGraphExec_t exec = nullptr;
     if (nchunks > 0) {
         if (!c->d_ntbase) HIPCHK(c, hipMalloc((void **)&c->d_ntbase, 256));
-        if (get_graph(c, a, n, adjoint, store, record, pack_mode, nsnap, CH, &exec)) return -1;
+        if (get_graph(c, a, n, adjoint, store, record, pack_mode, nsnap, CH, corr_res, fwd_extra, &exec)) return -1;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int cur = 0;
@@ -785,7 +792,8 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
         c->graph_replays += nchunks;
     }
     const int done = nchunks * CH;   // even: the fields are back in half 0
-    if (enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, done + 1, ntmax - done, nullptr, cur)) return -1;
+    if (enqueue_steps(c, a, n, adjoint, pack_mode, nsnap, done + 1, ntmax - done, nullptr, cur, corr_res, fwd_extra))
+        return -1;
     c->launches += ntmax;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));

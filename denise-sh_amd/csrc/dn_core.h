@@ -69,6 +69,7 @@ struct dn_ctx {
     struct Graph {
         int n, R, steps, kind;
         bool adjoint, store, record, hess;
+        std::vector<char> corr_res;   // record-length residues that select the correlating launches
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;

@@ -115,7 +115,9 @@ struct StepArgs {
 };
 
 // launchers (dn_kernels.hip)
-hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipStream_t st);
+// extra: the step may store snapshots / sum the pseudo-Hessian (forward) or
+// correlate with the stored snapshots (adjoint) for some shot of the launch
+hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, bool extra, hipStream_t st);
 
 struct ModelPrepArgs {
     Geo g;

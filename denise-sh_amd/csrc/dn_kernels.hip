@@ -21,10 +21,18 @@ __device__ __forceinline__ float wave_max(float v)
 }
 
 // 32-bit offset of (j, i) inside one padded plane (plane < 2^31 elements)
-__device__ __forceinline__ int gi32(const Geo &g, int j, int i)
+__device__ __forceinline__ unsigned gi32(const Geo &g, int j, int i)
 {
-    return (j - 1 + g.yoff) * g.pitch + (i - 1 + g.xoff);
+    return (unsigned)((j - 1 + g.yoff) * g.pitch + (i - 1 + g.xoff));
 }
+
+// loads and stores at a 32-bit byte offset from a uniform base: the
+// addresses become base (scalar registers) + offset (one vector register)
+__device__ __forceinline__ float ldf(const float *base, unsigned boff)
+{
+    return *(const float *)((const char *)base + boff);
+}
+__device__ __forceinline__ void stf(float *base, unsigned boff, float v) { *(float *)((char *)base + boff) = v; }
 
 __device__ __forceinline__ uint16_t f2bf16(float x)
 {
@@ -64,9 +72,12 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 // range of (tile, shot) pairs with the shot index fastest: the shots of one
 // tile share the static model fields through that XCD's L2.
 // ---------------------------------------------------------------------------
-template <int K, int L, bool ADJ>
+template <int K, int L, bool ADJ, bool EX>
 __global__ __launch_bounds__(64) void k_step(const StepArgs a)
 {
+    // EX: this launch may store snapshots / sum the pseudo-Hessian (forward)
+    // or correlate (adjoint); the launches without (most steps) carry none
+    // of that code and its pointers
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
     const int lin = (w & 7) * a.nblocks8 + (w >> 3);
@@ -96,7 +107,8 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
 
     bool fstore = false, corr = false, hsum = false;
     int slot = 0;
-    if (!ADJ) {
+    if (!EX) {
+    } else if (!ADJ) {
         if (a.store_mode != 0 && ((nt - 1) % a.dtinv) == 0) {
             fstore = true;
             slot = (nt - 1) / a.dtinv;
@@ -138,6 +150,11 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     float *__restrict__ grb = a.grad_rho ? a.grad_rho + shot_off : nullptr;
     float *__restrict__ gub = a.grad_u ? a.grad_u + shot_off : nullptr;
     float *__restrict__ wab = a.wacc ? a.wacc + shot_off : nullptr;
+    // the twelve CPML profile vectors are packed with stride 2*FW+1 in the
+    // order K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y,
+    // K_y_half, a_y_half, b_y_half (one base keeps the scalar registers free)
+    const float *__restrict__ pmlb = a.pml.Kx;
+    const int pn = fw2 + 1;
     const float *__restrict__ psxi = a.psx_in + (long long)s * g.ny * fw2;
     float *__restrict__ psxo = a.psx_out + (long long)s * g.ny * fw2;
     const float *__restrict__ psyi = a.psy_in + (long long)s * fw2 * g.nx;
@@ -184,15 +201,15 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
         return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
     };
     auto fetch = [&](int jv, Row &o) {
-        const int gv = gi32(g, jv, c);
-        o.syz = syz_in[gi32(g, jv + K - 1, c)];
-        o.sc = sxy_in[gv];
-        o.halo = sxy_in[gi32(g, jv, hcol)];
-        o.vy = vy_in[gv];
-        o.b = a.bfield[gv];
+        const unsigned gv = gi32(g, jv, c) * 4u;
+        o.syz = ldf(syz_in, gi32(g, jv + K - 1, c) * 4u);
+        o.sc = ldf(sxy_in, gv);
+        o.halo = ldf(sxy_in, gi32(g, jv, hcol) * 4u);
+        o.vy = ldf(vy_in, gv);
+        o.b = ldf(a.bfield, gv);
         o.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
         const bool vin = (jv >= j0 && jv <= j1) && own_x;
-        const int cv = (jv - 1) * g.nx + (c - 1);
+        const unsigned cv = (unsigned)((jv - 1) * g.nx + (c - 1));
         if ((ADJ && corr) || (!ADJ && hsum)) {
             o.gr = vin ? grb[cv] : 0.0f;
             if (ADJ && vin) {
@@ -204,7 +221,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
         }
         const int jsn = jv - K;
         const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
-        const int cs = (jsn - 1) * g.nx + (c - 1);
+        const unsigned cs = (unsigned)((jsn - 1) * g.nx + (c - 1));
         if ((ADJ && corr) || (!ADJ && fstore)) o.wa = sin_ ? wab[cs] : 0.0f;
         if ((ADJ && corr) || (!ADJ && hsum)) o.gu = sin_ ? gub[cs] : 0.0f;
         if (ADJ && corr && sin_) {
@@ -216,19 +233,20 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 o.hu = snb16[2 * nxny + cs];
             }
         }
-        const int gs = gi32(g, jv - K, c);
+        const unsigned gs = gi32(g, jv - K, c) * 4u;
         if (!VISCO) {
-            o.s1 = a.uip[gs];
-            o.s2 = a.ujp[gs];
+            o.s1 = ldf(a.uip, gs);
+            o.s2 = ldf(a.ujp, gs);
         } else {
-            o.s1 = a.fipjp[gs];
-            o.s2 = a.fcoef[gs];
+            o.s1 = ldf(a.fipjp, gs);
+            o.s2 = ldf(a.fcoef, gs);
 #pragma unroll
             for (int l = 0; l < L; ++l) {
-                o.dp[l] = a.dip[l * plane + gs];
-                o.dd[l] = a.dcoef[l * plane + gs];
-                o.r[l] = rbase[l * plane + gs];
-                o.q[l] = qbase[l * plane + gs];
+                const unsigned o4 = gs + (unsigned)(l * plane) * 4u;
+                o.dp[l] = ldf(a.dip, o4);
+                o.dd[l] = ldf(a.dcoef, o4);
+                o.r[l] = ldf(rbase, o4);
+                o.q[l] = ldf(qbase, o4);
             }
         }
     };
@@ -265,36 +283,36 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             if (fw > 0) {
                 if (c <= fw) {
                     const int pi = (jv - 1) * fw2 + (c - 1);
-                    const float p = a.pml.bx[c] * psxi[pi] + a.pml.ax[c] * sxy_x;
+                    const float p = pmlb[2 * pn + c] * psxi[pi] + pmlb[1 * pn + c] * sxy_x;
                     if (own_v) psxo[pi] = p;
-                    sxy_x = sxy_x / a.pml.Kx[c] + p;
+                    sxy_x = sxy_x / pmlb[0 * pn + c] + p;
                 }
                 if (c >= g.nx - fw + 1) {
                     const int h1 = c - g.nx + fw2;
                     const int pi = (jv - 1) * fw2 + (h1 - 1);
-                    const float p = a.pml.bx[h1] * psxi[pi] + a.pml.ax[h1] * sxy_x;
+                    const float p = pmlb[2 * pn + h1] * psxi[pi] + pmlb[1 * pn + h1] * sxy_x;
                     if (own_v) psxo[pi] = p;
-                    sxy_x = sxy_x / a.pml.Kx[h1] + p;
+                    sxy_x = sxy_x / pmlb[0 * pn + h1] + p;
                 }
                 if (!g.free_surf && jv <= fw) {
                     const int pi = (jv - 1) * g.nx + (c - 1);
-                    const float p = a.pml.by[jv] * psyi[pi] + a.pml.ay[jv] * syy_y;
+                    const float p = pmlb[8 * pn + jv] * psyi[pi] + pmlb[7 * pn + jv] * syy_y;
                     if (own_v) psyo[pi] = p;
-                    syy_y = syy_y / a.pml.Ky[jv] + p;
+                    syy_y = syy_y / pmlb[6 * pn + jv] + p;
                 }
                 if (jv >= g.ny - fw + 1) {
                     const int h1 = jv - g.ny + fw2;
                     const int pi = (h1 - 1) * g.nx + (c - 1);
-                    const float p = a.pml.by[h1] * psyi[pi] + a.pml.ay[h1] * syy_y;
+                    const float p = pmlb[8 * pn + h1] * psyi[pi] + pmlb[7 * pn + h1] * syy_y;
                     if (own_v) psyo[pi] = p;
-                    syy_y = syy_y / a.pml.Ky[h1] + p;
+                    syy_y = syy_y / pmlb[6 * pn + h1] + p;
                 }
             }
-            const int gi = gi32(g, jv, c);
+            const unsigned gi = gi32(g, jv, c);
             const float vold = cr.vy;
             const float bb = cr.b;
             const float tsum = sxy_x + syy_y;
-            const int ci = (jv - 1) * g.nx + (c - 1);
+            const unsigned ci = (unsigned)((jv - 1) * g.nx + (c - 1));
             if (!ADJ && hsum && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
                 grb[ci] = cr.gr + (acc * acc);
@@ -325,7 +343,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 }
             }
             if (own_v) {
-                vy_out[gi] = vnew;
+                stf(vy_out, gi * 4u, vnew);
                 if (!ADJ && a.seis != nullptr) {
                     const int off = cr.rrow;
                     if (off >= 0) {
@@ -360,23 +378,23 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 if (fw > 0) {
                     if (c <= fw) {
                         const int pi = (js - 1) * fw2 + (c - 1);
-                        const float p = a.pml.bxh[c] * pvxb[pi] + a.pml.axh[c] * vyx;
+                        const float p = pmlb[5 * pn + c] * pvxb[pi] + pmlb[4 * pn + c] * vyx;
                         pvxb[pi] = p;
-                        vyx = vyx / a.pml.Kxh[c] + p;
+                        vyx = vyx / pmlb[3 * pn + c] + p;
                     }
                     if (c >= g.nx - fw + 1) {
                         const int h1 = c - g.nx + fw2;
                         const int pi = (js - 1) * fw2 + (h1 - 1);
-                        const float p = a.pml.bxh[h1] * pvxb[pi] + a.pml.axh[h1] * vyx;
+                        const float p = pmlb[5 * pn + h1] * pvxb[pi] + pmlb[4 * pn + h1] * vyx;
                         pvxb[pi] = p;
-                        vyx = vyx / a.pml.Kxh[h1] + p;
+                        vyx = vyx / pmlb[3 * pn + h1] + p;
                     }
                     if (!g.free_surf && js <= fw) {
                         // Q2: the viscoelastic kernel uses the full-grid y profile on top
                         const int pi = (js - 1) * g.nx + (c - 1);
-                        const float bY = VISCO ? a.pml.by[js] : a.pml.byh[js];
-                        const float aY = VISCO ? a.pml.ay[js] : a.pml.ayh[js];
-                        const float KY = VISCO ? a.pml.Ky[js] : a.pml.Kyh[js];
+                        const float bY = VISCO ? pmlb[8 * pn + js] : pmlb[11 * pn + js];
+                        const float aY = VISCO ? pmlb[7 * pn + js] : pmlb[10 * pn + js];
+                        const float KY = VISCO ? pmlb[6 * pn + js] : pmlb[9 * pn + js];
                         const float p = bY * pvyb[pi] + aY * vyy;
                         pvyb[pi] = p;
                         vyy = vyy / KY + p;
@@ -384,12 +402,12 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                     if (js >= g.ny - fw + 1) {
                         const int h1 = js - g.ny + fw2;
                         const int pi = (h1 - 1) * g.nx + (c - 1);
-                        const float p = a.pml.byh[h1] * pvyb[pi] + a.pml.ayh[h1] * vyy;
+                        const float p = pmlb[11 * pn + h1] * pvyb[pi] + pmlb[10 * pn + h1] * vyy;
                         pvyb[pi] = p;
-                        vyy = vyy / a.pml.Kyh[h1] + p;
+                        vyy = vyy / pmlb[9 * pn + h1] + p;
                     }
                 }
-                const int gs = gi32(g, js, c);
+                const unsigned gs = gi32(g, js, c);
                 const float sxo = sq[0];
                 const float syo = zw[0];
                 float sxn, syn;
@@ -399,8 +417,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                 } else {
                     float rr[L > 0 ? L : 1], qq[L > 0 ? L : 1];
                     float sumr = 0.0f, sumq = 0.0f;
-                    float *rp = rbase + gs;
-                    float *qp = qbase + gs;
+                    const unsigned gs4 = gs * 4u;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
                         rr[l] = cr.r[l];
@@ -416,17 +433,17 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
                     for (int l = 0; l < L; ++l) {
                         rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cr.dp[l] * vyx));
                         qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cr.dd[l] * vyy));
-                        rp[l * plane] = rr[l];
-                        qp[l * plane] = qq[l];
+                        stf(rbase, gs4 + (unsigned)(l * plane) * 4u, rr[l]);
+                        stf(qbase, gs4 + (unsigned)(l * plane) * 4u, qq[l]);
                         sumr += rr[l];
                         sumq += qq[l];
                     }
                     sxn = sxn + (a.dthalbe * sumr);
                     syn = syn + (a.dthalbe * sumq);
                 }
-                sxy_out[gs] = sxn;
-                syz_out[gs] = syn;
-                const int ci = (js - 1) * g.nx + (c - 1);
+                stf(sxy_out, gs * 4u, sxn);
+                stf(syz_out, gs * 4u, syn);
+                const unsigned ci = (unsigned)((js - 1) * g.nx + (c - 1));
                 const float vc = vw[K - 1];
                 if (!ADJ && fstore) {
                     if (a.store_mode == 1) {
@@ -482,40 +499,47 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
 }
 
 template <int K, int L>
-static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, hipStream_t st)
+static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hipStream_t st)
 {
     dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave);
-    if (adjoint)
-        hipLaunchKernelGGL((k_step<K, L, true>), grid, block, 0, st, a);
-    else
-        hipLaunchKernelGGL((k_step<K, L, false>), grid, block, 0, st, a);
+    if (adjoint) {
+        if (extra)
+            hipLaunchKernelGGL((k_step<K, L, true, true>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((k_step<K, L, true, false>), grid, block, 0, st, a);
+    } else {
+        if (extra)
+            hipLaunchKernelGGL((k_step<K, L, false, true>), grid, block, 0, st, a);
+        else
+            hipLaunchKernelGGL((k_step<K, L, false, false>), grid, block, 0, st, a);
+    }
     return hipGetLastError();
 }
 
 template <int K>
-static hipError_t launch_step_k(const StepArgs &a, int L, bool adjoint, hipStream_t st)
+static hipError_t launch_step_k(const StepArgs &a, int L, bool adjoint, bool extra, hipStream_t st)
 {
     switch (L) {
-    case 0: return launch_step_kl<K, 0>(a, adjoint, st);
-    case 1: return launch_step_kl<K, 1>(a, adjoint, st);
-    case 2: return launch_step_kl<K, 2>(a, adjoint, st);
-    case 3: return launch_step_kl<K, 3>(a, adjoint, st);
-    case 4: return launch_step_kl<K, 4>(a, adjoint, st);
+    case 0: return launch_step_kl<K, 0>(a, adjoint, extra, st);
+    case 1: return launch_step_kl<K, 1>(a, adjoint, extra, st);
+    case 2: return launch_step_kl<K, 2>(a, adjoint, extra, st);
+    case 3: return launch_step_kl<K, 3>(a, adjoint, extra, st);
+    case 4: return launch_step_kl<K, 4>(a, adjoint, extra, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, hipStream_t st)
+hipError_t launch_step(const StepArgs &a, int K, bool visco, bool adjoint, bool extra, hipStream_t st)
 {
     const int L = visco ? a.g.L : 0;
     if (visco != (a.g.L > 0)) return hipErrorInvalidValue;
     switch (K) {
-    case 1: return launch_step_k<1>(a, L, adjoint, st);
-    case 2: return launch_step_k<2>(a, L, adjoint, st);
-    case 3: return launch_step_k<3>(a, L, adjoint, st);
-    case 4: return launch_step_k<4>(a, L, adjoint, st);
-    case 5: return launch_step_k<5>(a, L, adjoint, st);
-    case 6: return launch_step_k<6>(a, L, adjoint, st);
+    case 1: return launch_step_k<1>(a, L, adjoint, extra, st);
+    case 2: return launch_step_k<2>(a, L, adjoint, extra, st);
+    case 3: return launch_step_k<3>(a, L, adjoint, extra, st);
+    case 4: return launch_step_k<4>(a, L, adjoint, extra, st);
+    case 5: return launch_step_k<5>(a, L, adjoint, extra, st);
+    case 6: return launch_step_k<6>(a, L, adjoint, extra, st);
     default: return hipErrorInvalidValue;
     }
 }
