@@ -26,6 +26,7 @@ EXPORTS = [
     "dn_set_exec", "dn_autotune", "dn_pseudo_hessian", "dn_get_energy_maps", "dn_hessian_probe",
     "dn_unet_create", "dn_unet_destroy", "dn_unet_num_params", "dn_unet_get_params", "dn_unet_set_params",
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
+    "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
 ]
 
 
@@ -42,12 +43,34 @@ class Params(C.Structure):
                 ("dtinv", C.c_int), ("snap_precision", C.c_int), ("eprecond", C.c_int)]
 
 
+class Taper(C.Structure):
+    _fields_ = [("sws", C.c_int), ("gradt", C.c_int * 4), ("srtshape", C.c_int), ("srtradius", C.c_float),
+                ("filtsize", C.c_int), ("dh", C.c_float), ("map", C.POINTER(C.c_float))]
+
+
+def _taper(sws=0, gradt=(0, 0, 0, 0), srtshape=1, srtradius=1.0, filtsize=1, dh=0.0, taper_map=None):
+    """(Taper, kept array): the map array must outlive the call."""
+    t = Taper()
+    t.sws = sws
+    for k in range(4):
+        t.gradt[k] = int(gradt[k])
+    t.srtshape, t.srtradius, t.filtsize, t.dh = srtshape, srtradius, filtsize, dh
+    keep = None
+    if taper_map is not None:
+        keep = np.ascontiguousarray(taper_map, np.float32)
+        t.map = _f(keep)
+    return t, keep
+
+
 class FwiOpts(C.Structure):
     _fields_ = [("grad_method", C.c_int), ("nlbfgs", C.c_int), ("eps_scale", C.c_float), ("stepmax", C.c_int),
                 ("scalefac", C.c_float), ("testshot_start", C.c_int), ("testshot_end", C.c_int),
                 ("testshot_incr", C.c_int), ("vs_lower", C.c_float), ("vs_upper", C.c_float),
                 ("rho_lower", C.c_float), ("rho_upper", C.c_float), ("inv_vs_iter", C.c_int),
-                ("inv_rho_iter", C.c_int), ("scalerho", C.c_float), ("min_iter", C.c_int), ("pro", C.c_float)]
+                ("inv_rho_iter", C.c_int), ("scalerho", C.c_float), ("min_iter", C.c_int), ("pro", C.c_float),
+                ("spatfilter", C.c_int), ("wd_damp", C.c_float), ("wd_damp1", C.c_float), ("fc_end", C.c_float),
+                ("taper_vert", C.c_int), ("taper_hor", C.c_int), ("taper_sources", C.c_int),
+                ("taper_file", C.c_int), ("taper", Taper)]
 
 
 class FwiRecord(C.Structure):
@@ -60,7 +83,10 @@ class FwiRecord(C.Structure):
 
 FWI_DEFAULTS = dict(grad_method=2, nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1,
                     testshot_end=1, testshot_incr=1, vs_lower=0.0, vs_upper=1e9, rho_lower=0.0, rho_upper=1e9,
-                    inv_vs_iter=0, inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0)
+                    inv_vs_iter=0, inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0, spatfilter=0, wd_damp=0.0,
+                    wd_damp1=0.0, fc_end=0.0, taper_vert=0, taper_hor=0, taper_sources=0, taper_file=0)
+# taper parameters of fwi_start (go into FwiOpts.taper)
+TAPER_KEYS = ("gradt", "srtshape", "srtradius", "filtsize", "taper_map")
 
 _lib = None
 
@@ -210,8 +236,10 @@ class Context:
     # ---- FWI (dn_fwi_*) ----
     def fwi_start(self, **opts):
         o = FwiOpts()
+        tk = {k: opts.pop(k) for k in TAPER_KEYS if k in opts}
         for k, v in dict(FWI_DEFAULTS, **opts).items():
             setattr(o, k, v)
+        o.taper, self._taper_keep = _taper(**tk)
         lib().dn_fwi_start.argtypes = [C.c_void_p, C.POINTER(FwiOpts)]
         self._chk(lib().dn_fwi_start(self._h, C.byref(o)))
 
@@ -285,6 +313,20 @@ class Context:
         hr = np.zeros((self.ny, self.nx), np.float32)
         self._chk(lib().dn_pseudo_hessian(self._h, shot0, nshots, _f(hv), _f(hr)))
         return hv, hr
+
+    # ---- gradient post-processing (smooth_grad.c, taper_grad.c) ----
+    def smooth_grad(self, spatfilter, wd_damp, wd_damp1, fc_end):
+        """Smooths the context's gradients in place; returns the device ms."""
+        ms = C.c_float()
+        lib().dn_smooth_grad.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float,
+                                         C.POINTER(C.c_float)]
+        self._chk(lib().dn_smooth_grad(self._h, spatfilter, wd_damp, wd_damp1, fc_end, C.byref(ms)))
+        return ms.value
+
+    def taper_grad(self, which, sws, **kw):
+        t, keep = _taper(sws=sws, **kw)
+        lib().dn_taper_grad.argtypes = [C.c_void_p, C.c_int, C.POINTER(Taper)]
+        self._chk(lib().dn_taper_grad(self._h, which, C.byref(t)))
 
     def get_gradient(self):
         out = [np.zeros((self.ny, self.nx), np.float32) for _ in range(4)]
@@ -439,6 +481,38 @@ def timedomain_filt(data, dt, fc, order, method):
     if lib().dn_timedomain_filt(_f(d), d.shape[0], d.shape[1], dt, fc, order, method) != 0:
         raise DeniseError("dn_timedomain_filt failed")
     return d
+
+
+def gauss_filt_var(grad, vel, dh, wd_damp, wd_damp1, fc_end):
+    """gauss_filt_var.c on host arrays [ny][nx] (runs on the GPU)."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    v = np.ascontiguousarray(vel, np.float32)
+    ny, nx = g.shape
+    assert v.shape == g.shape
+    L = lib()
+    L.dn_gauss_filt_var.argtypes = [C.c_int, C.c_int] + [C.c_float] * 4 + [C.POINTER(C.c_float)] * 2
+    if L.dn_gauss_filt_var(nx, ny, dh, wd_damp, wd_damp1, fc_end, _f(v), _f(g)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return g
+
+
+def taper_grad(grad, sws, dh, src_i=(), src_j=(), rec_i=(), rec_j=(), **kw):
+    """taper_grad.c on a host array [ny][nx] (runs on the GPU); returns
+    (tapered gradient, taper coefficients)."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    ny, nx = g.shape
+    tap = np.zeros_like(g)
+    t, keep = _taper(sws=sws, dh=dh, **kw)
+    si, sj = np.ascontiguousarray(src_i, np.int32), np.ascontiguousarray(src_j, np.int32)
+    ri, rj = np.ascontiguousarray(rec_i, np.int32), np.ascontiguousarray(rec_j, np.int32)
+    L = lib()
+    ip = C.POINTER(C.c_int)
+    L.dn_taper_grad_host.argtypes = [C.c_int, C.c_int, C.POINTER(Taper), C.c_int, ip, ip, C.c_int, ip, ip,
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    if L.dn_taper_grad_host(nx, ny, C.byref(t), len(si), _i(si), _i(sj), len(ri), _i(ri), _i(rj), _f(g),
+                            _f(tap)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return g, tap
 
 
 class CheckFD(C.Structure):

@@ -35,6 +35,7 @@ SOURCES = [
     ("dn_io.cpp", False),
     ("dn_unet.hip", True),
     ("dn_unet.cpp", False),
+    ("dn_gradproc.hip", True),
 ]
 
 

@@ -33,12 +33,6 @@ int dn_fail(dn_ctx *c, const char *fmt, ...)
     return -1;
 }
 
-#define HIPCHK(ctx, expr)                                                                                  \
-    do {                                                                                                   \
-        hipError_t e_ = (expr);                                                                            \
-        if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
-    } while (0)
-
 static void dfree(void *&p)
 {
     if (p) (void)hipFree(p);
@@ -200,6 +194,7 @@ extern "C" void dn_destroy(dn_ctx *c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dn_fwi_destroy(c);
+    dn_gradproc_destroy(c);
     free_batch(c);
     dfree_t(c->d_pml);
     dfree_t(c->d_vs); dfree_t(c->d_rho); dfree_t(c->d_taus); dfree_t(c->d_uip); dfree_t(c->d_ujp);

@@ -9,6 +9,7 @@
 #include <vector>
 
 struct dn_fwi_state;
+struct dn_gradproc_state;
 struct dn_unet;
 // U-Net forward on device buffers (dn_unet.cpp): x [n][3][H][W] -> y [n][H][W]
 int dn_unet_forward_device(dn_unet *u, int n, int H, int W, const float *x_dev, float *y_dev);
@@ -87,9 +88,17 @@ struct dn_ctx {
     // learned preconditioner (owned by the caller) and the judge interval
     dn_unet *unet = nullptr;
     int judge_every = 5;
+    // gradient smoothing / taper workspace (dn_gradproc.hip)
+    dn_gradproc_state *gradproc = nullptr;
 };
 
 int dn_fail(dn_ctx *c, const char *fmt, ...);
+// returns dn_fail(ctx, ...) from the enclosing function when a HIP call fails
+#define HIPCHK(ctx, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
+    } while (0)
 int dn_prepare_model(dn_ctx *c);
 int dn_ensure_batch(dn_ctx *c, int want, bool gradient);
 int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record);
@@ -100,4 +109,10 @@ int dn_forward_shots(dn_ctx *c, const std::vector<int> &shots, float *seis_host)
 int dn_nsnap(const dn_ctx *c);
 void dn_reset_timing(dn_ctx *c);
 void dn_fwi_destroy(dn_ctx *c);
+void dn_gradproc_destroy(dn_ctx *c);
+// smooth_grad (SPATFILTER 0 or 4) and taper_grad on device planes [ny][nx]
+// (null planes are skipped), with the context's vs model and geometry
+int dn_smooth_grad_device(dn_ctx *c, int spatfilter, float wd_damp, float wd_damp1, float fc_end, float *g_vs,
+                          float *g_rho);
+int dn_taper_grad_device(dn_ctx *c, const dn_taper *T, float *g_vs, float *g_rho);
 void dn_drop_graphs(dn_ctx *c);

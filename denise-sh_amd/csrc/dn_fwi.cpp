@@ -20,12 +20,6 @@
 
 using namespace dn;
 
-#define HIPCHK(ctx, expr)                                                                                  \
-    do {                                                                                                   \
-        hipError_t e_ = (expr);                                                                            \
-        if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
-    } while (0)
-
 namespace {
 
 constexpr int kRedBlocks = 512;
@@ -373,11 +367,17 @@ static int lbfgs_direction(dn_ctx *c, const float *D)
         }
         HIPCHK(c, hipGetLastError());
         if (D) {
-            // r2 = gamma * [q_rho; D q_u], then the same second loop
-            HIPCHK(c, launch_fill(f->r2, v2, 0.0f, c->stream));
+            // r2 = gamma_D * [q_rho; D q_u], then the same second loop, with
+            // the scaling of the initial Hessian diag(1, D) taken from the
+            // newest pair: gamma_D = s'y / (y' diag(1, D) y)
+            hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, c->stream, f->r2 + n, D, yk + n, n);
+            double yDy_u, yy_rho;
+            if (dot(c, yk + n, f->r2 + n, n, &yDy_u) || dot(c, yk, yk, n, &yy_rho)) return -1;
+            const double yDy = yDy_u + yy_rho;
+            const double gamma_d = (yDy > 0.0) ? ys / yDy : gamma;
             hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, c->stream, f->r2 + n, D, f->q + n, n);
             HIPCHK(c, hipMemcpyAsync(f->r2, f->q, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-            hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c->stream, f->r2, f->r2, (float)gamma, v2);
+            hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c->stream, f->r2, f->r2, (float)gamma_d, v2);
             for (int k = 1; k <= NL; ++k) {
                 double d;
                 if (dot(c, f->y + (size_t)(k - 1) * v2, f->r2, v2, &d)) return -1;
@@ -677,6 +677,20 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
     // direction
     HIPCHK(c, hipMemcpyAsync(f->gu, c->d_gvs_out, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(f->grho, c->d_grho_out, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    // smooth_grad and the tapers (denise.c:1818-1847)
+    if (f->o.spatfilter &&
+        dn_smooth_grad_device(c, f->o.spatfilter, f->o.wd_damp, f->o.wd_damp1, f->o.fc_end, f->gu, f->grho))
+        return -1;
+    {
+        const int flags[4] = {f->o.taper_vert, f->o.taper_hor, f->o.taper_sources, f->o.taper_file};
+        const int sws[4] = {1, 2, 3, 5};
+        for (int k = 0; k < 4; ++k) {
+            if (!flags[k]) continue;
+            dn_taper T = f->o.taper;
+            T.sws = sws[k];
+            if (dn_taper_grad_device(c, &T, f->gu, f->grho)) return -1;
+        }
+    }
     if (c->precond_hook && c->precond_hook(c, f->gu, f->grho)) return -1;
     const bool learned = (c->unet != nullptr) && (f->o.grad_method == 2);
     if (learned) {

@@ -292,6 +292,45 @@ int dn_write_su(const char *path, int ntr, int ns, const float *data, const int 
 int dn_read_su(const char *path, int ntr, int ns, float *out);
 
 /* ------------------------------------------------------------------------
+ * Gradient post-processing after the chain rule (denise.c:1818-1847).
+ * Arrays are [NY][NX] row-major.
+ * ---------------------------------------------------------------------- */
+/* taper_grad.c:11-584 parameters.  sws 1: vertical window over columns
+ * GRADT1 < i < GRADT4 with Gaussian flanks, source and receiver cells zeroed;
+ * 2: zero above row GRADT1, Gaussian ramp to GRADT2, j*DH below; 3: erf
+ * (SRTSHAPE 1) or log (2) bowl of radius SRTRADIUS around every source, the
+ * (2 FILTSIZE + 1)^2 box at each source zeroed; 5/6: the coefficients of
+ * taper.bin given as map (5 for vs, 6 for rho in denise.c:1832, 1846). */
+typedef struct dn_taper {
+    int sws;
+    int gradt[4];         /* GRADT1..GRADT4 (grid indices)                         */
+    int srtshape;         /* SRTSHAPE                                              */
+    float srtradius;      /* SRTRADIUS [m]                                         */
+    int filtsize;         /* FILTSIZE [cells]                                      */
+    float dh;             /* DH [m]; the handle API uses the context's DH           */
+    const float *map;     /* sws 5/6: [NY][NX] taper coefficients                   */
+} dn_taper;
+
+/* smooth_grad.c:10-35 on the context's gradients (after dn_gradient):
+ * SPATFILTER 4 = gauss_filt_var.c:10-215, the spatially variable Gaussian
+ * whose half widths follow the local wavelength of the context's vs model
+ * (WD_DAMP1 along x, WD_DAMP along y, times vs / FC_END); 0 = none.
+ * SPATFILTER 1-3 are not available (error).  Bit-identical to the
+ * reference's float loop.  ms (may be NULL) receives the device time. */
+int dn_smooth_grad(dn_ctx *ctx, int spatfilter, float wd_damp, float wd_damp1, float fc_end, float *ms);
+/* taper_grad.c on the context's gradients: which 1 vs, 2 rho, 3 both;
+ * sources and receivers are the context's. */
+int dn_taper_grad(dn_ctx *ctx, int which, const dn_taper *taper);
+/* The same two on host arrays, without a context (gauss_filt_var(waveconv,
+ * vel_mod) and taper_grad(waveconv, taper_coeff, srcpos, nshots, recpos, ntr,
+ * iter, sws) of fd.h with grid-index sources): grad is filtered in place;
+ * taper_out (may be NULL) receives the coefficients. */
+int dn_gauss_filt_var(int nx, int ny, float dh, float wd_damp, float wd_damp1, float fc_end, const float *vel,
+                      float *grad);
+int dn_taper_grad_host(int nx, int ny, const dn_taper *taper, int nsrc, const int *src_i, const int *src_j, int nrec,
+                       const int *rec_i, const int *rec_j, float *grad, float *taper_out);
+
+/* ------------------------------------------------------------------------
  * FWI iteration on the device (denise.c:695-2018, line_search.c,
  * calc_mat_change_test.c, calc_opt_step.c, LBFGS.c, PCG.c and the
  * store/extract helpers).  Dot products and misfit sums are FP64; the
@@ -309,6 +348,12 @@ typedef struct dn_fwi_opts {
     float scalerho;       /* SCALERHO                                              */
     int min_iter;         /* MIN_ITER                                              */
     float pro;            /* PRO: relative misfit change that ends a stage          */
+    /* gradient post-processing (denise.c:1818-1847), applied to the summed
+     * gradient before the optimiser */
+    int spatfilter;       /* SPATFILTER of the stage: 0 none, 4 gauss_filt_var     */
+    float wd_damp, wd_damp1, fc_end;   /* WD_DAMP (y), WD_DAMP1 (x), FC_END           */
+    int taper_vert, taper_hor, taper_sources, taper_file;   /* SWS_TAPER_GRAD_*, SWS_TAPER_FILE */
+    dn_taper taper;       /* GRADT1..4, SRTSHAPE, SRTRADIUS, FILTSIZE, map (sws ignored) */
 } dn_fwi_opts;
 
 /* One line of the reference's misfit log (denise.c:1929-1933) plus extras. */
@@ -334,8 +379,9 @@ int dn_fwi_start(dn_ctx *ctx, const dn_fwi_opts *opts);
 int dn_fwi_iteration(dn_ctx *ctx, dn_fwi_record *rec);
 /* Learned preconditioner for L-BFGS (NULL: off).  The network's
  * prediction H of the vs diagonal Hessian gives the initial L-BFGS Hessian
- * gamma * diag(We / H) for vs (We the eprecond1.c energy weight, normalised
- * to mean 1); the plain direction (the reference's) judges it on the first
+ * gamma_D * diag(1, D), D = We / H for vs (We the eprecond1.c energy weight;
+ * D normalised to mean 1), gamma_D = s'y / (y' diag(1, D) y) from the newest
+ * pair; the plain direction (the reference's) judges it on the first
  * iteration of a stage and every judge_every-th (0: only the first), the
  * larger misfit drop of the two being taken, and replaces it whenever its
  * line search finds no decrease.

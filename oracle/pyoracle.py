@@ -253,3 +253,30 @@ class Oracle:
         ncl, n = gnew.shape
         self.fn("orc_pcg")(ncl, n, self._r(gnew), self._r(gold), self._r(d))
         return d
+
+    # ---- gradient post-processing ([ny][nx] arrays) ----
+    def gauss_filt_var(self, grad, vel, dh, wd_damp, wd_damp1, fc_end):
+        g = np.ascontiguousarray(grad, self.dt).copy()
+        v = np.ascontiguousarray(vel, np.float32)
+        ny, nx = g.shape
+        f32 = lambda x: float(np.float32(x))
+        self.fn("orc_gauss_filt_var")(nx, ny, C.c_double(f32(dh)), C.c_double(f32(wd_damp)),
+                                      C.c_double(f32(wd_damp1)), C.c_double(f32(fc_end)), self._f32(v), self._r(g))
+        return g
+
+    def taper_grad(self, grad, sws, dh, gradt=(0, 0, 0, 0), srtshape=1, srtradius=1.0, filtsize=1,
+                   src_i=(), src_j=(), rec_i=(), rec_j=()):
+        """Returns (tapered gradient, taper coefficients)."""
+        g = np.ascontiguousarray(grad, self.dt).copy()
+        ny, nx = g.shape
+        tap = np.zeros((ny, nx), np.float32)
+        gt = np.ascontiguousarray(gradt, np.int32)
+        si, sj = np.ascontiguousarray(src_i, np.int32), np.ascontiguousarray(src_j, np.int32)
+        ri, rj = np.ascontiguousarray(rec_i, np.int32), np.ascontiguousarray(rec_j, np.int32)
+        f32 = lambda x: float(np.float32(x))
+        rc = self.fn("orc_taper_grad")(nx, ny, C.c_double(f32(dh)), sws, self._i32(gt), srtshape,
+                                       C.c_double(f32(srtradius)), filtsize, len(si), self._i32(si), self._i32(sj),
+                                       len(ri), self._i32(ri), self._i32(rj), self._r(g), self._f32(tap))
+        if rc:
+            raise RuntimeError("oracle taper_grad: unknown sws %d" % sws)
+        return g, tap

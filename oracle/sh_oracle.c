@@ -1378,3 +1378,214 @@ double F(orc_calc_mat_change)(int nx, int ny, const real *gu, const real *grho, 
     if (eps_rho_out) *eps_rho_out = EPSILON_rho;
     return (double)EPSILON_u;
 }
+
+/* ------------------------------------------------------------------------
+ * Gradient post-processing after the chain rule (denise.c:1818-1847), one
+ * domain (NXG = NX, NYG = NY, POS = {0, 0}).  Arrays are [NY][NX] row-major
+ * (x fastest), index (j-1)*NX + (i-1) for the reference's [j][i].
+ * ---------------------------------------------------------------------- */
+
+/* util.c:38-72 minimum_m / maximum_m: first element, then all others. */
+static float orc_min_m(const float *m, size_t n)
+{
+    float v = m[0];
+    for (size_t t = 1; t < n; ++t)
+        if (m[t] < v) v = m[t];
+    return v;
+}
+static float orc_max_m(const float *m, size_t n)
+{
+    float v = m[0];
+    for (size_t t = 1; t < n; ++t)
+        if (m[t] > v) v = m[t];
+    return v;
+}
+
+/* gauss_filt_var.c:10-215 (SPATFILTER 4 of smooth_grad.c:31-33): every cell
+ * is replaced by the normalised sum of a Gaussian of the gradient around it,
+ * whose half widths follow the local wavelength lam = vel / FC_END:
+ *   FILT_SIZE_X = round(WD_DAMP1 lam / DH), FILT_SIZE_Z = round(WD_DAMP lam / DH)
+ *   (made odd, at least 6), hfs = FILT_SIZE / 2, sigma = hfs / 2 (integer
+ *   division), s = 2 sigma^2, kernel(ii, jj) = exp(-ii^2/sx - jj^2/sz),
+ *   ii in [-hfsx, hfsx] (x, outer loop), jj in [-hfsz, hfsz] (y, inner loop);
+ * cells with FILT_SIZE_Z <= 0 are copied.  Outside the grid the gradient is
+ * continued with its edge values (gauss_filt_var.c:126-157), i.e. indices
+ * are clamped.  The float build keeps the reference's float operands, the
+ * double exp() rounded to float, and the sequential float sums.
+ * vel: the current vs model (pu at denise.c:1818).
+ * Pins (tests/test_oracle_pins.py): a constant gradient is returned
+ * unchanged; for uniform velocity the result equals scipy.ndimage.correlate
+ * with the normalised kernel built from the closed form (mode "nearest", x
+ * and y half widths different so a transposed kernel fails); for a two-layer
+ * model, cells farther than their half width from the interface equal the
+ * uniform result of their own layer; FILT_SIZE_Z <= 0 copies. */
+void F(orc_gauss_filt_var)(int nx, int ny, double dh_in, double wd_damp_in, double wd_damp1_in, double fc_end_in,
+                           const float *vel, real *grad)
+{
+    const real DH = (real)dh_in, WD_DAMP = (real)wd_damp_in, WD_DAMP1 = (real)wd_damp1_in, FC_END = (real)fc_end_in;
+    const size_t n = (size_t)nx * (size_t)ny;
+    real *tmp = (real *)malloc((n ? n : 1) * sizeof(real));
+    memcpy(tmp, grad, n * sizeof(real));
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t t = (size_t)(j - 1) * nx + (size_t)(i - 1);
+            real lam = (real)vel[t] / FC_END;
+            int fx = (int)round((WD_DAMP1 * lam) / DH);
+            int fz = (int)round((WD_DAMP * lam) / DH);
+            if (fz > 0) {
+                if (!(fx % 2)) fx += 1;
+                if (!(fz % 2)) fz += 1;
+                if (fx < 6) fx = 6;
+                if (fz < 6) fz = 6;
+                const int hx = abs(fx) / 2, hz = abs(fz) / 2;
+                const real sigx = hx / 2, sigz = hz / 2;
+                const real sx = 2.0 * sigx * sigx, sz = 2.0 * sigz * sigz;
+                real conv = 0.0, sum = 0.0;
+                for (int ii = -hx; ii <= hx; ii++)
+                    for (int jj = -hz; jj <= hz; jj++) {
+                        const real kern = exp(-((ii * ii) / sx) - ((jj * jj) / sz));
+                        int y = j + jj, x = i + ii;
+                        y = y < 1 ? 1 : (y > ny ? ny : y);
+                        x = x < 1 ? 1 : (x > nx ? nx : x);
+                        conv += tmp[(size_t)(y - 1) * nx + (size_t)(x - 1)] * kern;
+                        sum += kern;
+                    }
+                grad[t] = conv / sum;
+            } else {
+                grad[t] = tmp[t];
+            }
+        }
+    free(tmp);
+}
+
+/* taper_grad.c:11-584 for sws 1 (vertical), 2 (horizontal) and 3 (taper
+ * around the sources); the taper coefficients go to taper [NY][NX], the
+ * gradient is multiplied by them and then zeroed where the reference zeroes
+ * it.  Sources and receivers are grid indices (iround(srcpos / DH) and
+ * recpos).  gradt = GRADT1..GRADT4.
+ *   sws 1: window over columns GRADT1 < i < GRADT4, Gaussian flanks of
+ *          lengths GRADT2-GRADT1 and GRADT4-GRADT3 (a = 3), zero elsewhere;
+ *          source and receiver cells zeroed (taper_grad.c:50-160).
+ *   sws 2: zero above row GRADT1, Gaussian ramp to row GRADT2, j*DH below
+ *          (EXP_TAPER_GRAD_HOR = 1) (taper_grad.c:166-290).
+ *   sws 3: erf (SRTSHAPE 1) or log (2) bowl of diameter 2*SRTRADIUS around
+ *          every source, minimum over the sources, shifted and normalised to
+ *          [0, 1]; the (2 FILTSIZE + 1)^2 box around each source zeroed
+ *          (taper_grad.c:295-520).
+ * Pins: the closed forms of each window and bowl in float64
+ * (tests/test_oracle_pins.py). Returns 0, or -1 for an unknown sws. */
+int F(orc_taper_grad)(int nx, int ny, double dh_in, int sws, const int *gradt, int srtshape, double srtradius_in,
+                      int filtsize, int nsrc, const int *src_i, const int *src_j, int nrec, const int *rec_i,
+                      const int *rec_j, real *grad, float *taper)
+{
+    const float DH = (float)dh_in, SRTRADIUS = (float)srtradius_in;
+    const int GRADT1 = gradt[0], GRADT2 = gradt[1], GRADT3 = gradt[2], GRADT4 = gradt[3];
+    const size_t n = (size_t)nx * (size_t)ny;
+#define T(j, i) taper[(size_t)((j)-1) * nx + (size_t)((i)-1)]
+#define G(j, i) grad[(size_t)((j)-1) * nx + (size_t)((i)-1)]
+    if (sws == 1) {
+        const int tl = GRADT2 - GRADT1, tl2 = GRADT4 - GRADT3, ifw = GRADT4 - GRADT1;
+        float *win = (float *)calloc((size_t)(ifw > 0 ? ifw : 0) + 2, sizeof(float));
+        const float a = 3;
+        for (int i = 1; i <= ifw; i++) {
+            win[i] = 1.0;
+            if (i <= tl) win[i] = exp(-(1.0 / 2.0) * (a * (i - tl) / (tl / 2.0)) * (a * (i - tl) / (tl / 2.0)));
+            if (i >= (ifw - tl2))
+                win[i] = exp(-(1.0 / 2.0) * (a * (i - (ifw - tl2)) / (tl2 / 2.0)) * (a * (i - (ifw - tl2)) / (tl2 / 2.0)));
+        }
+        for (int j = 1; j <= ny; j++) {
+            int h = 1;
+            for (int i = 1; i <= nx; i++) {
+                float tap = 0.0;
+                if ((i > GRADT1) && (i < GRADT4)) tap = win[h++];
+                T(j, i) = tap;
+            }
+        }
+        free(win);
+        for (size_t t = 0; t < n; t++) grad[t] *= taper[t];
+        for (int s = 0; s < nsrc; s++)
+            if (src_i[s] >= 1 && src_i[s] <= nx && src_j[s] >= 1 && src_j[s] <= ny) G(src_j[s], src_i[s]) = 0.0;
+        for (int r = 0; r < nrec; r++)
+            if (rec_i[r] >= 1 && rec_i[r] <= nx && rec_j[r] >= 1 && rec_j[r] <= ny) G(rec_j[r], rec_i[r]) = 0.0;
+        return 0;
+    }
+    if (sws == 2) {
+        const int tl = GRADT2 - GRADT1, ifw = GRADT2 - GRADT1 + 1;
+        float *win = (float *)calloc((size_t)(ifw > 0 ? ifw : 0) + 2, sizeof(float));
+        const float a = 3;
+        for (int i = 1; i <= ifw; i++) {
+            win[i] = 1.0;
+            if (i <= tl) win[i] = exp(-(1.0 / 2.0) * (a * (i - tl) / (tl / 2.0)) * (a * (i - tl) / (tl / 2.0)));
+        }
+        int h = 1;
+        for (int j = 1; j <= ny; j++) {
+            for (int i = 1; i <= nx; i++) {
+                float tap = 0.0;
+                if ((j >= GRADT1) && (j <= GRADT2)) tap = win[h];
+                if (j > GRADT2) tap = pow((float)(j * DH), 1.0);
+                T(j, i) = tap;
+            }
+            if (j >= GRADT1) h++;
+        }
+        free(win);
+        for (size_t t = 0; t < n; t++) grad[t] *= taper[t];
+        return 0;
+    }
+    if (sws == 3) {
+        int ng = (int)(ceil(2.0 * SRTRADIUS / DH));
+        if (ng < 5) ng = 5;
+        float *m = (float *)malloc((size_t)ng * ng * sizeof(float));
+        float *msum = (float *)malloc(n * sizeof(float));
+#define M(j, i) m[(size_t)((j)-1) * ng + (size_t)((i)-1)]
+        for (size_t t = 0; t < n; t++) msum[t] = 1.0;
+        const float a = 1.0;
+        const float maxrad = sqrt(2.0 * SRTRADIUS * SRTRADIUS);
+        for (int j = 1; j <= ng; j++)
+            for (int i = 1; i <= ng; i++) {
+                const float x = ((float)i - ((float)ng) / 2.0 - 0.5) * DH;
+                const float y = ((float)j - ((float)ng) / 2.0 - 0.5) * DH;
+                const float rad = sqrt(x * x + y * y);
+                if (srtshape == 1) M(j, i) = erf(a * rad / maxrad);
+                else if (srtshape == 2) M(j, i) = (rad > 0) ? log(rad) : 0.0;
+                else M(j, i) = 0.0;
+            }
+        /* minimum_m(m, ng, ng) visits i outer, j inner; the first minimum
+         * is the same value whatever the order */
+        const float minm = orc_min_m(m, (size_t)ng * ng);
+        for (size_t t = 0; t < (size_t)ng * ng; t++) m[t] -= minm;
+        float edge[4] = {M(1, ng / 2), M(ng / 2, 1), M(ng / 2, ng), M(ng, ng / 2)};
+        const float maxm = orc_max_m(edge, 4);
+        for (size_t t = 0; t < (size_t)ng * ng; t++) {
+            m[t] /= maxm;
+            if (m[t] > 1.0) m[t] = 1.0;
+        }
+        const int ijc = (int)(ceil((float)ng / 2));
+        for (int s = 0; s < nsrc; s++)
+            for (int iy = 1; iy <= ng; iy++)
+                for (int ix = 1; ix <= ng; ix++) {
+                    const int xx = src_i[s] + ix - ijc, yy = src_j[s] + iy - ijc;
+                    if ((xx < 1) || (xx > nx) || (yy < 1) || (yy > ny)) continue;
+                    float *p = &msum[(size_t)(yy - 1) * nx + (size_t)(xx - 1)];
+                    if (*p > M(iy, ix)) *p = M(iy, ix);
+                }
+#undef M
+        const float mn = orc_min_m(msum, n);
+        for (size_t t = 0; t < n; t++) msum[t] -= mn;
+        const float mx = orc_max_m(msum, n);
+        for (size_t t = 0; t < n; t++) taper[t] = msum[t] / mx;
+        free(m);
+        free(msum);
+        for (size_t t = 0; t < n; t++) grad[t] *= taper[t];
+        for (int s = 0; s < nsrc; s++)
+            for (int i = src_i[s] - filtsize; i <= src_i[s] + filtsize; i++)
+                for (int j = src_j[s] - filtsize; j <= src_j[s] + filtsize; j++)
+                    if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+                        G(j, i) = 0.0;
+                        T(j, i) = 0.0;
+                    }
+        return 0;
+    }
+#undef T
+#undef G
+    return -1;
+}

@@ -26,6 +26,12 @@ What pins what:
   lbfgs/pcg/opt    secant condition and textbook two-loop agreement;
                    A-conjugacy of PR directions with exact line search;
                    vertex of an exact parabola; relative step rule.
+  gauss_filt_var   scipy.ndimage.correlate (mode "nearest") with the
+                   normalised closed-form Gaussian, per velocity (uniform and
+                   two-layer models, hand-computed half widths); constant
+                   gradient unchanged; WD_DAMP = 0 copies.
+  taper_grad       closed forms of the vertical/horizontal windows and of the
+                   erf bowl around a source in float64; zeroed cells.
 """
 import ctypes as C
 import os
@@ -646,3 +652,135 @@ def test_pseudo_hessian_is_sum_of_squared_snapshots():
     assert np.max(last) > 0          # the left-out last stored step carries energy (quirk matters)
     h32v, h32r = O32.pseudo_hessian(c, vs, rho, qs, si, sj, sig)
     assert rel_l2(h32v, hv) < 1e-5 and rel_l2(h32r, hr) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# gradient post-processing (gauss_filt_var.c, taper_grad.c)
+# ---------------------------------------------------------------------------
+def _gauss_kernel(hx, hz):
+    """Normalised Gaussian with half widths hx (x) and hz (y), sigma = h // 2."""
+    sx, sz = 2.0 * (hx // 2) ** 2, 2.0 * (hz // 2) ** 2
+    jj = np.arange(-hz, hz + 1, dtype=np.float64)[:, None]
+    ii = np.arange(-hx, hx + 1, dtype=np.float64)[None, :]
+    k = np.exp(-ii * ii / sx - jj * jj / sz)
+    return k / k.sum()
+
+
+def test_gauss_filt_var_uniform_is_normalised_gaussian_correlation():
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(11)
+    ny, nx = 37, 53
+    g = rng.standard_normal((ny, nx))
+    # vs 400, FC_END 20: lam 20 m; WD_DAMP 0.25 -> 5 m / 0.25 m = 20 -> 21 cells,
+    # hz = 10; WD_DAMP1 0.15 -> 12 -> 13 cells, hx = 6
+    vel = np.full((ny, nx), 400.0, np.float32)
+    out = O64.gauss_filt_var(g, vel, 0.25, 0.25, 0.15, 20.0)
+    ref = correlate(g, _gauss_kernel(6, 10), mode="nearest")
+    assert np.abs(out - ref).max() < 1e-12 * np.abs(ref).max()
+    # a transposed kernel (hx <-> hz) is far off
+    wrong = correlate(g, _gauss_kernel(10, 6), mode="nearest")
+    assert np.abs(out - wrong).max() > 1e-2
+    out32 = O32.gauss_filt_var(g.astype(np.float32), vel, 0.25, 0.25, 0.15, 20.0)
+    assert rel_l2(out32, ref) < 1e-6
+
+
+def test_gauss_filt_var_minimum_size_and_two_layers():
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(12)
+    ny, nx = 30, 41
+    g = rng.standard_normal((ny, nx))
+    vel = np.full((ny, nx), 400.0, np.float32)
+    vel[ny // 2:, :] = 100.0
+    # top: hx 6, hz 10 as above; bottom (vs 100): lam 5 m -> 5 and 3 cells,
+    # odd, raised to 6 -> hx = hz = 3
+    out = O64.gauss_filt_var(g, vel, 0.25, 0.25, 0.15, 20.0)
+    top = correlate(g, _gauss_kernel(6, 10), mode="nearest")
+    bot = correlate(g, _gauss_kernel(3, 3), mode="nearest")
+    assert np.abs(out[:ny // 2] - top[:ny // 2]).max() < 1e-12
+    assert np.abs(out[ny // 2:] - bot[ny // 2:]).max() < 1e-12
+
+
+def test_gauss_filt_var_constant_and_copy():
+    vel = (150.0 + 100.0 * np.random.default_rng(3).random((20, 25))).astype(np.float32)
+    c = np.full((20, 25), 2.5)
+    assert np.abs(O64.gauss_filt_var(c, vel, 0.25, 0.3, 0.2, 30.0) - 2.5).max() < 1e-14
+    g = np.random.default_rng(4).standard_normal((20, 25)).astype(np.float32)
+    # WD_DAMP 0: FILT_SIZE_Z = 0 -> copy
+    assert np.array_equal(O32.gauss_filt_var(g, vel, 0.25, 0.0, 0.2, 30.0), g)
+
+
+def _win(h, tl):
+    return np.exp(-0.5 * (3.0 * (h - tl) / (tl / 2.0)) ** 2)
+
+
+def test_taper_grad_vertical_window_closed_form():
+    ny, nx = 12, 60
+    g = np.ones((ny, nx), np.float32)
+    gt = (5, 15, 40, 50)
+    src_i, src_j = [30], [4]
+    rec_i, rec_j = [20, 22], [2, 2]
+    out, tap = O32.taper_grad(g, 1, 0.25, gradt=gt, src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j)
+    exp = np.zeros(nx)
+    for i in range(1, nx + 1):
+        if 5 < i < 50:
+            h = i - 5
+            w = 1.0
+            if h <= 10:
+                w = _win(h, 10)
+            if h >= 45 - 10:
+                w = _win(h - 35 + 10, 10)
+            exp[i - 1] = w
+    assert np.abs(tap - exp[None, :]).max() < 1e-6
+    # equal flanks: the window is symmetric about the middle of (GRADT1, GRADT4),
+    # peaks at 1 and vanishes outside
+    assert np.array_equal(tap[0, 5:49], tap[0, 5:49][::-1])
+    assert tap.max() == 1.0 and not tap[:, :5].any() and not tap[:, 49:].any()
+    want = np.broadcast_to(exp, (ny, nx)).copy()
+    want[3, 29] = 0.0
+    want[1, 19] = want[1, 21] = 0.0
+    assert np.abs(out - want).max() < 1e-6
+    assert out[3, 29] == 0.0 and out[1, 19] == 0.0
+
+
+def test_taper_grad_horizontal_window_closed_form():
+    ny, nx = 40, 7
+    g = np.full((ny, nx), 2.0, np.float32)
+    out, tap = O32.taper_grad(g, 2, 0.5, gradt=(10, 20, 0, 0))
+    col = np.zeros(ny)
+    for j in range(1, ny + 1):
+        if 10 <= j <= 20:
+            col[j - 1] = _win(j - 10 + 1, 10) if j - 10 + 1 <= 10 else 1.0
+        elif j > 20:
+            col[j - 1] = j * 0.5
+    assert np.abs(tap - col[:, None]).max() < 1e-5
+    assert np.abs(out - 2.0 * col[:, None]).max() < 1e-5
+
+
+def test_taper_grad_source_bowl_closed_form():
+    from math import erf
+    ny, nx, dh, R, fs = 41, 45, 0.5, 4.0, 1
+    si, sj = 23, 21
+    g = np.ones((ny, nx), np.float32)
+    out, tap = O32.taper_grad(g, 3, dh, srtshape=1, srtradius=R, filtsize=fs, src_i=[si], src_j=[sj])
+    ng = int(np.ceil(2 * R / dh))
+    maxrad = np.sqrt(2 * R * R)
+    m = np.zeros((ng, ng))
+    for j in range(1, ng + 1):
+        for i in range(1, ng + 1):
+            x, y = (i - ng / 2 - 0.5) * dh, (j - ng / 2 - 0.5) * dh
+            m[j - 1, i - 1] = erf(np.hypot(x, y) / maxrad)
+    m -= m.min()
+    edge = max(m[0, ng // 2 - 1], m[ng // 2 - 1, 0], m[ng // 2 - 1, ng - 1], m[ng - 1, ng // 2 - 1])
+    m = np.minimum(m / edge, 1.0)
+    ms = np.ones((ny, nx))
+    c = int(np.ceil(ng / 2))
+    for iy in range(1, ng + 1):
+        for ix in range(1, ng + 1):
+            xx, yy = si + ix - c, sj + iy - c
+            if 1 <= xx <= nx and 1 <= yy <= ny:
+                ms[yy - 1, xx - 1] = min(ms[yy - 1, xx - 1], m[iy - 1, ix - 1])
+    ms = (ms - ms.min()) / (ms - ms.min()).max()
+    ms[sj - 1 - fs:sj + fs, si - 1 - fs:si + fs] = 0.0
+    assert np.abs(tap - ms).max() < 1e-6
+    assert np.abs(out - ms).max() < 1e-6
+    assert tap.max() == 1.0 and out[sj - 1, si - 1] == 0.0
