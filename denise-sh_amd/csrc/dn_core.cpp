@@ -551,7 +551,7 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
     return 0;
 }
 
-// rows per segment: enough workgroups to fill 256 CUs several times over
+// rows per segment: enough waves to fill 256 CUs several times over
 static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
 {
     if (c->rows_req > 0) return std::min(c->rows_req, c->g.ny);
@@ -560,9 +560,9 @@ static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
     int best = 16;
     for (int R : cand) {
         const int Rr = std::min(R, c->g.ny);
-        const long long blocks = (long long)nshots * nstrips * ((c->g.ny + Rr - 1) / Rr);
+        const long long waves = (long long)nshots * nstrips * ((c->g.ny + Rr - 1) / Rr);
         best = Rr;
-        if (blocks >= target) break;
+        if (waves >= target) break;
     }
     return std::max(std::min(best, c->g.ny), std::min(4 * c->K, c->g.ny));
 }
@@ -576,9 +576,10 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a.nshots_all = c->nshots;
     a.ow = kWave + 1 - 2 * c->K;
     a.nstrips = (g.nx + a.ow - 1) / a.ow;
-    a.R = choose_rows(c, n, a.nstrips);
+    a.ngroups = (a.nstrips + kStripsPerGroup - 1) / kStripsPerGroup;
+    a.R = choose_rows(c, n, a.ngroups * kStripsPerGroup);
     a.nsegs = (g.ny + a.R - 1) / a.R;
-    a.nblocks = n * a.nstrips * a.nsegs;
+    a.nblocks = n * a.ngroups * a.nsegs;
     a.nblocks8 = (a.nblocks + 7) / 8;
     a.DT = c->p.dt;
     a.DH = c->p.dh;

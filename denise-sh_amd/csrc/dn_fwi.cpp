@@ -292,7 +292,7 @@ extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
                                    g.nx * sizeof(float), g.ny, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(f->m0_vs, f->m_vs, nxny * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(f->m0_rho, f->m_rho, nxny * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        f->min_iter_help = o->min_iter;
+        f->min_iter_help = 0;   // denise.c:138: MIN_ITER first applies after a stage change
     } else {
         // a new stage: the reference resets the optimiser (zero_LBFGS / zero_PCG) and iter
         f->stage++;

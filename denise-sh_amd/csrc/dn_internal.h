@@ -8,6 +8,7 @@
 namespace dn {
 
 constexpr int kWave = 64;      // CDNA wavefront width: one wave per strip
+constexpr int kStripsPerGroup = 4;   // waves (adjacent strips) per workgroup of the time step
 constexpr int kMaxL = 4;       // relaxation mechanisms supported
 // Zero frame around the interior of every padded field.  The fused step
 // reads up to 2K-1 rows above/below the interior, 2K-1 columns left of it
@@ -64,7 +65,8 @@ struct StepArgs {
     int nshots;                    // shots in this launch
     int nshots_all;                // shots of the context (bound of shot_map entries)
     int ow, nstrips, R, nsegs;     // strip width (owned columns), strips, rows per segment, segments
-    int nblocks, nblocks8;         // real blocks and blocks per XCD slice
+    int ngroups;                   // workgroups of kStripsPerGroup adjacent strips per segment row
+    int nblocks, nblocks8;         // real workgroups and workgroups per XCD slice
     float DT, DH, dhi, dthalbe;
     float hc[7];
     float hcy6;                    // coefficient used for the 6th y-term of update_v (Q6: hc[5])

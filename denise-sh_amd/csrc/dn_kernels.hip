@@ -73,7 +73,7 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 // tile share the static model fields through that XCD's L2.
 // ---------------------------------------------------------------------------
 template <int K, int L, bool ADJ, bool EX>
-__global__ __launch_bounds__(64) void k_step(const StepArgs a)
+__global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs a)
 {
     // EX: this launch may store snapshots / sum the pseudo-Hessian (forward)
     // or correlate (adjoint); the launches without (most steps) carry none
@@ -81,11 +81,18 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
     const int lin = (w & 7) * a.nblocks8 + (w >> 3);
-    if (lin >= a.nblocks) return;
+    if (lin >= a.nblocks) return;                 // whole workgroup
     const int s = lin % a.nshots;
     const int tile = lin / a.nshots;
-    const int strip = tile % a.nstrips;
-    const int seg = tile / a.nstrips;
+    const int grp = tile % a.ngroups;
+    const int seg = tile / a.ngroups;
+    // the waves of a workgroup take adjacent strips of the same rows and
+    // step through the rows together (one s_barrier per row), so their
+    // loads of a row arrive at the memory as one contiguous run per field;
+    // spare waves of the last group re-read the last strip and store nothing
+    const int wv = threadIdx.x / kWave;
+    const bool live = grp * kStripsPerGroup + wv < a.nstrips;
+    const int strip = live ? grp * kStripsPerGroup + wv : a.nstrips - 1;
     const Geo &g = a.g;
     const int S = a.shot_map[s];
     if (S < 0 || S >= a.nshots_all) return;   // a stale or foreign shot map: touch nothing
@@ -93,11 +100,11 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     const int nt = a.nt_base ? (*a.nt_base + a.nt) : a.nt;
     if (nt > NTs) return;  // shot finished (uneven record lengths)
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % kWave;
     const int c0 = 1 + strip * a.ow;          // first owned column
     const int c = c0 - (K - 1) + lane;        // column of this lane
     const bool in_x = (c >= 1) && (c <= g.nx);
-    const bool own_x = (lane >= K - 1) && (lane < K - 1 + a.ow) && in_x;
+    const bool own_x = live && (lane >= K - 1) && (lane < K - 1 + a.ow) && in_x;
     const int j0 = 1 + seg * a.R;
     const int j1 = min(j0 + a.R - 1, g.ny);
     const long long so = (long long)s * g.plane;
@@ -133,8 +140,10 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
     float *__restrict__ sxy_out = a.sxy_out + so;
     float *__restrict__ syz_out = a.syz_out + so;
 
-    __shared__ float lsx[kWave + 2 * K];
-    __shared__ float lvy[kWave];
+    __shared__ float lsx_all[kStripsPerGroup][kWave + 2 * K];
+    __shared__ float lvy_all[kStripsPerGroup][kWave];
+    float *const lsx = lsx_all[wv];   // per-wave exchange rows
+    float *const lvy = lvy_all[wv];
 
     // The padded planes hold a zero frame of at least 2K-1 rows above and
     // below the interior, 2K-1 columns left and 64 right (dn_internal.h), so
@@ -470,12 +479,13 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
             }
         }
         __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_barrier();   // keep the workgroup's strips on the same row
 #pragma unroll
         for (int t = 0; t < 2 * K - 1; ++t) zw[t] = zw[t + 1];
     }
 
     // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
-    if (!ADJ && fstore && a.store_mode == 2) {
+    if (!ADJ && fstore && a.store_mode == 2 && live) {
         // recomputed from the staged values this wave wrote (owned cells only)
         float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
         if (own_x) {
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(64) void k_step(const StepArgs a)
 template <int K, int L>
 static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hipStream_t st)
 {
-    dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave);
+    dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave * kStripsPerGroup);
     if (adjoint) {
         if (extra)
             hipLaunchKernelGGL((k_step<K, L, true, true>), grid, block, 0, st, a);

@@ -21,7 +21,8 @@ class Cfg(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("maxrelerror", C.c_int), ("fw", C.c_int),
                 ("free_surf", C.c_int), ("L", C.c_int), ("nt", C.c_int), ("dtinv", C.c_int),
                 ("dt", C.c_double), ("dh", C.c_double), ("damping", C.c_double), ("fpml", C.c_double),
-                ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int)]
+                ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int),
+                ("c_vs", C.c_double), ("c_rho", C.c_double)]
 
 
 def build():

@@ -67,6 +67,9 @@ typedef struct {
     double damping, fpml, npower, k_max_pml;
     double fl[8];        /* relaxation frequencies FL[1..L] at fl[0..L-1]   */
     int eprecond;        /* EPRECOND: 1 = eprecond1 energy scaling, 0 = none */
+    double c_vs, c_rho;  /* > 0: the normalisation means C_vs, C_rho (fixed at
+                            the first FWI iteration, denise.c:745-816);
+                            0: the means of the model passed in            */
 } orc_cfg;
 
 #define ND(c) ((c)->fdorder / 2 + 1)
@@ -1142,6 +1145,8 @@ int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const f
     rho_avg /= Lcount;
     C_vs = Vs_avg;
     C_rho = rho_avg;
+    if (c->c_vs > 0.0) C_vs = (real)c->c_vs;
+    if (c->c_rho > 0.0) C_rho = (real)c->c_rho;
     *l2 = 0.0;
     *energy = 0.0;
 
