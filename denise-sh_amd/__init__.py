@@ -27,6 +27,7 @@ EXPORTS = [
     "dn_unet_create", "dn_unet_destroy", "dn_unet_num_params", "dn_unet_get_params", "dn_unet_set_params",
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
+    "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files",
 ]
 
 
@@ -60,6 +61,21 @@ def _taper(sws=0, gradt=(0, 0, 0, 0), srtshape=1, srtradius=1.0, filtsize=1, dh=
         keep = np.ascontiguousarray(taper_map, np.float32)
         t.map = _f(keep)
     return t, keep
+
+
+class DataCond(C.Structure):
+    _fields_ = [("trkill", C.c_int), ("kill", C.POINTER(C.c_int)), ("kill_nsrc", C.c_int), ("kill_ntr", C.c_int),
+                ("offset_mute", C.c_int), ("offsetc", C.c_float), ("src_x", C.POINTER(C.c_float)),
+                ("src_y", C.POINTER(C.c_float))]
+
+
+class StfOpts(C.Structure):
+    _fields_ = [("eps_stf", C.c_float), ("offsetc_stf", C.c_float), ("normalize", C.c_int)]
+
+
+class FwiFiles(C.Structure):
+    _fields_ = [("misfit_log", C.c_char_p), ("time_filt", C.c_int), ("fc_start", C.c_float), ("fc", C.c_float),
+                ("jacobian", C.c_char_p), ("inv_modelfile", C.c_char_p), ("write_it", C.c_int)]
 
 
 class FwiOpts(C.Structure):
@@ -214,6 +230,7 @@ class Context:
         sj = np.ascontiguousarray(src_j, np.int32)
         self._chk(lib().dn_set_shots(self._h, ns, ntm, _i(nt), _i(si), _i(sj), _f(sig)))
         self.nshots, self.nt_max = ns, ntm
+        self.shot_nt = nt.copy()
 
     def set_shots_multi(self, src_off, src_i, src_j, signals, nt):
         """Shots with several sources: those of shot s are src_off[s]..src_off[s+1]-1."""
@@ -224,6 +241,7 @@ class Context:
         self._chk(lib().dn_set_shots_multi(self._h, ns, ntm, _i(nt), _i(off), _i(np.ascontiguousarray(src_i, np.int32)),
                                            _i(np.ascontiguousarray(src_j, np.int32)), _f(sig)))
         self.nshots, self.nt_max = ns, ntm
+        self.shot_nt = nt.copy()
 
     def set_signals(self, signals):
         self._chk(lib().dn_set_signals(self._h, _f(np.ascontiguousarray(signals, np.float32))))
@@ -251,6 +269,22 @@ class Context:
         out["epst1"] = list(r.epst1)
         out["L2t"] = list(r.L2t)
         return out
+
+    def fwi_write_files(self, rec, misfit_log=None, time_filt=0, fc_start=0.0, fc=0.0, jacobian=None,
+                        inv_modelfile=None, write_it=0):
+        """Writes the reference's per-iteration files for the record rec of
+        fwi_iteration()."""
+        r = FwiRecord()
+        for k, _ in FwiRecord._fields_:
+            if k in ("epst1", "L2t"):
+                for i, v in enumerate(rec[k]):
+                    getattr(r, k)[i] = v
+            else:
+                setattr(r, k, rec[k])
+        enc = lambda x: None if x is None else x.encode()
+        f = FwiFiles(enc(misfit_log), time_filt, fc_start, fc, enc(jacobian), enc(inv_modelfile), write_it)
+        lib().dn_fwi_write_files.argtypes = [C.c_void_p, C.POINTER(FwiRecord), C.POINTER(FwiFiles)]
+        self._chk(lib().dn_fwi_write_files(self._h, C.byref(r), C.byref(f)))
 
     def fwi_model(self):
         vs = np.zeros((self.ny, self.nx), np.float32)
@@ -313,6 +347,33 @@ class Context:
         hr = np.zeros((self.ny, self.nx), np.float32)
         self._chk(lib().dn_pseudo_hessian(self._h, shot0, nshots, _f(hv), _f(hr)))
         return hv, hr
+
+    # ---- data conditioning (calc_res.c, calc_energy.c) and stf.c ----
+    def set_datacond(self, trkill=0, kill=None, offset_mute=0, offsetc=0.0, src_x=None, src_y=None):
+        """kill: the TRKILL_FILE integers as a [nsrc_file][ntr_file] array."""
+        d = DataCond()
+        keep = []
+        d.trkill, d.offset_mute, d.offsetc = trkill, offset_mute, offsetc
+        if kill is not None:
+            k = np.ascontiguousarray(kill, np.int32)
+            keep.append(k)
+            d.kill = _i(k)
+            d.kill_nsrc, d.kill_ntr = k.shape
+        for name, v in (("src_x", src_x), ("src_y", src_y)):
+            if v is not None:
+                a = np.ascontiguousarray(v, np.float32)
+                keep.append(a)
+                setattr(d, name, _f(a))
+        lib().dn_set_datacond.argtypes = [C.c_void_p, C.POINTER(DataCond)]
+        off = trkill == 0 and offset_mute == 0
+        self._chk(lib().dn_set_datacond(self._h, None if off else C.byref(d)))
+
+    def stf_shot(self, shot, eps_stf=0.05, offsetc_stf=0.0, normalize=0):
+        o = StfOpts(eps_stf, offsetc_stf, normalize)
+        out = np.zeros(int(self.shot_nt[shot]), np.float32)
+        lib().dn_stf_shot.argtypes = [C.c_void_p, C.c_int, C.POINTER(StfOpts), C.POINTER(C.c_float)]
+        self._chk(lib().dn_stf_shot(self._h, shot, C.byref(o), _f(out)))
+        return out
 
     # ---- gradient post-processing (smooth_grad.c, taper_grad.c) ----
     def smooth_grad(self, spatfilter, wd_damp, wd_damp1, fc_end):
@@ -513,6 +574,37 @@ def taper_grad(grad, sws, dh, src_i=(), src_j=(), rec_i=(), rec_j=(), **kw):
                             _f(tap)) != 0:
         raise DeniseError(L.dn_last_error(None).decode())
     return g, tap
+
+
+def time_window(data, dt, timewin, gamma, twlength_plus=0.0, picks=None):
+    """time_window.c on data [ntr][ns] (host); returns the windowed copy."""
+    d = np.ascontiguousarray(data, np.float32).copy()
+    ntr, ns = d.shape
+    p = None if picks is None else np.ascontiguousarray(picks, np.float32)
+    L = lib()
+    L.dn_time_window.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float,
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    if L.dn_time_window(ntr, ns, dt, timewin, gamma, twlength_plus, _f(p), _f(d)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return d
+
+
+def stf(obs, syn, signal, mute=None, eps_stf=0.05, normalize=0):
+    """stf.c's Wiener deconvolution on host arrays (runs on the GPU)."""
+    o = np.ascontiguousarray(obs, np.float32)
+    m = np.ascontiguousarray(syn, np.float32)
+    s = np.ascontiguousarray(signal, np.float32)
+    ntr, ns = o.shape
+    mu = None if mute is None else np.ascontiguousarray(mute, np.uint8)
+    out = np.zeros(ns, np.float32)
+    opts = StfOpts(eps_stf, 0.0, normalize)
+    L = lib()
+    L.dn_stf.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(C.c_ubyte), C.POINTER(StfOpts), C.POINTER(C.c_float)]
+    mp = None if mu is None else mu.ctypes.data_as(C.POINTER(C.c_ubyte))
+    if L.dn_stf(ntr, ns, _f(o), _f(m), _f(s), mp, C.byref(opts), _f(out)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return out
 
 
 class CheckFD(C.Structure):

@@ -36,6 +36,7 @@ SOURCES = [
     ("dn_unet.hip", True),
     ("dn_unet.cpp", False),
     ("dn_gradproc.hip", True),
+    ("dn_stf.hip", True),
 ]
 
 

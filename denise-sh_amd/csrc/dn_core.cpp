@@ -201,6 +201,7 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_b); dfree_t(c->d_tausipjp); dfree_t(c->d_fipjp); dfree_t(c->d_f); dfree_t(c->d_dip);
     dfree_t(c->d_d); dfree_t(c->d_gu); dfree_t(c->d_grho); dfree_t(c->d_gvs_out); dfree_t(c->d_grho_out);
     dfree_t(c->d_maxbits);
+    dfree_t(c->d_trace_flags);
     dfree_t(c->d_rec_row); dfree_t(c->d_rec_tab);
     dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
     dfree_t(c->d_obs);
@@ -324,6 +325,8 @@ extern "C" int dn_set_receivers(dn_ctx *c, int ntr, const int *ri, const int *rj
     c->ntr = ntr;
     c->rec_i.assign(ri, ri + ntr);
     c->rec_j.assign(rj, rj + ntr);
+    c->dc = dn_ctx::DataCond{};   // conditioning refers to receivers: reset
+    c->dc_dirty = true;
     free_batch(c);  // seismogram buffers depend on ntr
     return 0;
 }
@@ -361,6 +364,9 @@ extern "C" int dn_set_shots_multi(dn_ctx *c, int nshots, int nt_max, const int *
     c->shot_nt.assign(nt, nt + nshots);
     c->src_i.assign(si, si + nsrc);
     c->src_j.assign(sj, sj + nsrc);
+    c->src_off.assign(src_off, src_off + nshots + 1);
+    c->dc = dn_ctx::DataCond{};   // conditioning refers to shots: reset
+    c->dc_dirty = true;
     c->shot_l2.assign(nshots, 0.0);
     c->shot_energy.assign(nshots, 0.0);
     c->cur_map.clear();
@@ -814,11 +820,82 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     return 0;
 }
 
+extern "C" int dn_set_datacond(dn_ctx *c, const dn_datacond *d)
+{
+    if (!c) return dn_fail(nullptr, "dn_set_datacond: ctx is NULL");
+    dn_ctx::DataCond n{};
+    if (d) {
+        if (c->nshots < 1 || c->ntr < 1) return dn_fail(c, "dn_set_datacond: set receivers and shots first");
+        if (d->offset_mute < 0 || d->offset_mute > 2) return dn_fail(c, "OFFSET_MUTE must be 0, 1 or 2");
+        n.on = d->trkill != 0 || d->offset_mute != 0;
+        n.trkill = d->trkill;
+        n.offset_mute = d->offset_mute;
+        n.offsetc = d->offsetc;
+        if (d->trkill) {
+            if (!d->kill || d->kill_nsrc < c->nshots || d->kill_ntr < c->ntr)
+                return dn_fail(c, "TRKILL: the kill list must cover %d shots x %d receivers", c->nshots, c->ntr);
+            n.kill_nsrc = d->kill_nsrc;
+            n.kill_ntr = d->kill_ntr;
+            n.kill.assign(d->kill, d->kill + (size_t)d->kill_nsrc * d->kill_ntr);
+        }
+        if (d->offset_mute) {
+            n.src_x.resize(c->nshots);
+            n.src_y.resize(c->nshots);
+            for (int s = 0; s < c->nshots; ++s) {
+                const int l = c->src_off[s];
+                n.src_x[s] = d->src_x ? d->src_x[s] : (float)c->src_i[l] * c->p.dh;
+                n.src_y[s] = d->src_y ? d->src_y[s] : (float)c->src_j[l] * c->p.dh;
+            }
+        }
+    }
+    c->dc = std::move(n);
+    c->dc_dirty = true;
+    return 0;
+}
+
+// per (shot, receiver) flags of the residual kernel from the conditioning
+static int trace_flags(dn_ctx *c, const unsigned char **out)
+{
+    *out = nullptr;
+    if (!c->dc.on) return 0;
+    const size_t n = (size_t)c->nshots * c->ntr;
+    if (c->dc_dirty || !c->d_trace_flags) {
+        const dn_ctx::DataCond &d = c->dc;
+        std::vector<unsigned char> f(n, 0);
+        const float DH = c->p.dh;
+        for (int s = 0; s < c->nshots; ++s)
+            for (int r = 0; r < c->ntr; ++r) {
+                unsigned char v = 0;
+                if (d.trkill) {
+                    if (d.kill[(size_t)s * d.kill_ntr + r] == 1) v |= 1;   // calc_res.c:44-47 kill_tmp[ishot][rec]
+                    if (d.kill[(size_t)r * d.kill_nsrc + s] == 1) v |= 2;  // calc_energy.c:30-33 kill_tmp[rec][ishot]
+                }
+                if (d.offset_mute) {
+                    // calc_res.c:101-108 in float
+                    const float xr = (float)c->rec_i[r] * DH, yr = (float)c->rec_j[r] * DH;
+                    const float xs = d.src_x[s], ys = d.src_y[s];
+                    const float offset = (float)std::sqrt((double)(((xs - xr) * (xs - xr)) + ((ys - yr) * (ys - yr))));
+                    if ((d.offset_mute == 1) && (offset >= d.offsetc)) v |= 1;
+                    if ((d.offset_mute == 2) && (offset <= d.offsetc)) v |= 1;
+                }
+                f[(size_t)s * c->ntr + r] = v;
+            }
+        dfree_t(c->d_trace_flags);
+        HIPCHK(c, hipMalloc((void **)&c->d_trace_flags, n));
+        HIPCHK(c, hipMemcpy(c->d_trace_flags, f.data(), n, hipMemcpyHostToDevice));
+        c->dc_dirty = false;
+    }
+    *out = c->d_trace_flags;
+    return 0;
+}
+
 int dn_batch_residual(dn_ctx *c, const std::vector<int> &shots)
 {
     const int n = (int)shots.size();
+    const unsigned char *flags = nullptr;
+    if (trace_flags(c, &flags)) return -1;
     HIPCHK(c, launch_residual(n, c->nshots, c->ntr, c->nt_max, c->d_shot_map, c->d_shot_nt, c->d_seis, c->d_obs, c->d_resid,
-                              c->d_part, c->p.dt, c->stream));
+                              c->d_part, c->p.dt, flags, c->stream));
     std::vector<double> part((size_t)n * c->ntr * 2);
     HIPCHK(c, hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -90,6 +90,18 @@ struct dn_ctx {
     int judge_every = 5;
     // gradient smoothing / taper workspace (dn_gradproc.hip)
     dn_gradproc_state *gradproc = nullptr;
+    // data conditioning of the residual (dn_set_datacond): parameters and the
+    // per (shot, receiver) flags derived from them for the residual kernel
+    struct DataCond {
+        bool on = false;
+        int trkill = 0, kill_nsrc = 0, kill_ntr = 0, offset_mute = 0;
+        float offsetc = 0.0f;
+        std::vector<int> kill;
+        std::vector<float> src_x, src_y;
+    } dc;
+    bool dc_dirty = false;
+    unsigned char *d_trace_flags = nullptr;
+    std::vector<int> src_off;        // host copy of the per-shot source ranges
 };
 
 int dn_fail(dn_ctx *c, const char *fmt, ...);

@@ -13,7 +13,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -804,6 +806,49 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
     f->iter++;
     f->iter_true++;
     if (rec) *rec = R;
+    return 0;
+}
+
+extern "C" int dn_fwi_write_files(dn_ctx *c, const dn_fwi_record *rec, const dn_fwi_files *F)
+{
+    if (!c || !c->fwi || !rec || !F) return dn_fail(c, "dn_fwi_write_files: NULL argument or no FWI state");
+    const Geo &g = c->g;
+    const size_t n = c->fwi->nxny;
+    if (F->misfit_log) {
+        FILE *fp = std::fopen(F->misfit_log, rec->iter_true == 1 ? "w" : "a");
+        if (!fp) return dn_fail(c, "Could not open %s", F->misfit_log);
+        const float *e = rec->epst1, *L = rec->L2t;
+        if (F->time_filt == 0)
+            std::fprintf(fp, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t  %e \t %d \n", rec->opteps_vs,
+                         e[0], e[1], e[2], L[0], L[1], L[2], L[3], rec->M2_vs, rec->M2_rho, rec->stage);
+        else
+            std::fprintf(fp, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %f \t %f \t %d \n",
+                         rec->opteps_vs, e[0], e[1], e[2], L[0], L[1], L[2], L[3], rec->M2_vs, rec->M2_rho,
+                         F->fc_start, F->fc, rec->stage);
+        std::fclose(fp);
+    }
+    std::vector<float> a(n), b(n);
+    auto write = [&](const std::string &path, const float *v) { return dn_write_model_file(path.c_str(), g.nx, g.ny, v); };
+    if (F->jacobian) {
+        HIPCHK(c, hipMemcpy(a.data(), c->fwi->gu, n * sizeof(float), hipMemcpyDeviceToHost));
+        if (write(std::string(F->jacobian) + "_grad_vs.bin", a.data())) return -1;
+    }
+    if (F->inv_modelfile) {
+        const std::string base = F->inv_modelfile;
+        HIPCHK(c, hipMemcpy(a.data(), c->fwi->m_vs, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(b.data(), c->fwi->m_rho, n * sizeof(float), hipMemcpyDeviceToHost));
+        if (write(base + "_vs.bin", a.data()) || write(base + "_rho.bin", b.data())) return -1;
+        if (F->write_it) {
+            const std::string it = std::to_string(rec->iter);
+            if (write(base + "_vs_it_" + it + ".bin", a.data()) || write(base + "_rho_it_" + it + ".bin", b.data()))
+                return -1;
+        }
+        if (rec->next_stage) {
+            const std::string st = std::to_string(rec->stage);
+            if (write(base + "_vs_stage_" + st + ".bin", a.data()) || write(base + "_rho_stage_" + st + ".bin", b.data()))
+                return -1;
+        }
+    }
     return 0;
 }
 

@@ -141,7 +141,8 @@ hipError_t launch_hessian_finish(int nshots, int nx, int ny, const Geo &g, const
 
 // per-shot residual (calc_res / calc_energy): one thread per trace
 hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
-                           const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st);
+                           const float *syn, const float *obs, float *resid, double *part, float DT,
+                           const unsigned char *flags, hipStream_t st);
 
 // energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)
 // (precond 0: EPRECOND 0, plain sum).  ws_sum/wr_sum collect the energy maps

@@ -669,8 +669,12 @@ hipError_t launch_model_prep(const ModelPrepArgs &a, hipStream_t st)
 // calc_energy.c:53-61).  One thread per (shot, trace); per-trace FP64 partial
 // sums are written in place and reduced on the host in a fixed order.
 // ---------------------------------------------------------------------------
+// flags [nshots_all][ntr] (or null): bit 0 leaves the trace out of the
+// residual and L2 (trace kill or offset mute, calc_res.c:99-113), bit 1 out
+// of the energy (trace kill as calc_energy.c:23-49 reads the kill file)
 __global__ void k_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
-                           const float *syn, const float *obs, float *resid, double *part, float DT)
+                           const float *syn, const float *obs, float *resid, double *part, float DT,
+                           const unsigned char *flags)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nshots * ntr) return;
@@ -685,15 +689,20 @@ __global__ void k_residual(int nshots, int nshots_all, int ntr, int nt_max, cons
     const float *sy = syn + (long long)t * nt_max;
     const float *ob = obs + ((long long)S * ntr + r) * nt_max;
     float *df = resid + (long long)t * nt_max;
+    const unsigned fl = flags ? flags[(long long)S * ntr + r] : 0u;
+    const bool res = !(fl & 1u), nrg = !(fl & 2u);
     double l2 = 0.0, en = 0.0;
     float intseis = 0.0f;
     for (int j = 1; j <= ns; ++j) {
-        const float o = (j == 1) ? sy[0] : ob[j - 1];   // calc_res.c:152
-        const float d = sy[j - 1] - o;
+        // calc_res.c:152 sets obs[1] = syn[1] for the traces it covers
+        const float o = (j == 1 && res) ? sy[0] : ob[j - 1];
+        const float d = res ? sy[j - 1] - o : 0.0f;
         df[ns - j] = d;                                 // time reversal, calc_res.c:224
         l2 += (double)d * (double)d;
-        intseis = intseis + o;                          // calc_energy.c:59-60
-        en += (double)(intseis * intseis * DT * DT);
+        if (nrg) {
+            intseis = intseis + o;                      // calc_energy.c:59-60
+            en += (double)(intseis * intseis * DT * DT);
+        }
     }
     for (int j = ns; j < nt_max; ++j) df[j] = 0.0f;
     part[2 * (long long)t] = l2;
@@ -701,11 +710,12 @@ __global__ void k_residual(int nshots, int nshots_all, int ntr, int nt_max, cons
 }
 
 hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, const int *shot_map, const int *shot_nt,
-                           const float *syn, const float *obs, float *resid, double *part, float DT, hipStream_t st)
+                           const float *syn, const float *obs, float *resid, double *part, float DT,
+                           const unsigned char *flags, hipStream_t st)
 {
     const int n = nshots * ntr;
     hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64), dim3(64), 0, st, nshots, nshots_all, ntr, nt_max, shot_map,
-                       shot_nt, syn, obs, resid, part, DT);
+                       shot_nt, syn, obs, resid, part, DT, flags);
     return hipGetLastError();
 }
 

@@ -369,3 +369,27 @@ extern "C" int dn_checkfd(int nx, int ny, int fdorder, int maxrelerror, int L, c
     return (out->dt_unstable || out->fw_too_large) ? 1 : 0;
 }
 
+
+int dn_fail(dn_ctx *c, const char *fmt, ...);   // dn_core.cpp
+
+// time_window.c:40-60: damping exp(-GAMMA (t - pick)^2) for t = j DT >= pick
+extern "C" int dn_time_window(int ntr, int ns, float dt, int timewin, float gamma, float twlength_plus,
+                              const float *picks, float *data)
+{
+    if (ntr < 0 || ns < 0 || !data || (timewin != 1 && timewin != 2) || (timewin == 1 && !picks)) {
+        dn_fail(nullptr, "dn_time_window: TIMEWIN must be 1 (with picks) or 2");
+        return -1;
+    }
+    const float DT = dt, GAMMA = gamma;
+    for (int i = 0; i < ntr; ++i) {
+        const float pick = (timewin == 1) ? picks[i] + twlength_plus : DT;
+        float *row = data + (size_t)i * ns;
+        for (int j = 1; j <= ns; ++j) {
+            const float time = (float)(j * DT);
+            const float dump = time - pick;
+            const float taper = (float)std::exp((double)(-GAMMA * dump * dump));
+            if (time >= pick) row[j - 1] = row[j - 1] * taper;
+        }
+    }
+    return 0;
+}

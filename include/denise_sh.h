@@ -292,6 +292,58 @@ int dn_write_su(const char *path, int ntr, int ns, const float *data, const int 
 int dn_read_su(const char *path, int ntr, int ns, float *out);
 
 /* ------------------------------------------------------------------------
+ * Data conditioning of the residual (calc_res.c:37-113, calc_energy.c:23-49)
+ * and of observed data (time_window.c).
+ * ---------------------------------------------------------------------- */
+typedef struct dn_datacond {
+    int trkill;            /* TRKILL: 1 = leave out the traces the kill list marks        */
+    const int *kill;       /* TRKILL_FILE contents in file order (kill_nsrc x kill_ntr)   */
+    int kill_nsrc, kill_ntr;   /* its dimensions: the reference's nsrc_glob, ntr_glob     */
+    int offset_mute;       /* OFFSET_MUTE: 1 mute offsets >= OFFSETC, 2 offsets <= OFFSETC */
+    float offsetc;         /* OFFSETC [m]                                                 */
+    const float *src_x, *src_y;   /* per shot source position [m] (srcpos[1], srcpos[2]);
+                                     NULL: the shot's first source grid index times DH    */
+} dn_datacond;
+/* Applies to every later residual of the context (gradients, misfits, line
+ * search): a killed or muted trace has a zero residual and no L2 share
+ * (calc_res.c); a killed trace has no energy share, where the kill list is
+ * read as calc_energy.c reads it, with receiver and shot indices swapped
+ * (calc_energy.c:23-33 vs calc_res.c:37-47).  The reference's offset uses
+ * srcpos in metres and recpos * DH.  NULL switches conditioning off.
+ * Call after dn_set_receivers / dn_set_shots (they reset it). */
+int dn_set_datacond(dn_ctx *ctx, const dn_datacond *dc);
+/* time_window.c on data [ntr][ns] in place: samples at t = j DT >= pick are
+ * multiplied by exp(-GAMMA (t - pick)^2).  TIMEWIN 2: pick = DT for every
+ * trace; TIMEWIN 1: pick = picks[i] + TWLENGTH_PLUS (the reference reads the
+ * picks from a file whose name it never sets, time_window.c:17-33; here the
+ * caller passes them). */
+int dn_time_window(int ntr, int ns, float dt, int timewin, float gamma, float twlength_plus, const float *picks,
+                   float *data);
+
+/* stf.c:11-250: source time function of one shot by Wiener deconvolution of
+ * the observed by the modelled traces, applied to the current signal:
+ *   STF = Re IFFT( S sum_i D_i conj(M_i) / (sum_i |M_i|^2 + EPS_STF ntr Ebar) )
+ * on Npad = 2^(ceil(log2 ns)+2) points; killed / muted traces left out;
+ * NORMALIZE 1 / 2 trace scaling (util.c:489-579).  FFT and sums on the
+ * device (FP64; the reference sums Ebar in float). */
+typedef struct dn_stf_opts {
+    float eps_stf;        /* EPS_STF                                              */
+    float offsetc_stf;    /* OFFSETC_STF [m]; <= 0: no STF offset mute            */
+    int normalize;        /* NORMALIZE: 0, 1 (each trace to its max), 2 (observed
+                             trace to the modelled trace's max)                   */
+} dn_stf_opts;
+/* On the context: the shot's observed data and its synthetics modelled with
+ * its current signal, the context's trace kill and offset mute plus
+ * OFFSETC_STF; stf receives the shot's nt samples.  The caller installs the
+ * result as the shot's signal (dn_set_signals), as denise.c:1063-1066 does
+ * by switching to QUELLART 3. */
+int dn_stf_shot(dn_ctx *ctx, int shot, const dn_stf_opts *opts, float *stf);
+/* The same on host arrays: obs, syn [ntr][ns], signal [ns], mute [ntr]
+ * (nonzero: trace left out) or NULL. */
+int dn_stf(int ntr, int ns, const float *obs, const float *syn, const float *signal, const unsigned char *mute,
+           const dn_stf_opts *opts, float *stf);
+
+/* ------------------------------------------------------------------------
  * Gradient post-processing after the chain rule (denise.c:1818-1847).
  * Arrays are [NY][NX] row-major.
  * ---------------------------------------------------------------------- */
@@ -377,6 +429,29 @@ int dn_fwi_start(dn_ctx *ctx, const dn_fwi_opts *opts);
 /* One iteration: gradient over all shots, direction, line search on the
  * test shots, model update; fills rec. */
 int dn_fwi_iteration(dn_ctx *ctx, dn_fwi_record *rec);
+/* Files of an FWI iteration under the reference's names and formats:
+ *   misfit_log: one line per iteration (denise.c:1929-1933), the file
+ *     created at iteration 1 over all stages and appended to after;
+ *   <jacobian>_grad_vs.bin: the vs search direction (denise.c:1889-1891);
+ *   <inv_modelfile>_vs.bin / _rho.bin: the updated model
+ *     (calc_mat_change_test.c:191-220), with write_it also
+ *     <inv_modelfile>_vs_it_<iter>.bin / _rho_it_<iter>.bin (at iter ==
+ *     nfstart, calc_mat_change_test.c:221-241);
+ *   after the last iteration of a stage (rec->next_stage)
+ *     <inv_modelfile>_vs_stage_<stage>.bin / _rho_stage_<stage>.bin
+ *     (model_freq_out.c).
+ * Model files are x-major float32 (writemod.c + mergemod.c on one domain).
+ * NULL names skip a file. */
+typedef struct dn_fwi_files {
+    const char *misfit_log;   /* MISFIT_LOG_FILE                                    */
+    int time_filt;            /* TIME_FILT: the log line carries FC_START and FC    */
+    float fc_start, fc;       /* FC_START, FC (FC_END of the stage)                 */
+    const char *jacobian;     /* JACOBIAN                                           */
+    const char *inv_modelfile;/* INV_MODELFILE                                      */
+    int write_it;             /* also the per-iteration model files                 */
+} dn_fwi_files;
+int dn_fwi_write_files(dn_ctx *ctx, const dn_fwi_record *rec, const dn_fwi_files *files);
+
 /* Learned preconditioner for L-BFGS (NULL: off).  The network's
  * prediction H of the vs diagonal Hessian gives the initial L-BFGS Hessian
  * gamma_D * diag(1, D), D = We / H for vs (We the eprecond1.c energy weight;

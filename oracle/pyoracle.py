@@ -22,7 +22,7 @@ class Cfg(C.Structure):
                 ("free_surf", C.c_int), ("L", C.c_int), ("nt", C.c_int), ("dtinv", C.c_int),
                 ("dt", C.c_double), ("dh", C.c_double), ("damping", C.c_double), ("fpml", C.c_double),
                 ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int),
-                ("c_vs", C.c_double), ("c_rho", C.c_double)]
+                ("c_vs", C.c_double), ("c_rho", C.c_double), ("skip_res", C.c_void_p), ("skip_en", C.c_void_p)]
 
 
 def build():
@@ -281,3 +281,12 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle taper_grad: unknown sws %d" % sws)
         return g, tap
+
+    def time_window(self, data, dt, timewin, gamma, twlength_plus=0.0, picks=None):
+        d = np.ascontiguousarray(data, self.dt).copy()
+        ntr, ns = d.shape
+        p = None if picks is None else np.ascontiguousarray(picks, np.float32)
+        f32 = lambda x: float(np.float32(x))
+        self.fn("orc_time_window")(ntr, ns, C.c_double(f32(dt)), timewin, C.c_double(f32(gamma)),
+                                   C.c_double(f32(twlength_plus)), self._f32(p), self._r(d))
+        return d

@@ -70,6 +70,8 @@ typedef struct {
     double c_vs, c_rho;  /* > 0: the normalisation means C_vs, C_rho (fixed at
                             the first FWI iteration, denise.c:745-816);
                             0: the means of the model passed in            */
+    const unsigned char *skip_res, *skip_en;   /* [nshots][ntr] data
+                            conditioning masks of orc_residual_cond, or NULL */
 } orc_cfg;
 
 #define ND(c) ((c)->fdorder / 2 + 1)
@@ -1031,12 +1033,18 @@ int F(orc_forward)(const orc_cfg *c, const float *vs, const float *rho, const fl
 
 /* ------------------------------------------------------------------------
  * calc_res.c:150-257 (LNORM 2, GRAD_FORM 2, RTM 0) and calc_energy.c:53-61.
- * obs is modified in place (obs[i][1] := syn[i][1], Q7).  diff [ntr][ns] is
- * time-reversed.  l2 and energy are accumulated in double here; the
- * reference accumulates in float (noted where compared).
+ * obs is modified in place (obs[i][1] := syn[i][1], Q7) for the traces the
+ * residual covers.  diff [ntr][ns] is time-reversed.  l2 and energy are
+ * accumulated in double here; the reference accumulates in float (noted
+ * where compared).
+ * Data conditioning (calc_res.c:37-113, calc_energy.c:23-49): skip_res[i]
+ * != 0 (trace killed by TRKILL or muted by OFFSET_MUTE) leaves the trace's
+ * residual zero, out of L2 and its obs[1] unchanged; skip_en[i] != 0 (the
+ * trace kill as calc_energy reads the kill file) leaves it out of the
+ * energy.  NULL masks: no conditioning.
  * ---------------------------------------------------------------------- */
-void F(orc_residual)(int ntr, int ns, double dt_in, const real *syn, real *obs, real *diff, double *l2,
-                     double *energy)
+void F(orc_residual_cond)(int ntr, int ns, double dt_in, const real *syn, real *obs, real *diff, double *l2,
+                          double *energy, const unsigned char *skip_res, const unsigned char *skip_en)
 {
     const real DT = (real)dt_in;
     int i, j, invtime;
@@ -1045,22 +1053,33 @@ void F(orc_residual)(int ntr, int ns, double dt_in, const real *syn, real *obs, 
         const real *sy = syn + (size_t)i * ns;
         real *ob = obs + (size_t)i * ns, *df = diff + (size_t)i * ns;
         real intseis;
-        invtime = ns;
-        for (j = 1; j <= ns; j++) {
-            if (j == 1) ob[0] = sy[0];
-            intseis = sy[j - 1] - ob[j - 1];
-            df[invtime - 1] = intseis;
-            invtime--;
+        for (j = 0; j < ns; j++) df[j] = 0.0;
+        if (!(skip_res && skip_res[i])) {
+            invtime = ns;
+            for (j = 1; j <= ns; j++) {
+                if (j == 1) ob[0] = sy[0];
+                intseis = sy[j - 1] - ob[j - 1];
+                df[invtime - 1] = intseis;
+                invtime--;
+            }
+            for (j = 0; j < ns; j++) L2 += (double)df[j] * (double)df[j];
         }
-        for (j = 0; j < ns; j++) L2 += (double)df[j] * (double)df[j];
-        intseis = 0.0;
-        for (j = 0; j < ns; j++) {
-            intseis += ob[j];
-            en += (double)(intseis * intseis * DT * DT);
+        if (!(skip_en && skip_en[i])) {
+            intseis = 0.0;
+            for (j = 0; j < ns; j++) {
+                intseis += ob[j];
+                en += (double)(intseis * intseis * DT * DT);
+            }
         }
     }
     *l2 += L2;
     *energy += en;
+}
+
+void F(orc_residual)(int ntr, int ns, double dt_in, const real *syn, real *obs, real *diff, double *l2,
+                     double *energy)
+{
+    F(orc_residual_cond)(ntr, ns, dt_in, syn, obs, diff, l2, energy, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------
@@ -1159,7 +1178,9 @@ int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const f
                             rec_i, rec_j, seis, snaps, Ws);
         if (rc) break;
         for (t = 0; t < (size_t)ntr * NT; t++) obsr[t] = (real)obs[(size_t)ishot * ntr * NT + t];
-        F(orc_residual)(ntr, NT, c->dt, seis, obsr, diff, l2, energy);
+        F(orc_residual_cond)(ntr, NT, c->dt, seis, obsr, diff, l2, energy,
+                             c->skip_res ? c->skip_res + (size_t)ishot * ntr : NULL,
+                             c->skip_en ? c->skip_en + (size_t)ishot * ntr : NULL);
         rc = F(orc_adjoint)(c, vs, rho, qs, ntr, rec_i, rec_j, diff, snaps, gu_shot, grho_shot, Wr);
         if (rc) break;
         if (!c->eprecond) { /* EPRECOND 0: the shot gradients are summed unscaled (denise.c:1665) */
@@ -1593,4 +1614,23 @@ int F(orc_taper_grad)(int nx, int ny, double dh_in, int sws, const int *gradt, i
 #undef T
 #undef G
     return -1;
+}
+
+/* time_window.c:40-60 on data [ntr][ns]: samples with t = j DT >= pick are
+ * multiplied by exp(-GAMMA (t - pick)^2); pick = DT (TIMEWIN 2) or
+ * picks[i] + TWLENGTH_PLUS (TIMEWIN 1).  Pin: the closed form in float64
+ * (tests/test_oracle_pins.py). */
+void F(orc_time_window)(int ntr, int ns, double dt_in, int timewin, double gamma_in, double twplus_in,
+                        const float *picks, real *data)
+{
+    const real DT = (real)dt_in, GAMMA = (real)gamma_in, TWLENGTH_PLUS = (real)twplus_in;
+    for (int i = 0; i < ntr; i++) {
+        const real pick = (timewin == 1) ? (real)picks[i] + TWLENGTH_PLUS : DT;
+        for (int j = 1; j <= ns; j++) {
+            const real time = (real)(j * DT);
+            const real dump = time - pick;
+            const real taper = exp(-GAMMA * dump * dump);
+            if (time >= pick) data[(size_t)i * ns + (j - 1)] = data[(size_t)i * ns + (j - 1)] * taper;
+        }
+    }
 }

@@ -155,3 +155,31 @@ def test_fwi_iteration_with_smoothing_and_taper():
     assert rel_l2(unit(dv), unit(w)) < 1e-3
     # without the post-processing the update would point elsewhere
     assert rel_l2(unit(dv), unit(gvs)) > 1e-1
+
+
+def test_fwi_files_have_the_reference_names_and_formats(tmp_path):
+    dn = product()
+    ctx, vs, rho, _, _ = _ctx_case(dn)
+    ctx.fwi_start(testshot_start=1, testshot_end=3, testshot_incr=1, vs_lower=50.0, vs_upper=800.0,
+                  rho_lower=500.0, rho_upper=4000.0, pro=1e9)
+    log = str(tmp_path / "misfit.dat")
+    lines = []
+    for it in range(2):
+        r = ctx.fwi_iteration()
+        ctx.fwi_write_files(r, misfit_log=log, time_filt=1, fc_start=0.0, fc=40.0, jacobian=str(tmp_path / "jac"),
+                            inv_modelfile=str(tmp_path / "model"), write_it=1)
+        e, L = r["epst1"], r["L2t"]
+        lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %f \t %f \t %d \n" % (
+            r["opteps_vs"], e[0], e[1], e[2], L[0], L[1], L[2], L[3], r["M2_vs"], r["M2_rho"], 0.0, 40.0, r["stage"]))
+    assert open(log).read() == "".join(lines)
+    vs1, rho1 = ctx.fwi_model()
+    ny, nx = vs1.shape
+    back = np.fromfile(str(tmp_path / "model_vs.bin"), np.float32).reshape(nx, ny).T   # x-major
+    assert np.array_equal(back, vs1)
+    it_file = tmp_path / ("model_vs_it_%d.bin" % r["iter"])
+    assert np.array_equal(np.fromfile(str(it_file), np.float32).reshape(nx, ny).T, vs1)
+    gd = np.fromfile(str(tmp_path / "jac_grad_vs.bin"), np.float32).reshape(nx, ny).T
+    assert np.abs(gd).max() > 0
+    if r["next_stage"]:
+        assert (tmp_path / ("model_vs_stage_%d.bin" % r["stage"])).exists()
+    ctx.close()

@@ -32,6 +32,10 @@ What pins what:
                    gradient unchanged; WD_DAMP = 0 copies.
   taper_grad       closed forms of the vertical/horizontal windows and of the
                    erf bowl around a source in float64; zeroed cells.
+  time_window      exp(-GAMMA (t - pick)^2) in float64 for t >= pick.
+  stf (numpy)      Wiener deconvolution identity: observed = modelled * h
+                   with EPS_STF = 0 returns signal * h; NORMALIZE helpers on
+                   hand-computed traces.
 """
 import ctypes as C
 import os
@@ -784,3 +788,57 @@ def test_taper_grad_source_bowl_closed_form():
     assert np.abs(tap - ms).max() < 1e-6
     assert np.abs(out - ms).max() < 1e-6
     assert tap.max() == 1.0 and out[sj - 1, si - 1] == 0.0
+
+
+# ---------------------------------------------------------------------------
+# data conditioning: time_window.c, stf.c
+# ---------------------------------------------------------------------------
+def test_time_window_closed_form_and_product_host_code():
+    rng = np.random.default_rng(41)
+    ntr, ns, dt, gamma = 5, 300, 1e-3, 2000.0
+    d = rng.standard_normal((ntr, ns)).astype(np.float32)
+    t = (np.arange(1, ns + 1) * np.float32(dt)).astype(np.float64)
+    out2 = O32.time_window(d, dt, 2, gamma)
+    want2 = d * np.exp(-gamma * (t - np.float32(dt)) ** 2)[None, :]
+    assert np.abs(out2 - want2).max() < 1e-6 * np.abs(d).max()
+    assert np.array_equal(out2[:, 0], d[:, 0])        # t = DT: taper 1
+    picks = np.array([0.01, 0.05, 0.1, 0.2, 0.25], np.float32)
+    out1 = O32.time_window(d, dt, 1, gamma, twlength_plus=0.02, picks=picks)
+    for i in range(ntr):
+        p = float(np.float32(picks[i] + np.float32(0.02)))
+        after = t >= p
+        assert np.array_equal(out1[i, ~after], d[i, ~after])
+        assert np.abs(out1[i, after] - d[i, after] * np.exp(-gamma * (t[after] - p) ** 2)).max() < 1e-6 * np.abs(d).max()
+    dn = product()
+    assert np.array_equal(dn.time_window(d, dt, 2, gamma), out2)
+    assert np.array_equal(dn.time_window(d, dt, 1, gamma, 0.02, picks), out1)
+
+
+def test_stf_oracle_recovers_the_convolution_filter():
+    import stf_oracle
+    rng = np.random.default_rng(42)
+    ntr, ns = 7, 200
+    h = np.array([0.0, 0.5, 1.0, -0.3, 0.1])
+    syn = rng.standard_normal((ntr, ns))
+    syn[:, ns - len(h):] = 0.0                       # the convolution fits the record
+    obs = np.stack([np.convolve(syn[i], h)[:ns] for i in range(ntr)])
+    sig = rng.standard_normal(ns)
+    sig[ns - len(h):] = 0.0
+    est = stf_oracle.stf(obs, syn, sig, eps_stf=0.0)
+    want = np.convolve(sig, h)[:ns]
+    assert np.abs(est - want).max() < 1e-9 * np.abs(want).max()
+    # a correlation (conjugate on the wrong factor) would not match
+    wrong = np.correlate(np.concatenate([sig, np.zeros(len(h))]), h, "full")[len(h) - 1:len(h) - 1 + ns]
+    assert np.abs(est - wrong).max() > 1e-2
+    # muted traces drop out: muting half of them keeps the identity
+    mute = np.arange(ntr) % 2 == 0
+    assert np.abs(stf_oracle.stf(obs, syn, sig, mute=mute, eps_stf=0.0) - want).max() < 1e-9 * np.abs(want).max()
+
+
+def test_stf_normalisation_helpers():
+    import stf_oracle
+    x = np.array([[10.0, 1.0, -2.0, 0.5], [3.0, 0.0, 0.0, 0.0]], np.float32)
+    y = stf_oracle.normalize_max(x)
+    assert np.array_equal(y[0], np.array([10.0, 0.5, -1.0, 0.25], np.float32))   # sample 1 excluded
+    assert np.array_equal(y[1], x[1])                                            # zero max -> 1
+    assert np.array_equal(stf_oracle.trace_max(x), np.array([2.0, 1.0], np.float32))
