@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="run only the copy kernel used to scale HBM counters")
     ap.add_argument("--autotune", action="store_true", help="tune rows per segment and graph replay first")
+    ap.add_argument("--wave-shots", type=int, default=1, choices=[1, 2],
+                    help="shots per wavefront on the plain steps (--autotune chooses)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     args = ap.parse_args()
@@ -158,6 +160,7 @@ def main():
                      fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], dtinv=W["dtinv"],
                      snap_precision=prec)
     ctx.set_exec(-1 if args.exec == "graph" else 0)
+    ctx.set_wave_shots(args.wave_shots)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
     ctx.set_shots(W["src_i"][:args.shots], W["src_j"][:args.shots], sig)
@@ -220,6 +223,7 @@ def main():
     if tuned is not None:
         out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
     out["config"]["exec"] = args.exec
+    out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     ctx.close()

@@ -27,7 +27,7 @@ EXPORTS = [
     "dn_unet_create", "dn_unet_destroy", "dn_unet_num_params", "dn_unet_get_params", "dn_unet_set_params",
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
-    "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files",
+    "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
 ]
 
 
@@ -307,6 +307,10 @@ class Context:
 
     def set_exec(self, graph_steps=-1):
         self._chk(lib().dn_set_exec(self._h, graph_steps))
+
+    def set_wave_shots(self, n=2):
+        lib().dn_set_wave_shots.argtypes = [C.c_void_p, C.c_int]
+        self._chk(lib().dn_set_wave_shots(self._h, n))
 
     def autotune(self, nshots, nsteps=40):
         rows, graph = C.c_int(), C.c_int()

@@ -419,6 +419,15 @@ extern "C" int dn_set_exec(dn_ctx *c, int graph_steps)
     return 0;
 }
 
+extern "C" int dn_set_wave_shots(dn_ctx *c, int n)
+{
+    if (!c) return -1;
+    if (n != 1 && n != 2) return dn_fail(c, "dn_set_wave_shots: 1 or 2 shots per wave");
+    c->two_shots = (n == 2);
+    dn_drop_graphs(c);
+    return 0;
+}
+
 extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int *graph_out)
 {
     if (!c) return -1;
@@ -462,6 +471,22 @@ extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int
             best = ms;
             best_rows = Rr;
         }
+    }
+    // one or two shots per wave on the plain steps
+    if (n >= 2) {
+        const bool saved_two = c->two_shots;
+        c->two_shots = !saved_two;
+        double ms_other = 0.0;
+        if (time_it(best_rows, 0, &ms_other)) {
+            c->two_shots = saved_two;
+            c->rows_req = saved_rows;
+            c->graph_steps = saved_graph;
+            return -1;
+        }
+        if (ms_other < best)
+            best = ms_other;
+        else
+            c->two_shots = saved_two;
     }
     double ms_graph = 0.0;
     if (time_it(best_rows, -1, &ms_graph)) {
@@ -587,6 +612,8 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a.nsegs = (g.ny + a.R - 1) / a.R;
     a.nblocks = n * a.ngroups * a.nsegs;
     a.nblocks8 = (a.nblocks + 7) / 8;
+    a.nblocks2 = (n >= 2 && c->two_shots) ? ((n + 1) / 2) * a.ngroups * a.nsegs : 0;
+    a.nblocks2_8 = (a.nblocks2 + 7) / 8;
     a.DT = c->p.dt;
     a.DH = c->p.dh;
     a.dhi = (c->p.L > 0) ? (float)(1.0 / (double)c->p.dh) : c->p.dt / c->p.dh;

@@ -66,7 +66,8 @@ struct StepArgs {
     int nshots_all;                // shots of the context (bound of shot_map entries)
     int ow, nstrips, R, nsegs;     // strip width (owned columns), strips, rows per segment, segments
     int ngroups;                   // workgroups of kStripsPerGroup adjacent strips per segment row
-    int nblocks, nblocks8;         // real workgroups and workgroups per XCD slice
+    int nblocks, nblocks8;         // real workgroups and workgroups per XCD slice (one shot per wave)
+    int nblocks2, nblocks2_8;      // the same with two shots per wave (0: not used)
     float DT, DH, dhi, dthalbe;
     float hc[7];
     float hcy6;                    // coefficient used for the 6th y-term of update_v (Q6: hc[5])

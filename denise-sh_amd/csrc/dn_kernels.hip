@@ -72,18 +72,25 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 // range of (tile, shot) pairs with the shot index fastest: the shots of one
 // tile share the static model fields through that XCD's L2.
 // ---------------------------------------------------------------------------
-template <int K, int L, bool ADJ, bool EX>
+template <int K, int L, bool ADJ, bool EX, int NSH>
 __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs a)
 {
     // EX: this launch may store snapshots / sum the pseudo-Hessian (forward)
     // or correlate (adjoint); the launches without (most steps) carry none
-    // of that code and its pointers
+    // of that code and its pointers.  NSH: shots per wave.  With two, a wave
+    // advances the same strip of two shots of the batch: the static model
+    // fields and the receiver table are loaded once for both, and the two
+    // shots' independent arithmetic hides each other's latency.
+    static_assert(NSH == 1 || (NSH == 2 && !EX), "two shots per wave only on steps without EX work");
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
-    const int lin = (w & 7) * a.nblocks8 + (w >> 3);
-    if (lin >= a.nblocks) return;                 // whole workgroup
-    const int s = lin % a.nshots;
-    const int tile = lin / a.nshots;
+    const int nblk = (NSH == 1) ? a.nblocks : a.nblocks2;
+    const int nblk8 = (NSH == 1) ? a.nblocks8 : a.nblocks2_8;
+    const int lin = (w & 7) * nblk8 + (w >> 3);
+    if (lin >= nblk) return;                      // whole workgroup
+    const int npair = (a.nshots + NSH - 1) / NSH;
+    const int p = lin % npair;
+    const int tile = lin / npair;
     const int grp = tile % a.ngroups;
     const int seg = tile / a.ngroups;
     // the waves of a workgroup take adjacent strips of the same rows and
@@ -94,11 +101,29 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     const bool live = grp * kStripsPerGroup + wv < a.nstrips;
     const int strip = live ? grp * kStripsPerGroup + wv : a.nstrips - 1;
     const Geo &g = a.g;
-    const int S = a.shot_map[s];
-    if (S < 0 || S >= a.nshots_all) return;   // a stale or foreign shot map: touch nothing
-    const int NTs = a.shot_nt[S];
     const int nt = a.nt_base ? (*a.nt_base + a.nt) : a.nt;
-    if (nt > NTs) return;  // shot finished (uneven record lengths)
+
+    // the wave's shots: batch slot, absolute shot, record length, active
+    int sl[NSH], Sx[NSH], NTs[NSH];
+    bool act[NSH];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < NSH; ++u) {
+        sl[u] = p * NSH + u;
+        Sx[u] = 0;
+        NTs[u] = 0;
+        act[u] = false;
+        if (sl[u] < a.nshots) {
+            const int S = a.shot_map[sl[u]];
+            if (S >= 0 && S < a.nshots_all) {   // a stale or foreign shot map: touch nothing
+                Sx[u] = S;
+                NTs[u] = a.shot_nt[S];
+                act[u] = nt <= NTs[u];          // finished shots (uneven record lengths) rest
+            }
+        }
+        any = any || act[u];
+    }
+    if (!any) return;                             // uniform over the workgroup
 
     const int lane = threadIdx.x % kWave;
     const int c0 = 1 + strip * a.ow;          // first owned column
@@ -107,11 +132,39 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     const bool own_x = live && (lane >= K - 1) && (lane < K - 1 + a.ow) && in_x;
     const int j0 = 1 + seg * a.R;
     const int j1 = min(j0 + a.R - 1, g.ny);
-    const long long so = (long long)s * g.plane;
-    const int nxny = g.nx * g.ny;                 // per-lane offsets are 32-bit; bases are per shot
+    const int nxny = g.nx * g.ny;                 // per-lane offsets are 32-bit
     const int plane = (int)g.plane;
     const int fw = g.fw, fw2 = 2 * g.fw;
 
+    // base pointers at the wave's first shot (batch slot sl[0]) and the byte
+    // offset of shot u from it (the shots of a wave are consecutive slots);
+    // per-lane offsets stay 32-bit, as one shot's planes are below 2^31 bytes
+    const long long s0 = sl[0];
+    const float *__restrict__ vyi = a.vy_in + s0 * g.plane;
+    const float *__restrict__ sxi = a.sxy_in + s0 * g.plane;
+    const float *__restrict__ szi = a.syz_in + s0 * g.plane;
+    float *__restrict__ vyo = a.vy_out + s0 * g.plane;
+    float *__restrict__ sxo_ = a.sxy_out + s0 * g.plane;
+    float *__restrict__ szo = a.syz_out + s0 * g.plane;
+    float *__restrict__ rb = VISCO ? a.r + s0 * L * g.plane : nullptr;
+    float *__restrict__ qb = VISCO ? a.q + s0 * L * g.plane : nullptr;
+    const float *__restrict__ pxi = a.psx_in + s0 * g.ny * fw2;
+    float *__restrict__ pxo = a.psx_out + s0 * g.ny * fw2;
+    const float *__restrict__ pyi = a.psy_in + s0 * fw2 * g.nx;
+    float *__restrict__ pyo = a.psy_out + s0 * fw2 * g.nx;
+    float *__restrict__ pvxb = a.pvx + s0 * g.ny * fw2;
+    float *__restrict__ pvyb = a.pvy + s0 * fw2 * g.nx;
+    unsigned sof[NSH], sor[NSH], spx[NSH], spy[NSH];
+#pragma unroll
+    for (int u = 0; u < NSH; ++u) {
+        sof[u] = (unsigned)u * (unsigned)plane * 4u;
+        sor[u] = (unsigned)u * (unsigned)(L > 0 ? L : 1) * (unsigned)plane * 4u;
+        spx[u] = (unsigned)u * (unsigned)(g.ny * fw2) * 4u;
+        spy[u] = (unsigned)u * (unsigned)(fw2 * g.nx) * 4u;
+    }
+
+    // EX work of the single shot (NSH == 1)
+    const int s = sl[0];
     bool fstore = false, corr = false, hsum = false;
     int slot = 0;
     if (!EX) {
@@ -121,29 +174,18 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             slot = (nt - 1) / a.dtinv;
         }
         // hessian.c:91: stored steps 1..NTDTINV-1 of this shot
-        if (a.hess && ((nt - 1) % a.dtinv) == 0 && (nt - 1) / a.dtinv < (NTs + a.dtinv - 1) / a.dtinv - 1)
+        if (a.hess && ((nt - 1) % a.dtinv) == 0 && (nt - 1) / a.dtinv < (NTs[0] + a.dtinv - 1) / a.dtinv - 1)
             hsum = true;
     } else {
-        const int sf = NTs - nt + 1;  // forward step paired with this adjoint step
+        const int sf = NTs[0] - nt + 1;  // forward step paired with this adjoint step
         if (((sf - 1) % a.dtinv) == 0) {
             corr = true;
             slot = (sf - 1) / a.dtinv;
         }
     }
-    const int sb = ADJ ? 0 : a.src_off[S];
-    const int se = ADJ ? 0 : a.src_off[S + 1];
 
-    const float *__restrict__ vy_in = a.vy_in + so;
-    const float *__restrict__ sxy_in = a.sxy_in + so;
-    const float *__restrict__ syz_in = a.syz_in + so;
-    float *__restrict__ vy_out = a.vy_out + so;
-    float *__restrict__ sxy_out = a.sxy_out + so;
-    float *__restrict__ syz_out = a.syz_out + so;
-
-    __shared__ float lsx_all[kStripsPerGroup][kWave + 2 * K];
-    __shared__ float lvy_all[kStripsPerGroup][kWave];
-    float *const lsx = lsx_all[wv];   // per-wave exchange rows
-    float *const lvy = lvy_all[wv];
+    __shared__ float lsx_all[kStripsPerGroup][NSH][kWave + 2 * K];
+    __shared__ float lvy_all[kStripsPerGroup][NSH][kWave];
 
     // The padded planes hold a zero frame of at least 2K-1 rows above and
     // below the interior, 2K-1 columns left and 64 right (dn_internal.h), so
@@ -153,52 +195,51 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // columns left of the wave, lanes K..2K-2 the K-1 columns right of it.
     const int hcol = (lane < K) ? (c0 - 2 * K + 1 + lane) : ((lane < 2 * K - 1) ? (c0 + 65 - 2 * K + lane) : c);
     const int hslot = (lane < K) ? lane : (kWave + lane);
-    float *__restrict__ rbase = VISCO ? a.r + (long long)s * L * g.plane : nullptr;
-    float *__restrict__ qbase = VISCO ? a.q + (long long)s * L * g.plane : nullptr;
     const long long shot_off = (long long)s * nxny;
-    float *__restrict__ grb = a.grad_rho ? a.grad_rho + shot_off : nullptr;
-    float *__restrict__ gub = a.grad_u ? a.grad_u + shot_off : nullptr;
-    float *__restrict__ wab = a.wacc ? a.wacc + shot_off : nullptr;
+    float *__restrict__ grb = (EX && a.grad_rho) ? a.grad_rho + shot_off : nullptr;
+    float *__restrict__ gub = (EX && a.grad_u) ? a.grad_u + shot_off : nullptr;
+    float *__restrict__ wab = (EX && a.wacc) ? a.wacc + shot_off : nullptr;
     // the twelve CPML profile vectors are packed with stride 2*FW+1 in the
     // order K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y,
     // K_y_half, a_y_half, b_y_half (one base keeps the scalar registers free)
     const float *__restrict__ pmlb = a.pml.Kx;
     const int pn = fw2 + 1;
-    const float *__restrict__ psxi = a.psx_in + (long long)s * g.ny * fw2;
-    float *__restrict__ psxo = a.psx_out + (long long)s * g.ny * fw2;
-    const float *__restrict__ psyi = a.psy_in + (long long)s * fw2 * g.nx;
-    float *__restrict__ psyo = a.psy_out + (long long)s * fw2 * g.nx;
-    float *__restrict__ pvxb = a.pvx + (long long)s * g.ny * fw2;
-    float *__restrict__ pvyb = a.pvy + (long long)s * fw2 * g.nx;
 
-    float zw[2 * K];      // old syz, slot t <-> row jv-K+t (after the head load)
-    float vw[2 * K];      // new vy, slot t <-> row jv-2K+1+t (after the push)
-    float sq[K + 1];      // old sxy at the lane's column, slot t <-> row jv-K+t
+    float zw[NSH][2 * K];   // old syz, slot t <-> row jv-K+t (after the head load)
+    float vw[NSH][2 * K];   // new vy, slot t <-> row jv-2K+1+t (after the push)
+    float sq[NSH][K + 1];   // old sxy at the lane's column, slot t <-> row jv-K+t
     const int jstart = j0 - K + 1;
     const int jend = j1 + K;
 #pragma unroll
-    for (int t = 0; t < 2 * K - 1; ++t) zw[t] = syz_in[gi32(g, jstart - K + t, c)];
+    for (int u = 0; u < NSH; ++u) {
 #pragma unroll
-    for (int t = 0; t < 2 * K; ++t) vw[t] = 0.0f;
+        for (int t = 0; t < 2 * K - 1; ++t) zw[u][t] = act[u] ? ldf(szi, sof[u] + gi32(g, jstart - K + t, c) * 4u) : 0.0f;
 #pragma unroll
-    for (int t = 0; t <= K; ++t) sq[t] = 0.0f;
+        for (int t = 0; t < 2 * K; ++t) vw[u][t] = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= K; ++t) sq[u][t] = 0.0f;
+    }
 
     // one row of inputs, fetched one iteration ahead: velocity row jv and
     // the static fields / memory variables of stress row jv - K
-    struct Row {
-        float syz, sc, halo, vy, b, s1, s2;
-        float dp[L > 0 ? L : 1], dd[L > 0 ? L : 1], r[L > 0 ? L : 1], q[L > 0 ? L : 1];
-        int rrow;              // receiver row table entry of row jv (-1: none)
-        float gr, f0;          // velocity row: grad_rho / H_rho sum, stored vyp1 (fp32 snapshots)
-        float wa, gu, fy, fu;  // stress row: Ws / Wr, grad_u / H_mu sum, stored uy, uyx (fp32)
-        uint16_t h0, hy, hu;   // the stored fields of 16-bit snapshots (decoded where used)
+    struct RowM {               // shared by the wave's shots
+        float b, s1, s2;
+        float dp[L > 0 ? L : 1], dd[L > 0 ? L : 1];
+        int rrow;               // receiver row table entry of row jv (-1: none)
+    };
+    struct RowS {               // per shot
+        float syz, sc, halo, vy;
+        float r[L > 0 ? L : 1], q[L > 0 ? L : 1];
+        float gr, f0;           // velocity row: grad_rho / H_rho sum, stored vyp1 (fp32 snapshots)
+        float wa, gu, fy, fu;   // stress row: Ws / Wr, grad_u / H_mu sum, stored uy, uyx (fp32)
+        uint16_t h0, hy, hu;    // the stored fields of 16-bit snapshots (decoded where used)
     };
     // per-shot snapshot slot of this step and its decode scales
     const long long sbase = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny;
-    const float *__restrict__ snb = a.snap ? a.snap + sbase : nullptr;
-    float *__restrict__ snw = a.snap ? a.snap + sbase : nullptr;
-    const uint16_t *__restrict__ snb16 = a.snap16 ? a.snap16 + sbase : nullptr;
-    float *__restrict__ stg = a.stage ? a.stage + (long long)s * 3 * nxny : nullptr;
+    const float *__restrict__ snb = (EX && a.snap) ? a.snap + sbase : nullptr;
+    float *__restrict__ snw = (EX && a.snap) ? a.snap + sbase : nullptr;
+    const uint16_t *__restrict__ snb16 = (EX && a.snap16) ? a.snap16 + sbase : nullptr;
+    float *__restrict__ stg = (EX && a.stage) ? a.stage + (long long)s * 3 * nxny : nullptr;
     float dsc[3] = {1.0f, 1.0f, 1.0f};
     if (ADJ && corr && a.snap_kind != 0) {
         const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
@@ -209,283 +250,328 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     auto decode = [&](uint16_t h, float scale) -> float {
         return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
     };
-    auto fetch = [&](int jv, Row &o) {
+    auto fetch = [&](int jv, RowM &m, RowS (&o)[NSH]) {
         const unsigned gv = gi32(g, jv, c) * 4u;
-        o.syz = ldf(syz_in, gi32(g, jv + K - 1, c) * 4u);
-        o.sc = ldf(sxy_in, gv);
-        o.halo = ldf(sxy_in, gi32(g, jv, hcol) * 4u);
-        o.vy = ldf(vy_in, gv);
-        o.b = ldf(a.bfield, gv);
-        o.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
-        const bool vin = (jv >= j0 && jv <= j1) && own_x;
-        const unsigned cv = (unsigned)((jv - 1) * g.nx + (c - 1));
-        if ((ADJ && corr) || (!ADJ && hsum)) {
-            o.gr = vin ? grb[cv] : 0.0f;
-            if (ADJ && vin) {
-                if (a.snap_kind == 0)
-                    o.f0 = snb[cv];
-                else
-                    o.h0 = snb16[cv];
-            }
-        }
-        const int jsn = jv - K;
-        const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
-        const unsigned cs = (unsigned)((jsn - 1) * g.nx + (c - 1));
-        if ((ADJ && corr) || (!ADJ && fstore)) o.wa = sin_ ? wab[cs] : 0.0f;
-        if ((ADJ && corr) || (!ADJ && hsum)) o.gu = sin_ ? gub[cs] : 0.0f;
-        if (ADJ && corr && sin_) {
-            if (a.snap_kind == 0) {
-                o.fy = snb[nxny + cs];
-                o.fu = snb[2 * nxny + cs];
-            } else {
-                o.hy = snb16[nxny + cs];
-                o.hu = snb16[2 * nxny + cs];
-            }
-        }
         const unsigned gs = gi32(g, jv - K, c) * 4u;
+        m.b = ldf(a.bfield, gv);
+        m.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
         if (!VISCO) {
-            o.s1 = ldf(a.uip, gs);
-            o.s2 = ldf(a.ujp, gs);
+            m.s1 = ldf(a.uip, gs);
+            m.s2 = ldf(a.ujp, gs);
         } else {
-            o.s1 = ldf(a.fipjp, gs);
-            o.s2 = ldf(a.fcoef, gs);
+            m.s1 = ldf(a.fipjp, gs);
+            m.s2 = ldf(a.fcoef, gs);
 #pragma unroll
             for (int l = 0; l < L; ++l) {
                 const unsigned o4 = gs + (unsigned)(l * plane) * 4u;
-                o.dp[l] = ldf(a.dip, o4);
-                o.dd[l] = ldf(a.dcoef, o4);
-                o.r[l] = ldf(rbase, o4);
-                o.q[l] = ldf(qbase, o4);
+                m.dp[l] = ldf(a.dip, o4);
+                m.dd[l] = ldf(a.dcoef, o4);
+            }
+        }
+        const unsigned gz = gi32(g, jv + K - 1, c) * 4u, gh = gi32(g, jv, hcol) * 4u;
+#pragma unroll
+        for (int u = 0; u < NSH; ++u) {
+            if (!act[u]) continue;
+            o[u].syz = ldf(szi, sof[u] + gz);
+            o[u].sc = ldf(sxi, sof[u] + gv);
+            o[u].halo = ldf(sxi, sof[u] + gh);
+            o[u].vy = ldf(vyi, sof[u] + gv);
+            if (VISCO) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    const unsigned o4 = sor[u] + gs + (unsigned)(l * plane) * 4u;
+                    o[u].r[l] = ldf(rb, o4);
+                    o[u].q[l] = ldf(qb, o4);
+                }
+            }
+        }
+        if (EX) {
+            RowS &e = o[0];
+            const bool vin = (jv >= j0 && jv <= j1) && own_x;
+            const unsigned cv = (unsigned)((jv - 1) * g.nx + (c - 1));
+            if ((ADJ && corr) || (!ADJ && hsum)) {
+                e.gr = vin ? grb[cv] : 0.0f;
+                if (ADJ && vin) {
+                    if (a.snap_kind == 0)
+                        e.f0 = snb[cv];
+                    else
+                        e.h0 = snb16[cv];
+                }
+            }
+            const int jsn = jv - K;
+            const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
+            const unsigned cs = (unsigned)((jsn - 1) * g.nx + (c - 1));
+            if ((ADJ && corr) || (!ADJ && fstore)) e.wa = sin_ ? wab[cs] : 0.0f;
+            if ((ADJ && corr) || (!ADJ && hsum)) e.gu = sin_ ? gub[cs] : 0.0f;
+            if (ADJ && corr && sin_) {
+                if (a.snap_kind == 0) {
+                    e.fy = snb[nxny + cs];
+                    e.fu = snb[2 * nxny + cs];
+                } else {
+                    e.hy = snb16[nxny + cs];
+                    e.hu = snb16[2 * nxny + cs];
+                }
             }
         }
     };
-    // does a source of this shot fall into the wave's columns and rows?
-    bool has_src = false;
-    if (!ADJ)
-        for (int l = sb; l < se; ++l) {
-            const int si = a.src_i[l], sj = a.src_j[l];
-            if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src = true;
-        }
-    Row nx_;
-    fetch(jstart, nx_);
+    // does a source of the shot fall into the wave's columns and rows?
+    bool has_src[NSH];
+    int sb[NSH], se[NSH];
+#pragma unroll
+    for (int u = 0; u < NSH; ++u) {
+        has_src[u] = false;
+        sb[u] = (ADJ || !act[u]) ? 0 : a.src_off[Sx[u]];
+        se[u] = (ADJ || !act[u]) ? 0 : a.src_off[Sx[u] + 1];
+        if (!ADJ)
+            for (int l = sb[u]; l < se[u]; ++l) {
+                const int si = a.src_i[l], sj = a.src_j[l];
+                if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src[u] = true;
+            }
+    }
+    RowM nm_;
+    RowS ns_[NSH];
+    fetch(jstart, nm_, ns_);
 
     for (int jv = jstart; jv <= jend; ++jv) {
-        const Row cr = nx_;
-        if (jv < jend) fetch(jv + 1, nx_);
-        zw[2 * K - 1] = cr.syz;
-        const float sc = cr.sc;
-        lsx[K + lane] = sc;
-        if (lane < 2 * K - 1) lsx[hslot] = cr.halo;
+        const RowM cm = nm_;
+        RowS cr[NSH];
+#pragma unroll
+        for (int u = 0; u < NSH; ++u) cr[u] = ns_[u];
+        if (jv < jend) fetch(jv + 1, nm_, ns_);
+        float vnew[NSH], sc[NSH];
+#pragma unroll
+        for (int u = 0; u < NSH; ++u) {
+            zw[u][2 * K - 1] = cr[u].syz;
+            sc[u] = cr[u].sc;
+            lsx_all[wv][u][K + lane] = sc[u];
+            if (lane < 2 * K - 1) lsx_all[wv][u][hslot] = cr[u].halo;
+        }
         __builtin_amdgcn_wave_barrier();
 
-        float vnew = 0.0f;
-        if (jv >= 1 && jv <= g.ny && in_x) {
+#pragma unroll
+        for (int u = 0; u < NSH; ++u) {
+            vnew[u] = 0.0f;
+            if (!act[u] || !(jv >= 1 && jv <= g.ny && in_x)) continue;
+            const float *lsx = lsx_all[wv][u];
             const bool own_v = own_x && jv >= j0 && jv <= j1;
             float sxy_x = a.hc[1] * (lsx[K + lane] - lsx[K + lane - 1]);
-            float syy_y = a.hc[1] * (zw[K] - zw[K - 1]);
+            float syy_y = a.hc[1] * (zw[u][K] - zw[u][K - 1]);
 #pragma unroll
             for (int m = 2; m <= K; ++m) {
                 sxy_x = sxy_x + a.hc[m] * (lsx[K + lane + m - 1] - lsx[K + lane - m]);
                 const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
-                syy_y = syy_y + hy * (zw[K + m - 1] - zw[K - m]);
+                syy_y = syy_y + hy * (zw[u][K + m - 1] - zw[u][K - m]);
             }
             if (fw > 0) {
                 if (c <= fw) {
-                    const int pi = (jv - 1) * fw2 + (c - 1);
-                    const float p = pmlb[2 * pn + c] * psxi[pi] + pmlb[1 * pn + c] * sxy_x;
-                    if (own_v) psxo[pi] = p;
+                    const unsigned pi4 = spx[u] + (unsigned)((jv - 1) * fw2 + (c - 1)) * 4u;
+                    const float p = pmlb[2 * pn + c] * ldf(pxi, pi4) + pmlb[1 * pn + c] * sxy_x;
+                    if (own_v) stf(pxo, pi4, p);
                     sxy_x = sxy_x / pmlb[0 * pn + c] + p;
                 }
                 if (c >= g.nx - fw + 1) {
                     const int h1 = c - g.nx + fw2;
-                    const int pi = (jv - 1) * fw2 + (h1 - 1);
-                    const float p = pmlb[2 * pn + h1] * psxi[pi] + pmlb[1 * pn + h1] * sxy_x;
-                    if (own_v) psxo[pi] = p;
+                    const unsigned pi4 = spx[u] + (unsigned)((jv - 1) * fw2 + (h1 - 1)) * 4u;
+                    const float p = pmlb[2 * pn + h1] * ldf(pxi, pi4) + pmlb[1 * pn + h1] * sxy_x;
+                    if (own_v) stf(pxo, pi4, p);
                     sxy_x = sxy_x / pmlb[0 * pn + h1] + p;
                 }
                 if (!g.free_surf && jv <= fw) {
-                    const int pi = (jv - 1) * g.nx + (c - 1);
-                    const float p = pmlb[8 * pn + jv] * psyi[pi] + pmlb[7 * pn + jv] * syy_y;
-                    if (own_v) psyo[pi] = p;
+                    const unsigned pi4 = spy[u] + (unsigned)((jv - 1) * g.nx + (c - 1)) * 4u;
+                    const float p = pmlb[8 * pn + jv] * ldf(pyi, pi4) + pmlb[7 * pn + jv] * syy_y;
+                    if (own_v) stf(pyo, pi4, p);
                     syy_y = syy_y / pmlb[6 * pn + jv] + p;
                 }
                 if (jv >= g.ny - fw + 1) {
                     const int h1 = jv - g.ny + fw2;
-                    const int pi = (h1 - 1) * g.nx + (c - 1);
-                    const float p = pmlb[8 * pn + h1] * psyi[pi] + pmlb[7 * pn + h1] * syy_y;
-                    if (own_v) psyo[pi] = p;
+                    const unsigned pi4 = spy[u] + (unsigned)((h1 - 1) * g.nx + (c - 1)) * 4u;
+                    const float p = pmlb[8 * pn + h1] * ldf(pyi, pi4) + pmlb[7 * pn + h1] * syy_y;
+                    if (own_v) stf(pyo, pi4, p);
                     syy_y = syy_y / pmlb[6 * pn + h1] + p;
                 }
             }
             const unsigned gi = gi32(g, jv, c);
-            const float vold = cr.vy;
-            const float bb = cr.b;
+            const float vold = cr[u].vy;
+            const float bb = cm.b;
             const float tsum = sxy_x + syy_y;
             const unsigned ci = (unsigned)((jv - 1) * g.nx + (c - 1));
-            if (!ADJ && hsum && own_v) {
+            if (EX && !ADJ && hsum && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
-                grb[ci] = cr.gr + (acc * acc);
+                grb[ci] = cr[u].gr + (acc * acc);
             }
-            if (!ADJ && fstore && own_v) {
+            if (EX && !ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
                 if (a.store_mode == 1)
                     snw[ci] = acc;
                 else
                     stg[ci] = acc;
             }
-            if (ADJ && corr && own_v) {
+            if (EX && ADJ && corr && own_v) {
                 // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
-                const float f0 = (a.snap_kind == 0) ? cr.f0 : decode(cr.h0, dsc[0]);
-                grb[ci] = cr.gr + (vold * f0);
+                const float f0 = (a.snap_kind == 0) ? cr[u].f0 : decode(cr[u].h0, dsc[0]);
+                grb[ci] = cr[u].gr + (vold * f0);
             }
-            vnew = vold + ((bb * a.DT) * tsum) / a.DH;
+            float vn = vold + ((bb * a.DT) * tsum) / a.DH;
             if (!ADJ) {
-                if (has_src)
-                    for (int l = sb; l < se; ++l)
+                if (has_src[u])
+                    for (int l = sb[u]; l < se[u]; ++l)
                         if (a.src_j[l] == jv && a.src_i[l] == c)
-                            vnew = vnew + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
+                            vn = vn + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
             } else {
-                const int off = cr.rrow;
+                const int off = cm.rrow;
                 if (off >= 0) {
                     const int r = a.rec_tab[off + c];
-                    if (r >= 0) vnew = vnew + a.resid[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)];
+                    if (r >= 0) vn = vn + a.resid[((long long)sl[u] * a.ntr + r) * a.nt_max + (nt - 1)];
                 }
             }
+            vnew[u] = vn;
             if (own_v) {
-                stf(vy_out, gi * 4u, vnew);
+                stf(vyo, sof[u] + gi * 4u, vn);
                 if (!ADJ && a.seis != nullptr) {
-                    const int off = cr.rrow;
+                    const int off = cm.rrow;
                     if (off >= 0) {
                         const int r = a.rec_tab[off + c];
-                        if (r >= 0) a.seis[((long long)s * a.ntr + r) * a.nt_max + (nt - 1)] = vnew;
+                        if (r >= 0) a.seis[((long long)sl[u] * a.ntr + r) * a.nt_max + (nt - 1)] = vn;
                     }
                 }
             }
         }
 
 #pragma unroll
-        for (int t = 0; t < 2 * K - 1; ++t) vw[t] = vw[t + 1];
-        vw[2 * K - 1] = vnew;
+        for (int u = 0; u < NSH; ++u) {
 #pragma unroll
-        for (int t = 0; t < K; ++t) sq[t] = sq[t + 1];
-        sq[K] = sc;
+            for (int t = 0; t < 2 * K - 1; ++t) vw[u][t] = vw[u][t + 1];
+            vw[u][2 * K - 1] = vnew[u];
+#pragma unroll
+            for (int t = 0; t < K; ++t) sq[u][t] = sq[u][t + 1];
+            sq[u][K] = sc[u];
+        }
 
         const int js = jv - K;  // stress row of this iteration
         if (js >= j0) {
-            lvy[lane] = vw[K - 1];
+#pragma unroll
+            for (int u = 0; u < NSH; ++u) lvy_all[wv][u][lane] = vw[u][K - 1];
             __builtin_amdgcn_wave_barrier();
-            if (own_x) {
+#pragma unroll
+            for (int u = 0; u < NSH; ++u) {
+                if (!act[u] || !own_x) continue;
+                const float *lvy = lvy_all[wv][u];
                 float vyx = a.hc[1] * (lvy[lane + 1] - lvy[lane]);
-                float vyy = a.hc[1] * (vw[K] - vw[K - 1]);
+                float vyy = a.hc[1] * (vw[u][K] - vw[u][K - 1]);
 #pragma unroll
                 for (int m = 2; m <= K; ++m) {
                     vyx = vyx + a.hc[m] * (lvy[lane + m] - lvy[lane - m + 1]);
-                    vyy = vyy + a.hc[m] * (vw[K - 1 + m] - vw[K - m]);
+                    vyy = vyy + a.hc[m] * (vw[u][K - 1 + m] - vw[u][K - m]);
                 }
                 vyx = vyx * a.dhi;
                 vyy = vyy * a.dhi;
                 if (fw > 0) {
                     if (c <= fw) {
-                        const int pi = (js - 1) * fw2 + (c - 1);
-                        const float p = pmlb[5 * pn + c] * pvxb[pi] + pmlb[4 * pn + c] * vyx;
-                        pvxb[pi] = p;
+                        const unsigned pi4 = spx[u] + (unsigned)((js - 1) * fw2 + (c - 1)) * 4u;
+                        const float p = pmlb[5 * pn + c] * ldf(pvxb, pi4) + pmlb[4 * pn + c] * vyx;
+                        stf(pvxb, pi4, p);
                         vyx = vyx / pmlb[3 * pn + c] + p;
                     }
                     if (c >= g.nx - fw + 1) {
                         const int h1 = c - g.nx + fw2;
-                        const int pi = (js - 1) * fw2 + (h1 - 1);
-                        const float p = pmlb[5 * pn + h1] * pvxb[pi] + pmlb[4 * pn + h1] * vyx;
-                        pvxb[pi] = p;
+                        const unsigned pi4 = spx[u] + (unsigned)((js - 1) * fw2 + (h1 - 1)) * 4u;
+                        const float p = pmlb[5 * pn + h1] * ldf(pvxb, pi4) + pmlb[4 * pn + h1] * vyx;
+                        stf(pvxb, pi4, p);
                         vyx = vyx / pmlb[3 * pn + h1] + p;
                     }
                     if (!g.free_surf && js <= fw) {
                         // Q2: the viscoelastic kernel uses the full-grid y profile on top
-                        const int pi = (js - 1) * g.nx + (c - 1);
+                        const unsigned pi4 = spy[u] + (unsigned)((js - 1) * g.nx + (c - 1)) * 4u;
                         const float bY = VISCO ? pmlb[8 * pn + js] : pmlb[11 * pn + js];
                         const float aY = VISCO ? pmlb[7 * pn + js] : pmlb[10 * pn + js];
                         const float KY = VISCO ? pmlb[6 * pn + js] : pmlb[9 * pn + js];
-                        const float p = bY * pvyb[pi] + aY * vyy;
-                        pvyb[pi] = p;
+                        const float p = bY * ldf(pvyb, pi4) + aY * vyy;
+                        stf(pvyb, pi4, p);
                         vyy = vyy / KY + p;
                     }
                     if (js >= g.ny - fw + 1) {
                         const int h1 = js - g.ny + fw2;
-                        const int pi = (h1 - 1) * g.nx + (c - 1);
-                        const float p = pmlb[11 * pn + h1] * pvyb[pi] + pmlb[10 * pn + h1] * vyy;
-                        pvyb[pi] = p;
+                        const unsigned pi4 = spy[u] + (unsigned)((h1 - 1) * g.nx + (c - 1)) * 4u;
+                        const float p = pmlb[11 * pn + h1] * ldf(pvyb, pi4) + pmlb[10 * pn + h1] * vyy;
+                        stf(pvyb, pi4, p);
                         vyy = vyy / pmlb[9 * pn + h1] + p;
                     }
                 }
                 const unsigned gs = gi32(g, js, c);
-                const float sxo = sq[0];
-                const float syo = zw[0];
+                const float sxo = sq[u][0];
+                const float syo = zw[u][0];
                 float sxn, syn;
                 if (!VISCO) {
-                    sxn = sxo + cr.s1 * vyx;
-                    syn = syo + cr.s2 * vyy;
+                    sxn = sxo + cm.s1 * vyx;
+                    syn = syo + cm.s2 * vyy;
                 } else {
                     float rr[L > 0 ? L : 1], qq[L > 0 ? L : 1];
                     float sumr = 0.0f, sumq = 0.0f;
-                    const unsigned gs4 = gs * 4u;
+                    const unsigned gs4 = sor[u] + gs * 4u;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
-                        rr[l] = cr.r[l];
-                        qq[l] = cr.q[l];
+                        rr[l] = cr[u].r[l];
+                        qq[l] = cr[u].q[l];
                         sumr += rr[l];
                         sumq += qq[l];
                     }
-                    sxn = sxo + ((cr.s1 * vyx) + (a.dthalbe * sumr));
-                    syn = syo + ((cr.s2 * vyy) + (a.dthalbe * sumq));
+                    sxn = sxo + ((cm.s1 * vyx) + (a.dthalbe * sumr));
+                    syn = syo + ((cm.s2 * vyy) + (a.dthalbe * sumq));
                     sumr = 0.0f;
                     sumq = 0.0f;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
-                        rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cr.dp[l] * vyx));
-                        qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cr.dd[l] * vyy));
-                        stf(rbase, gs4 + (unsigned)(l * plane) * 4u, rr[l]);
-                        stf(qbase, gs4 + (unsigned)(l * plane) * 4u, qq[l]);
+                        rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cm.dp[l] * vyx));
+                        qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cm.dd[l] * vyy));
+                        stf(rb, gs4 + (unsigned)(l * plane) * 4u, rr[l]);
+                        stf(qb, gs4 + (unsigned)(l * plane) * 4u, qq[l]);
                         sumr += rr[l];
                         sumq += qq[l];
                     }
                     sxn = sxn + (a.dthalbe * sumr);
                     syn = syn + (a.dthalbe * sumq);
                 }
-                stf(sxy_out, gs * 4u, sxn);
-                stf(syz_out, gs * 4u, syn);
-                const unsigned ci = (unsigned)((js - 1) * g.nx + (c - 1));
-                const float vc = vw[K - 1];
-                if (!ADJ && fstore) {
-                    if (a.store_mode == 1) {
-                        float *b0 = snw + ci;
-                        b0[nxny] = vyy;        // uy  (forward_prop_y)
-                        b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
-                    } else {
-                        float *b0 = stg + ci;
-                        b0[nxny] = vyy;
-                        b0[2 * nxny] = vyx;
+                stf(sxo_, sof[u] + gs * 4u, sxn);
+                stf(szo, sof[u] + gs * 4u, syn);
+                if (EX) {
+                    const unsigned ci = (unsigned)((js - 1) * g.nx + (c - 1));
+                    const float vc = vw[u][K - 1];
+                    if (!ADJ && fstore) {
+                        if (a.store_mode == 1) {
+                            float *b0 = snw + ci;
+                            b0[nxny] = vyy;        // uy  (forward_prop_y)
+                            b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
+                        } else {
+                            float *b0 = stg + ci;
+                            b0[nxny] = vyy;
+                            b0[2 * nxny] = vyx;
+                        }
+                        wab[ci] = cr[u].wa + vc * vc;   // eprecond.c:11
                     }
-                    wab[ci] = cr.wa + vc * vc;   // eprecond.c:11
-                }
-                if (!ADJ && hsum) {
-                    const float eyx = 0.5f * vyx;     // hess_save.c:13 (exact in float)
-                    gub[ci] = cr.gu + ((vyy * vyy) + (eyx * eyx));
-                }
-                if (ADJ && corr) {
-                    const float fy = (a.snap_kind == 0) ? cr.fy : decode(cr.hy, dsc[1]);
-                    const float fu = (a.snap_kind == 0) ? cr.fu : decode(cr.hu, dsc[2]);
-                    gub[ci] = cr.gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
-                    wab[ci] = cr.wa + vc * vc;
+                    if (!ADJ && hsum) {
+                        const float eyx = 0.5f * vyx;     // hess_save.c:13 (exact in float)
+                        gub[ci] = cr[u].gu + ((vyy * vyy) + (eyx * eyx));
+                    }
+                    if (ADJ && corr) {
+                        const float fy = (a.snap_kind == 0) ? cr[u].fy : decode(cr[u].hy, dsc[1]);
+                        const float fu = (a.snap_kind == 0) ? cr[u].fu : decode(cr[u].hu, dsc[2]);
+                        gub[ci] = cr[u].gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
+                        wab[ci] = cr[u].wa + vc * vc;
+                    }
                 }
             }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_s_barrier();   // keep the workgroup's strips on the same row
 #pragma unroll
-        for (int t = 0; t < 2 * K - 1; ++t) zw[t] = zw[t + 1];
+        for (int u = 0; u < NSH; ++u) {
+#pragma unroll
+            for (int t = 0; t < 2 * K - 1; ++t) zw[u][t] = zw[u][t + 1];
+        }
     }
 
     // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
-    if (!ADJ && fstore && a.store_mode == 2 && live) {
+    if (EX && !ADJ && fstore && a.store_mode == 2 && live) {
         // recomputed from the staged values this wave wrote (owned cells only)
         float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
         if (own_x) {
@@ -511,17 +597,23 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
 template <int K, int L>
 static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hipStream_t st)
 {
-    dim3 grid((unsigned)(a.nblocks8 * 8)), block(kWave * kStripsPerGroup);
+    // plain steps of batches with two or more shots take two shots per wave
+    const bool two = !extra && a.nshots >= 2 && a.nblocks2 > 0;
+    dim3 grid((unsigned)((two ? a.nblocks2_8 : a.nblocks8) * 8)), block(kWave * kStripsPerGroup);
     if (adjoint) {
         if (extra)
-            hipLaunchKernelGGL((k_step<K, L, true, true>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, true, true, 1>), grid, block, 0, st, a);
+        else if (two)
+            hipLaunchKernelGGL((k_step<K, L, true, false, 2>), grid, block, 0, st, a);
         else
-            hipLaunchKernelGGL((k_step<K, L, true, false>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, true, false, 1>), grid, block, 0, st, a);
     } else {
         if (extra)
-            hipLaunchKernelGGL((k_step<K, L, false, true>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, false, true, 1>), grid, block, 0, st, a);
+        else if (two)
+            hipLaunchKernelGGL((k_step<K, L, false, false, 2>), grid, block, 0, st, a);
         else
-            hipLaunchKernelGGL((k_step<K, L, false, false>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, false, false, 1>), grid, block, 0, st, a);
     }
     return hipGetLastError();
 }

@@ -216,10 +216,16 @@ int dn_pseudo_hessian(dn_ctx *ctx, int shot0, int nshots, float *h_vs, float *h_
  * > 0 captures chunks of about that many steps once per batch shape
  * (hipGraph) and replays them; -1 (default) uses 64. */
 int dn_set_exec(dn_ctx *ctx, int graph_steps);
+/* Shots per wavefront on the steps without snapshot / correlation work:
+ * 1 (default) one shot per wave; 2 advances two shots of a batch in one
+ * wave, sharing the loads of the static model fields (more registers per
+ * wave).  Results are identical; dn_autotune picks the faster. */
+int dn_set_wave_shots(dn_ctx *ctx, int shots_per_wave);
 
 /* Tile-shape autotuner: times nsteps forward steps of the first nshots
- * shots for each candidate rows-per-segment, with and without graph
- * replay, keeps the fastest setting and returns it. */
+ * shots for each candidate rows-per-segment, then one or two shots per
+ * wave and graph replay, keeps the fastest setting and returns rows and
+ * graph use. */
 int dn_autotune(dn_ctx *ctx, int nshots, int nsteps, int *rows_per_segment, int *graph_steps);
 
 /* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis

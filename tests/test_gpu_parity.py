@@ -326,3 +326,26 @@ def test_gradient_after_forward_on_same_shots():
     g2 = fresh.get_gradient()
     for a, b in zip(g1, g2):
         assert np.array_equal(a, b)
+
+
+def test_two_shots_per_wave_equals_one():
+    """dn_set_wave_shots(2) (the default) and (1) give identical seismograms
+    and gradients, with an odd shot count and uneven record lengths."""
+    nx, ny, nt = 110, 48, 300
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, nts=[300, 260, 290])
+    obs = _obs_for(ctx, cfg, d, nx, ny, 3)
+    for s, n in enumerate([300, 260, 290]):
+        obs[s, :, n:] = 0.0
+    ctx.set_observed(obs)
+    out = {}
+    for nsh in (1, 2):
+        ctx.set_wave_shots(nsh)
+        seis = ctx.forward()
+        l2, en = ctx.gradient()
+        gvs, grho, _, _ = ctx.get_gradient()
+        out[nsh] = (seis, gvs, grho, l2, en)
+    a, b = out[1], out[2]
+    say("  seismograms equal %s, gradients equal %s" % (np.array_equal(a[0], b[0]), np.array_equal(a[1], b[1])))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3] and a[4] == b[4]
+    ctx.close()
