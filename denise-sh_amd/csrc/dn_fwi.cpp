@@ -582,7 +582,7 @@ extern "C" int dn_precond_features(dn_ctx *c, float *x)
 extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
 {
     if (!c) return -1;
-    if (judge_every < 0) return dn_fail(c, "dn_fwi_set_learned: judge_every must be >= 0");
+    if (judge_every < -1000) return dn_fail(c, "dn_fwi_set_learned: judge_every must be >= -1000");
     c->unet = net;
     c->judge_every = judge_every;
     return 0;
@@ -694,7 +694,10 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
         }
     }
     if (c->precond_hook && c->precond_hook(c, f->gu, f->grho)) return -1;
-    const bool learned = (c->unet != nullptr) && (f->o.grad_method == 2);
+    // judge_every < 0: the learned direction only on the first -judge_every
+    // iterations of a stage, plain L-BFGS (with the same history) after
+    const bool learned = (c->unet != nullptr) && (f->o.grad_method == 2) &&
+                         !(c->judge_every < 0 && f->iter > -c->judge_every);
     if (learned) {
         if (!f->r2) {
             HIPCHK(c, hipMalloc((void **)&f->r2, 2 * n * sizeof(float)));

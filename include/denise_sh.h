@@ -465,7 +465,9 @@ int dn_fwi_write_files(dn_ctx *ctx, const dn_fwi_record *rec, const dn_fwi_files
  * pair; the plain direction (the reference's) judges it on the first
  * iteration of a stage and every judge_every-th (0: only the first), the
  * larger misfit drop of the two being taken, and replaces it whenever its
- * line search finds no decrease.
+ * line search finds no decrease.  judge_every = -k uses the learned
+ * direction on the first k iterations of a stage only (judged on the
+ * first) and the plain L-BFGS direction, with the same history, after.
  * The network stays owned by the caller.  Not in the reference. */
 int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
 /* The network's input maps from the last gradient and the current model,
