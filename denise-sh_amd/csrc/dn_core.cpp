@@ -75,6 +75,12 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
         dn_fail(nullptr, "dn_create: FW=%d larger than the grid", p->fw);
         return nullptr;
     }
+    // the fused time step takes the absorbing frames on opposite sides to be
+    // disjoint (the reference would apply both to a cell where they overlap)
+    if (p->fw > 0 && (p->nx < 2 * p->fw + 1 || (!p->free_surf && p->ny < 2 * p->fw + 1))) {
+        dn_fail(nullptr, "dn_create: the CPML frames of width FW=%d overlap on a %dx%d grid", p->fw, p->nx, p->ny);
+        return nullptr;
+    }
     if (p->dtinv < 1) {
         dn_fail(nullptr, "dn_create: DTINV must be >= 1");
         return nullptr;

@@ -204,6 +204,19 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // K_y_half, a_y_half, b_y_half (one base keeps the scalar registers free)
     const float *__restrict__ pmlb = a.pml.Kx;
     const int pn = fw2 + 1;
+    // CPML: the host guarantees that the left and right (top and bottom)
+    // absorbing regions do not overlap, so a lane (row) lies in at most one.
+    // x: the lane's profile index (1..2FW, 0 outside) and its six profile
+    // values, constant over the rows; y: the profile index of a row
+    const int xk = (fw > 0 && in_x) ? ((c <= fw) ? c : ((c >= g.nx - fw + 1) ? c - g.nx + fw2 : 0)) : 0;
+    const bool xin = xk > 0;
+    const float vKx = pmlb[0 * pn + xk], vax = pmlb[1 * pn + xk], vbx = pmlb[2 * pn + xk];
+    const float sKx = pmlb[3 * pn + xk], sax = pmlb[4 * pn + xk], sbx = pmlb[5 * pn + xk];
+    auto ykof = [&](int r) -> int {
+        if (fw <= 0 || r < 1 || r > g.ny) return 0;
+        if (!g.free_surf && r <= fw) return r;
+        return (r >= g.ny - fw + 1) ? r - g.ny + fw2 : 0;
+    };
 
     float zw[NSH][2 * K];   // old syz, slot t <-> row jv-K+t (after the head load)
     float vw[NSH][2 * K];   // new vy, slot t <-> row jv-2K+1+t (after the push)
@@ -226,9 +239,13 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         float b, s1, s2;
         float dp[L > 0 ? L : 1], dd[L > 0 ? L : 1];
         int rrow;               // receiver row table entry of row jv (-1: none)
+        float vKy, vay, vby;    // y-CPML profile of velocity row jv (K, a, b)
+        float sKy, say, sby;    // y-CPML profile of stress row jv - K
     };
     struct RowS {               // per shot
         float syz, sc, halo, vy;
+        float px, py;           // CPML memory of velocity row jv: psi_sxy_x, psi_syy_y
+        float qx, qy;           // CPML memory of stress row jv - K: psi_vyx, psi_vyy
         float r[L > 0 ? L : 1], q[L > 0 ? L : 1];
         float gr, f0;           // velocity row: grad_rho / H_rho sum, stored vyp1 (fp32 snapshots)
         float wa, gu, fy, fu;   // stress row: Ws / Wr, grad_u / H_mu sum, stored uy, uyx (fp32)
@@ -268,6 +285,20 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 m.dd[l] = ldf(a.dcoef, o4);
             }
         }
+        const int ykv = ykof(jv), jsr = jv - K, yks = ykof(jsr);
+        if (ykv > 0) {
+            m.vKy = pmlb[6 * pn + ykv];
+            m.vay = pmlb[7 * pn + ykv];
+            m.vby = pmlb[8 * pn + ykv];
+        }
+        if (yks > 0) {
+            // Q2: the viscoelastic kernel uses the full-grid y profile on top
+            const int b0 = (VISCO && !g.free_surf && jsr <= fw) ? 6 : 9;
+            m.sKy = pmlb[b0 * pn + yks];
+            m.say = pmlb[(b0 + 1) * pn + yks];
+            m.sby = pmlb[(b0 + 2) * pn + yks];
+        }
+        const bool vrow = jv >= 1 && jv <= g.ny, srow = jsr >= 1 && jsr <= g.ny;
         const unsigned gz = gi32(g, jv + K - 1, c) * 4u, gh = gi32(g, jv, hcol) * 4u;
 #pragma unroll
         for (int u = 0; u < NSH; ++u) {
@@ -276,6 +307,10 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             o[u].sc = ldf(sxi, sof[u] + gv);
             o[u].halo = ldf(sxi, sof[u] + gh);
             o[u].vy = ldf(vyi, sof[u] + gv);
+            if (xin && vrow) o[u].px = ldf(pxi, spx[u] + (unsigned)((jv - 1) * fw2 + (xk - 1)) * 4u);
+            if (ykv > 0 && in_x) o[u].py = ldf(pyi, spy[u] + (unsigned)((ykv - 1) * g.nx + (c - 1)) * 4u);
+            if (xin && srow) o[u].qx = ldf(pvxb, spx[u] + (unsigned)((jsr - 1) * fw2 + (xk - 1)) * 4u);
+            if (yks > 0 && in_x) o[u].qy = ldf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u);
             if (VISCO) {
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
@@ -362,32 +397,17 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
                 syy_y = syy_y + hy * (zw[u][K + m - 1] - zw[u][K - m]);
             }
-            if (fw > 0) {
-                if (c <= fw) {
-                    const unsigned pi4 = spx[u] + (unsigned)((jv - 1) * fw2 + (c - 1)) * 4u;
-                    const float p = pmlb[2 * pn + c] * ldf(pxi, pi4) + pmlb[1 * pn + c] * sxy_x;
-                    if (own_v) stf(pxo, pi4, p);
-                    sxy_x = sxy_x / pmlb[0 * pn + c] + p;
-                }
-                if (c >= g.nx - fw + 1) {
-                    const int h1 = c - g.nx + fw2;
-                    const unsigned pi4 = spx[u] + (unsigned)((jv - 1) * fw2 + (h1 - 1)) * 4u;
-                    const float p = pmlb[2 * pn + h1] * ldf(pxi, pi4) + pmlb[1 * pn + h1] * sxy_x;
-                    if (own_v) stf(pxo, pi4, p);
-                    sxy_x = sxy_x / pmlb[0 * pn + h1] + p;
-                }
-                if (!g.free_surf && jv <= fw) {
-                    const unsigned pi4 = spy[u] + (unsigned)((jv - 1) * g.nx + (c - 1)) * 4u;
-                    const float p = pmlb[8 * pn + jv] * ldf(pyi, pi4) + pmlb[7 * pn + jv] * syy_y;
-                    if (own_v) stf(pyo, pi4, p);
-                    syy_y = syy_y / pmlb[6 * pn + jv] + p;
-                }
-                if (jv >= g.ny - fw + 1) {
-                    const int h1 = jv - g.ny + fw2;
-                    const unsigned pi4 = spy[u] + (unsigned)((h1 - 1) * g.nx + (c - 1)) * 4u;
-                    const float p = pmlb[8 * pn + h1] * ldf(pyi, pi4) + pmlb[7 * pn + h1] * syy_y;
-                    if (own_v) stf(pyo, pi4, p);
-                    syy_y = syy_y / pmlb[6 * pn + h1] + p;
+            if (xin) {   // update_v_PML.c: x-CPML, left (c <= FW) or right (c >= NX-FW+1)
+                const float p = vbx * cr[u].px + vax * sxy_x;
+                if (own_v) stf(pxo, spx[u] + (unsigned)((jv - 1) * fw2 + (xk - 1)) * 4u, p);
+                sxy_x = sxy_x / vKx + p;
+            }
+            {
+                const int ykv = ykof(jv);
+                if (ykv > 0) {   // y-CPML, top (no free surface) or bottom
+                    const float p = cm.vby * cr[u].py + cm.vay * syy_y;
+                    if (own_v) stf(pyo, spy[u] + (unsigned)((ykv - 1) * g.nx + (c - 1)) * 4u, p);
+                    syy_y = syy_y / cm.vKy + p;
                 }
             }
             const unsigned gi = gi32(g, jv, c);
@@ -465,36 +485,17 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 }
                 vyx = vyx * a.dhi;
                 vyy = vyy * a.dhi;
-                if (fw > 0) {
-                    if (c <= fw) {
-                        const unsigned pi4 = spx[u] + (unsigned)((js - 1) * fw2 + (c - 1)) * 4u;
-                        const float p = pmlb[5 * pn + c] * ldf(pvxb, pi4) + pmlb[4 * pn + c] * vyx;
-                        stf(pvxb, pi4, p);
-                        vyx = vyx / pmlb[3 * pn + c] + p;
-                    }
-                    if (c >= g.nx - fw + 1) {
-                        const int h1 = c - g.nx + fw2;
-                        const unsigned pi4 = spx[u] + (unsigned)((js - 1) * fw2 + (h1 - 1)) * 4u;
-                        const float p = pmlb[5 * pn + h1] * ldf(pvxb, pi4) + pmlb[4 * pn + h1] * vyx;
-                        stf(pvxb, pi4, p);
-                        vyx = vyx / pmlb[3 * pn + h1] + p;
-                    }
-                    if (!g.free_surf && js <= fw) {
-                        // Q2: the viscoelastic kernel uses the full-grid y profile on top
-                        const unsigned pi4 = spy[u] + (unsigned)((js - 1) * g.nx + (c - 1)) * 4u;
-                        const float bY = VISCO ? pmlb[8 * pn + js] : pmlb[11 * pn + js];
-                        const float aY = VISCO ? pmlb[7 * pn + js] : pmlb[10 * pn + js];
-                        const float KY = VISCO ? pmlb[6 * pn + js] : pmlb[9 * pn + js];
-                        const float p = bY * ldf(pvyb, pi4) + aY * vyy;
-                        stf(pvyb, pi4, p);
-                        vyy = vyy / KY + p;
-                    }
-                    if (js >= g.ny - fw + 1) {
-                        const int h1 = js - g.ny + fw2;
-                        const unsigned pi4 = spy[u] + (unsigned)((h1 - 1) * g.nx + (c - 1)) * 4u;
-                        const float p = pmlb[11 * pn + h1] * ldf(pvyb, pi4) + pmlb[10 * pn + h1] * vyy;
-                        stf(pvyb, pi4, p);
-                        vyy = vyy / pmlb[9 * pn + h1] + p;
+                if (xin) {   // update_s_*_PML.c: x-CPML of the stress
+                    const float p = sbx * cr[u].qx + sax * vyx;
+                    stf(pvxb, spx[u] + (unsigned)((js - 1) * fw2 + (xk - 1)) * 4u, p);
+                    vyx = vyx / sKx + p;
+                }
+                {
+                    const int yks = ykof(js);
+                    if (yks > 0) {   // y-CPML of the stress (profile chosen in fetch, Q2)
+                        const float p = cm.sby * cr[u].qy + cm.say * vyy;
+                        stf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u, p);
+                        vyy = vyy / cm.sKy + p;
                     }
                 }
                 const unsigned gs = gi32(g, js, c);
