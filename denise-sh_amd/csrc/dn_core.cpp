@@ -444,7 +444,7 @@ extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int
     std::vector<int> shots(n);
     for (int s = 0; s < n; ++s) shots[s] = s;
     const int saved_rows = c->rows_req, saved_graph = c->graph_steps;
-    const int cand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 20};
+    const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 20};
     auto time_it = [&](int rows, int graph, double *ms) -> int {
         c->rows_req = rows;
         c->graph_steps = graph;
@@ -588,11 +588,13 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
     return 0;
 }
 
-// rows per segment: enough waves to fill 256 CUs several times over
+// rows per segment: enough waves to fill 256 CUs several times over (about
+// 36 per CU: the latency of the row loads is hidden by many resident waves
+// rather than by long segments; 1200x300, 20 shots, FDORDER 8 -> 16 rows)
 static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
 {
     if (c->rows_req > 0) return std::min(c->rows_req, c->g.ny);
-    const int target = 4096;
+    const int target = 9216;
     const int cand[] = {1 << 20, 512, 256, 128, 96, 64, 48, 32, 24, 16};
     int best = 16;
     for (int R : cand) {

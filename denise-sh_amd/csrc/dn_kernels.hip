@@ -249,7 +249,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         float r[L > 0 ? L : 1], q[L > 0 ? L : 1];
         float gr, f0;           // velocity row: grad_rho / H_rho sum, stored vyp1 (fp32 snapshots)
         float wa, gu, fy, fu;   // stress row: Ws / Wr, grad_u / H_mu sum, stored uy, uyx (fp32)
-        uint16_t h0, hy, hu;    // the stored fields of 16-bit snapshots (decoded where used)
+        unsigned h0, hy, hu;    // the stored fields of 16-bit snapshots (decoded where used;
+                                // full registers, so their loads never merge into one)
     };
     // per-shot snapshot slot of this step and its decode scales
     const long long sbase = (long long)s * a.snap_shot_stride + (long long)slot * 3 * nxny;
@@ -264,7 +265,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         dsc[1] = scp[1];
         dsc[2] = scp[2];
     }
-    auto decode = [&](uint16_t h, float scale) -> float {
+    auto decode = [&](unsigned h, float scale) -> float {
+        // h holds a 16-bit code in its low half
         return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
     };
     auto fetch = [&](int jv, RowM &m, RowS (&o)[NSH]) {
@@ -321,30 +323,40 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
         }
         if (EX) {
+            // The accumulators and snapshots are read only where this wave
+            // writes them back.  Every lane loads (others at element 0 of the
+            // same arrays, unused) and both registers of a snapshot field are
+            // set on either storage path, so no lane keeps an older value in
+            // a register with a load in flight and nothing waits here.
             RowS &e = o[0];
             const bool vin = (jv >= j0 && jv <= j1) && own_x;
-            const unsigned cv = (unsigned)((jv - 1) * g.nx + (c - 1));
+            const unsigned cv = vin ? (unsigned)((jv - 1) * g.nx + (c - 1)) : 0u;
             if ((ADJ && corr) || (!ADJ && hsum)) {
-                e.gr = vin ? grb[cv] : 0.0f;
-                if (ADJ && vin) {
-                    if (a.snap_kind == 0)
+                e.gr = grb[cv];
+                if (ADJ) {
+                    if (a.snap_kind == 0) {
                         e.f0 = snb[cv];
-                    else
+                        e.h0 = 0u;
+                    } else {
                         e.h0 = snb16[cv];
+                        e.f0 = 0.0f;
+                    }
                 }
             }
             const int jsn = jv - K;
             const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
-            const unsigned cs = (unsigned)((jsn - 1) * g.nx + (c - 1));
-            if ((ADJ && corr) || (!ADJ && fstore)) e.wa = sin_ ? wab[cs] : 0.0f;
-            if ((ADJ && corr) || (!ADJ && hsum)) e.gu = sin_ ? gub[cs] : 0.0f;
-            if (ADJ && corr && sin_) {
+            const unsigned cs = sin_ ? (unsigned)((jsn - 1) * g.nx + (c - 1)) : 0u;
+            if ((ADJ && corr) || (!ADJ && fstore)) e.wa = wab[cs];
+            if ((ADJ && corr) || (!ADJ && hsum)) e.gu = gub[cs];
+            if (ADJ && corr) {
                 if (a.snap_kind == 0) {
                     e.fy = snb[nxny + cs];
                     e.fu = snb[2 * nxny + cs];
+                    e.hy = e.hu = 0u;
                 } else {
                     e.hy = snb16[nxny + cs];
                     e.hu = snb16[2 * nxny + cs];
+                    e.fy = e.fu = 0.0f;
                 }
             }
         }
