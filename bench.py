@@ -142,6 +142,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="tune rows per segment and graph replay first")
     ap.add_argument("--wave-shots", type=int, default=1, choices=[1, 2],
                     help="shots per wavefront on the plain steps (--autotune chooses)")
+    ap.add_argument("--rows", type=int, default=0, help="rows per segment of the step kernel (0: automatic)")
+    ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
+                    help="workgroups of adjacent strips (0) or of the shots of one strip (1)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     args = ap.parse_args()
@@ -161,6 +164,9 @@ def main():
                      snap_precision=prec)
     ctx.set_exec(-1 if args.exec == "graph" else 0)
     ctx.set_wave_shots(args.wave_shots)
+    if args.rows:
+        ctx.set_batch(0, args.rows)
+    ctx.set_group_layout(args.layout)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
     ctx.set_shots(W["src_i"][:args.shots], W["src_j"][:args.shots], sig)
@@ -222,8 +228,11 @@ def main():
     }
     if tuned is not None:
         out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
+    elif args.rows:
+        out["config"]["rows_per_segment"] = args.rows
     out["config"]["exec"] = args.exec
     out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
+    out["config"]["group_layout"] = "tuned" if args.autotune else ("shots" if args.layout else "strips")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     ctx.close()

@@ -28,6 +28,7 @@ EXPORTS = [
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
     "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
+    "dn_set_group_layout",
 ]
 
 
@@ -311,6 +312,11 @@ class Context:
     def set_wave_shots(self, n=2):
         lib().dn_set_wave_shots.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_set_wave_shots(self._h, n))
+
+    def set_group_layout(self, mode):
+        """0: workgroups of four adjacent strips (default); 1: of one strip of four shots."""
+        lib().dn_set_group_layout.argtypes = [C.c_void_p, C.c_int]
+        self._chk(lib().dn_set_group_layout(self._h, mode))
 
     def autotune(self, nshots, nsteps=40):
         rows, graph = C.c_int(), C.c_int()

@@ -84,22 +84,40 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     static_assert(NSH == 1 || (NSH == 2 && !EX), "two shots per wave only on steps without EX work");
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
-    const int nblk = (NSH == 1) ? a.nblocks : a.nblocks2;
-    const int nblk8 = (NSH == 1) ? a.nblocks8 : a.nblocks2_8;
+    const int nblk = (NSH == 2) ? a.nblocks2 : (a.shot_groups ? a.nblocks_sg : a.nblocks);
+    const int nblk8 = (NSH == 2) ? a.nblocks2_8 : (a.shot_groups ? a.nblocks_sg8 : a.nblocks8);
     const int lin = (w & 7) * nblk8 + (w >> 3);
     if (lin >= nblk) return;                      // whole workgroup
-    const int npair = (a.nshots + NSH - 1) / NSH;
-    const int p = lin % npair;
-    const int tile = lin / npair;
-    const int grp = tile % a.ngroups;
-    const int seg = tile / a.ngroups;
-    // the waves of a workgroup take adjacent strips of the same rows and
-    // step through the rows together (one s_barrier per row), so their
-    // loads of a row arrive at the memory as one contiguous run per field;
-    // spare waves of the last group re-read the last strip and store nothing
-    const int wv = threadIdx.x / kWave;
-    const bool live = grp * kStripsPerGroup + wv < a.nstrips;
-    const int strip = live ? grp * kStripsPerGroup + wv : a.nstrips - 1;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // The waves of a workgroup step through the rows together (one
+    // s_barrier per row).  Strip groups (default): they take adjacent strips
+    // of one shot, so their loads of a row arrive at the memory as one
+    // contiguous run per field; spare waves of the last group re-read the
+    // last strip and store nothing.  Shot groups (one shot per wave only):
+    // they take the same strip of consecutive shots, so their loads of the
+    // static model rows meet in the CU's L1; a wave without an active shot
+    // fetches only those rows and stores nothing.
+    const bool shot_grp = NSH == 1 && a.shot_groups;
+    int p, strip, seg, q0 = 0;
+    bool live;
+    if (shot_grp) {
+        const int nq = (a.nshots + kStripsPerGroup - 1) / kStripsPerGroup;
+        const int q = lin % nq;
+        const int tile = lin / nq;
+        strip = tile % a.nstrips;
+        seg = tile / a.nstrips;
+        q0 = q * kStripsPerGroup;
+        p = q0 + wv;
+        live = true;
+    } else {
+        const int npair = (a.nshots + NSH - 1) / NSH;
+        p = lin % npair;
+        const int tile = lin / npair;
+        const int grp = tile % a.ngroups;
+        seg = tile / a.ngroups;
+        live = grp * kStripsPerGroup + wv < a.nstrips;
+        strip = live ? grp * kStripsPerGroup + wv : a.nstrips - 1;
+    }
     const Geo &g = a.g;
     const int nt = a.nt_base ? (*a.nt_base + a.nt) : a.nt;
 
@@ -122,6 +140,16 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
         }
         any = any || act[u];
+    }
+    if (shot_grp) {
+        // the workgroup goes on while any of its shots does (barriers)
+        for (int k = 0; k < kStripsPerGroup && !any; ++k) {
+            const int t = q0 + k;
+            if (t < a.nshots) {
+                const int S = a.shot_map[t];
+                any = S >= 0 && S < a.nshots_all && nt <= a.shot_nt[S];
+            }
+        }
     }
     if (!any) return;                             // uniform over the workgroup
 
@@ -167,7 +195,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     const int s = sl[0];
     bool fstore = false, corr = false, hsum = false;
     int slot = 0;
-    if (!EX) {
+    if (!EX || !act[0]) {
     } else if (!ADJ) {
         if (a.store_mode != 0 && ((nt - 1) % a.dtinv) == 0) {
             fstore = true;
@@ -612,7 +640,8 @@ static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hi
 {
     // plain steps of batches with two or more shots take two shots per wave
     const bool two = !extra && a.nshots >= 2 && a.nblocks2 > 0;
-    dim3 grid((unsigned)((two ? a.nblocks2_8 : a.nblocks8) * 8)), block(kWave * kStripsPerGroup);
+    const int n8 = two ? a.nblocks2_8 : (a.shot_groups ? a.nblocks_sg8 : a.nblocks8);
+    dim3 grid((unsigned)(n8 * 8)), block(kWave * kStripsPerGroup);
     if (adjoint) {
         if (extra)
             hipLaunchKernelGGL((k_step<K, L, true, true, 1>), grid, block, 0, st, a);

@@ -221,11 +221,18 @@ int dn_set_exec(dn_ctx *ctx, int graph_steps);
  * wave, sharing the loads of the static model fields (more registers per
  * wave).  Results are identical; dn_autotune picks the faster. */
 int dn_set_wave_shots(dn_ctx *ctx, int shots_per_wave);
+/* Workgroup layout of the time-step kernel (one shot per wave): 0 (default)
+ * four adjacent strips of one shot; 1 the same strip of four consecutive
+ * shots of the batch, whose loads of the static model fields meet in the
+ * compute unit's cache.  Results are identical; dn_autotune picks the
+ * faster.  No counterpart in the reference (a launch-shape choice). */
+int dn_set_group_layout(dn_ctx *ctx, int mode);
 
 /* Tile-shape autotuner: times nsteps forward steps of the first nshots
  * shots for each candidate rows-per-segment, then one or two shots per
- * wave and graph replay, keeps the fastest setting and returns rows and
- * graph use. */
+ * wave, the other workgroup layout (at half, equal and double the rows)
+ * and graph replay, keeps the fastest setting and returns rows and graph
+ * use. */
 int dn_autotune(dn_ctx *ctx, int nshots, int nsteps, int *rows_per_segment, int *graph_steps);
 
 /* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis

@@ -329,7 +329,7 @@ def test_gradient_after_forward_on_same_shots():
 
 
 def test_two_shots_per_wave_equals_one():
-    """dn_set_wave_shots(2) (the default) and (1) give identical seismograms
+    """dn_set_wave_shots(2) and (1) (the default) give identical seismograms
     and gradients, with an odd shot count and uneven record lengths."""
     nx, ny, nt = 110, 48, 300
     ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, nts=[300, 260, 290])
@@ -349,3 +349,39 @@ def test_two_shots_per_wave_equals_one():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     assert a[3] == b[3] and a[4] == b[4]
     ctx.close()
+
+
+@pytest.mark.parametrize("snap", [0, 1])
+def test_shot_group_layout_equals_strip_groups(snap):
+    """dn_set_group_layout(1) (workgroups of one strip of four shots) gives
+    the seismograms and gradients of the default layout bit for bit: six
+    shots (the second group has two spare waves), uneven record lengths (a
+    group's shots finish at different steps), with one and two shots per
+    wave on the plain steps, FP32 and BF16 snapshots."""
+    nx, ny, nt = 130, 52, 300
+    nts = [300, 240, 280, 300, 200, 260]
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=6, dtinv=3, nts=nts, snap_precision=snap)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 6)
+    for s, n in enumerate(nts):
+        obs[s, :, n:] = 0.0
+    ctx.set_observed(obs)
+    out = {}
+    for layout in (0, 1):
+        for nsh in (1, 2):
+            ctx.set_group_layout(layout)
+            ctx.set_wave_shots(nsh)
+            seis = ctx.forward()
+            l2, en = ctx.gradient()
+            gvs, grho, _, _ = ctx.get_gradient()
+            out[(layout, nsh)] = (seis, gvs, grho, l2, en)
+    ctx.set_group_layout(0)
+    ctx.set_wave_shots(1)
+    a = out[(0, 1)]
+    for k, b in out.items():
+        say("  layout %d, %d shot(s) per wave: seismograms equal %s, gradients equal %s" % (
+            k[0], k[1], np.array_equal(a[0], b[0]), np.array_equal(a[1], b[1])))
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+        assert a[3] == b[3] and a[4] == b[4]
+    assert np.abs(a[1]).max() > 0
+    ctx.close()
+
