@@ -300,9 +300,15 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     auto fetch = [&](int jv, RowM &m, RowS (&o)[NSH]) {
         const unsigned gv = gi32(g, jv, c) * 4u;
         const unsigned gs = gi32(g, jv - K, c) * 4u;
+        // the stress-row inputs are fetched only where the iteration updates
+        // the stress (row jv - K inside the segment): the first 2K-1
+        // iterations of a segment advance the velocity alone (wave-uniform)
+        const int jsr = jv - K;
+        const bool sneed = jsr >= j0;
         m.b = ldf(a.bfield, gv);
         m.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
-        if (!VISCO) {
+        if (!sneed) {
+        } else if (!VISCO) {
             m.s1 = ldf(a.uip, gs);
             m.s2 = ldf(a.ujp, gs);
         } else {
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 m.dd[l] = ldf(a.dcoef, o4);
             }
         }
-        const int ykv = ykof(jv), jsr = jv - K, yks = ykof(jsr);
+        const int ykv = ykof(jv), yks = sneed ? ykof(jsr) : 0;
         if (ykv > 0) {
             m.vKy = pmlb[6 * pn + ykv];
             m.vay = pmlb[7 * pn + ykv];
@@ -339,9 +345,9 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             o[u].vy = ldf(vyi, sof[u] + gv);
             if (xin && vrow) o[u].px = ldf(pxi, spx[u] + (unsigned)((jv - 1) * fw2 + (xk - 1)) * 4u);
             if (ykv > 0 && in_x) o[u].py = ldf(pyi, spy[u] + (unsigned)((ykv - 1) * g.nx + (c - 1)) * 4u);
-            if (xin && srow) o[u].qx = ldf(pvxb, spx[u] + (unsigned)((jsr - 1) * fw2 + (xk - 1)) * 4u);
+            if (xin && srow && sneed) o[u].qx = ldf(pvxb, spx[u] + (unsigned)((jsr - 1) * fw2 + (xk - 1)) * 4u);
             if (yks > 0 && in_x) o[u].qy = ldf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u);
-            if (VISCO) {
+            if (VISCO && sneed) {
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
                     const unsigned o4 = sor[u] + gs + (unsigned)(l * plane) * 4u;
@@ -403,16 +409,9 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src[u] = true;
             }
     }
-    RowM nm_;
-    RowS ns_[NSH];
-    fetch(jstart, nm_, ns_);
-
-    for (int jv = jstart; jv <= jend; ++jv) {
-        const RowM cm = nm_;
-        RowS cr[NSH];
-#pragma unroll
-        for (int u = 0; u < NSH; ++u) cr[u] = ns_[u];
-        if (jv < jend) fetch(jv + 1, nm_, ns_);
+    // one iteration: velocity row jv and stress row jv - K, from inputs
+    // fetched earlier (cm, cr)
+    auto row = [&](const int jv, const RowM &cm, const RowS (&cr)[NSH]) {
         float vnew[NSH], sc[NSH];
 #pragma unroll
         for (int u = 0; u < NSH; ++u) {
@@ -609,6 +608,24 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
 #pragma unroll
             for (int t = 0; t < 2 * K - 1; ++t) zw[u][t] = zw[u][t + 1];
         }
+    };
+
+    // inputs are fetched one row ahead: the wait at the end of an
+    // iteration is for the loads issued at its start
+    RowM nm_;
+    RowS ns_[NSH];
+    fetch(jstart, nm_, ns_);
+    for (int jv = jstart; jv <= jend; ++jv) {
+        const RowM cm = nm_;
+        RowS cr[NSH];
+#pragma unroll
+        for (int u = 0; u < NSH; ++u) cr[u] = ns_[u];
+        // the next row's loads go out at raised wave priority, so a wave
+        // that reaches its fetch issues it before the others' arithmetic
+        __builtin_amdgcn_s_setprio(2);
+        if (jv < jend) fetch(jv + 1, nm_, ns_);
+        __builtin_amdgcn_s_setprio(0);
+        row(jv, cm, cr);
     }
 
     // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
