@@ -103,12 +103,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_visco_gradient_1200x300_hb
 
 
 def measured_traffic(cells_per_launch):
-    """HBM bytes per launch of the adjoint step kernel on this workload, from
-    the FETCH_SIZE / WRITE_SIZE profile (scaled by the copy-probe factors)
-    kept in profiles/: bytes per cell-update x cell-updates per launch."""
+    """HBM bytes per launch of the step kernel on this workload (mean over
+    the forward and adjoint launches of a gradient), from the FETCH_SIZE /
+    WRITE_SIZE profile (scaled by the copy-probe factors) kept in profiles/:
+    bytes per cell-update x cell-updates per launch."""
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_cell_update_adjoint:"):
+            if line.startswith("traffic_bytes_per_cell_update_gradient:"):
                 return round(float(line.split(":")[1]) * cells_per_launch)
     except OSError:
         pass
@@ -194,15 +195,19 @@ def main():
     wall = time.perf_counter() - t0
     cells = float(ce[0] + ce[1])
     value = cells / wall / 1e9
-    # dominant kernel: the fused step kernel of the pass with the larger total time
-    dom = 1 if ms[1] >= ms[0] else 0
-    per_launch_ms = ms[dom] / max(la[dom], 1)
-    units_per_launch = ce[dom] / max(la[dom], 1)          # cell-updates per launch
-    bytes_per_cell = 12.0 * (3 + 2 * W["L"]) + (36.0 if dom == 1 else 20.0) / W["dtinv"]
-    if prec != 0:
-        bytes_per_cell -= (6.0 / W["dtinv"])
+    # dominant kernel: the fused step kernel k_step (all but a fraction of a
+    # percent of the GPU time), over the launches of both passes: the
+    # forward pass moves 84 + 20/DTINV bytes per cell-update, the adjoint
+    # 84 + 36/DTINV (30/DTINV with 16-bit snapshots)
+    nla = max(la[0] + la[1], 1)
+    per_launch_ms = (ms[0] + ms[1]) / nla
+    units_per_launch = (ce[0] + ce[1]) / nla              # cell-updates per launch
+    base = 12.0 * (3 + 2 * W["L"])
+    b_fwd = base + 20.0 / W["dtinv"]
+    b_adj = base + (36.0 - (6.0 if prec != 0 else 0.0)) / W["dtinv"]
+    bytes_per_cell = (ce[0] * b_fwd + ce[1] * b_adj) / max(ce[0] + ce[1], 1)
     achieved = units_per_launch * bytes_per_cell / (per_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(units_per_launch) if (dom == 1 and prec == 0 and args.nt >= 1000) else None
+    traffic = measured_traffic(units_per_launch) if (prec == 0 and args.nt >= 1000) else None
     out = {
         "metric": "Gcell-updates/s fwd+adj",
         "value": round(value, 3),
@@ -221,8 +226,8 @@ def main():
                    "nx": W["nx"], "ny": W["ny"], "shots": args.shots, "nt": args.nt, "fdorder": 8, "L": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_step<4,visco,%s>" % ("adjoint" if dom else "forward"),
-                     "ms_per_launch": round(per_launch_ms, 4), "bytes_per_cell_update": bytes_per_cell,
+                     "kernel": "k_step<4,visco> (forward and adjoint launches of the gradient)",
+                     "ms_per_launch": round(per_launch_ms, 4), "bytes_per_cell_update": round(bytes_per_cell, 3),
                      "cell_updates_per_launch": units_per_launch},
         "misfit_l2": l2,
     }

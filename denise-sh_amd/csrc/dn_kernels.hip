@@ -72,7 +72,7 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 // range of (tile, shot) pairs with the shot index fastest: the shots of one
 // tile share the static model fields through that XCD's L2.
 // ---------------------------------------------------------------------------
-template <int K, int L, bool ADJ, bool EX, int NSH>
+template <int K, int L, bool ADJ, int EX, int NSH>
 __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs a)
 {
     // EX: this launch may store snapshots / sum the pseudo-Hessian (forward)
@@ -81,7 +81,11 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // advances the same strip of two shots of the batch: the static model
     // fields and the receiver table are loaded once for both, and the two
     // shots' independent arithmetic hides each other's latency.
+    // EX: 0 none; 1 with FP32 snapshots (and every forward EX launch); 2
+    // adjoint correlation with 16-bit snapshots (decoded where used).
     static_assert(NSH == 1 || (NSH == 2 && !EX), "two shots per wave only on steps without EX work");
+    static_assert(EX >= 0 && EX <= 2 && (EX < 2 || ADJ), "EX variant");
+    constexpr bool S16 = ADJ && EX == 2;
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
     const int nblk = (NSH == 2) ? a.nblocks2 : (a.shot_groups ? a.nblocks_sg : a.nblocks);
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     const uint16_t *__restrict__ snb16 = (EX && a.snap16) ? a.snap16 + sbase : nullptr;
     float *__restrict__ stg = (EX && a.stage) ? a.stage + (long long)s * 3 * nxny : nullptr;
     float dsc[3] = {1.0f, 1.0f, 1.0f};
-    if (ADJ && corr && a.snap_kind != 0) {
+    if (S16 && corr) {
         const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
         dsc[0] = scp[0];
         dsc[1] = scp[1];
@@ -359,39 +363,29 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         if (EX) {
             // The accumulators and snapshots are read only where this wave
             // writes them back.  Every lane loads (others at element 0 of the
-            // same arrays, unused) and both registers of a snapshot field are
-            // set on either storage path, so no lane keeps an older value in
-            // a register with a load in flight and nothing waits here.
+            // same arrays, unused), so no lane keeps an older value in a
+            // register with a load in flight and nothing waits here; the
+            // snapshot storage (FP32 or 16-bit) is fixed per variant.
             RowS &e = o[0];
             const bool vin = (jv >= j0 && jv <= j1) && own_x;
             const unsigned cv = vin ? (unsigned)((jv - 1) * g.nx + (c - 1)) : 0u;
             if ((ADJ && corr) || (!ADJ && hsum)) {
                 e.gr = grb[cv];
-                if (ADJ) {
-                    if (a.snap_kind == 0) {
-                        e.f0 = snb[cv];
-                        e.h0 = 0u;
-                    } else {
-                        e.h0 = snb16[cv];
-                        e.f0 = 0.0f;
-                    }
-                }
+                if (ADJ && !S16) e.f0 = snb[cv];
+                if (S16) e.h0 = snb16[cv];
             }
             const int jsn = jv - K;
             const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
             const unsigned cs = sin_ ? (unsigned)((jsn - 1) * g.nx + (c - 1)) : 0u;
             if ((ADJ && corr) || (!ADJ && fstore)) e.wa = wab[cs];
             if ((ADJ && corr) || (!ADJ && hsum)) e.gu = gub[cs];
-            if (ADJ && corr) {
-                if (a.snap_kind == 0) {
-                    e.fy = snb[nxny + cs];
-                    e.fu = snb[2 * nxny + cs];
-                    e.hy = e.hu = 0u;
-                } else {
-                    e.hy = snb16[nxny + cs];
-                    e.hu = snb16[2 * nxny + cs];
-                    e.fy = e.fu = 0.0f;
-                }
+            if (ADJ && corr && !S16) {
+                e.fy = snb[nxny + cs];
+                e.fu = snb[2 * nxny + cs];
+            }
+            if (S16 && corr) {
+                e.hy = snb16[nxny + cs];
+                e.hu = snb16[2 * nxny + cs];
             }
         }
     };
@@ -467,7 +461,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
             if (EX && ADJ && corr && own_v) {
                 // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
-                const float f0 = (a.snap_kind == 0) ? cr[u].f0 : decode(cr[u].h0, dsc[0]);
+                const float f0 = S16 ? decode(cr[u].h0, dsc[0]) : cr[u].f0;
                 grb[ci] = cr[u].gr + (vold * f0);
             }
             float vn = vold + ((bb * a.DT) * tsum) / a.DH;
@@ -593,8 +587,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                         gub[ci] = cr[u].gu + ((vyy * vyy) + (eyx * eyx));
                     }
                     if (ADJ && corr) {
-                        const float fy = (a.snap_kind == 0) ? cr[u].fy : decode(cr[u].hy, dsc[1]);
-                        const float fu = (a.snap_kind == 0) ? cr[u].fu : decode(cr[u].hu, dsc[2]);
+                        const float fy = S16 ? decode(cr[u].hy, dsc[1]) : cr[u].fy;
+                        const float fu = S16 ? decode(cr[u].hu, dsc[2]) : cr[u].fu;
                         gub[ci] = cr[u].gu + ((fu * sxn) + (fy * syn));   // denise.c:1609
                         wab[ci] = cr[u].wa + vc * vc;
                     }
@@ -660,19 +654,21 @@ static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hi
     const int n8 = two ? a.nblocks2_8 : (a.shot_groups ? a.nblocks_sg8 : a.nblocks8);
     dim3 grid((unsigned)(n8 * 8)), block(kWave * kStripsPerGroup);
     if (adjoint) {
-        if (extra)
-            hipLaunchKernelGGL((k_step<K, L, true, true, 1>), grid, block, 0, st, a);
+        if (extra && a.snap_kind != 0)
+            hipLaunchKernelGGL((k_step<K, L, true, 2, 1>), grid, block, 0, st, a);
+        else if (extra)
+            hipLaunchKernelGGL((k_step<K, L, true, 1, 1>), grid, block, 0, st, a);
         else if (two)
-            hipLaunchKernelGGL((k_step<K, L, true, false, 2>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, true, 0, 2>), grid, block, 0, st, a);
         else
-            hipLaunchKernelGGL((k_step<K, L, true, false, 1>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, true, 0, 1>), grid, block, 0, st, a);
     } else {
         if (extra)
-            hipLaunchKernelGGL((k_step<K, L, false, true, 1>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, false, 1, 1>), grid, block, 0, st, a);
         else if (two)
-            hipLaunchKernelGGL((k_step<K, L, false, false, 2>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, false, 0, 2>), grid, block, 0, st, a);
         else
-            hipLaunchKernelGGL((k_step<K, L, false, false, 1>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_step<K, L, false, 0, 1>), grid, block, 0, st, a);
     }
     return hipGetLastError();
 }
