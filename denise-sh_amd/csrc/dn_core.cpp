@@ -261,6 +261,25 @@ int dn_prepare_model(dn_ctx *c)
     return 0;
 }
 
+// denise.c:778-816: the means of vs and rho as float sums, i outer, j
+// inner, divided by the cell count
+static void model_means(const float *vs, const float *rho, int nx, int ny, float &C_vs, float &C_rho)
+{
+    float vs_avg = 0.0f, rho_avg = 0.0f;
+    int count = 1;
+    for (int i = 1; i <= nx; ++i)
+        for (int j = 1; j <= ny; ++j) {
+            vs_avg += vs[(size_t)(j - 1) * nx + (i - 1)];
+            rho_avg += rho[(size_t)(j - 1) * nx + (i - 1)];
+            ++count;
+        }
+    --count;
+    vs_avg /= (float)count;
+    rho_avg /= (float)count;
+    C_vs = vs_avg;
+    C_rho = rho_avg;
+}
+
 extern "C" int dn_set_model(dn_ctx *c, const float *vs, const float *rho, const float *qs)
 {
     if (!c || !vs || !rho) return dn_fail(c, "dn_set_model: NULL argument");
@@ -274,24 +293,23 @@ extern "C" int dn_set_model(dn_ctx *c, const float *vs, const float *rho, const 
     if (upload_padded(c, c->d_vs, vs) || upload_padded(c, c->d_rho, rho)) return -1;
     if (L > 0 && upload_padded(c, c->d_taus, taus.data())) return -1;
     if (dn_prepare_model(c)) return -1;
-    if (!c->have_C) {
-        // denise.c:778-816: float sums, i outer, j inner
-        float vs_avg = 0.0f, rho_avg = 0.0f;
-        int count = 1;
-        for (int i = 1; i <= nx; ++i)
-            for (int j = 1; j <= ny; ++j) {
-                vs_avg += vs[(size_t)(j - 1) * nx + (i - 1)];
-                rho_avg += rho[(size_t)(j - 1) * nx + (i - 1)];
-                ++count;
-            }
-        --count;
-        vs_avg /= (float)count;
-        rho_avg /= (float)count;
-        c->C_vs = vs_avg;
-        c->C_rho = rho_avg;
-        c->have_C = true;
-    }
+    model_means(vs, rho, nx, ny, c->C_vs, c->C_rho);
+    c->have_C = true;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// The means of the current model that normalise the gradient and the L-BFGS
+// vectors, recomputed where the reference recomputes them: at every FWI
+// iteration, before the gradient (denise.c:773-816).  The model on the
+// device may have changed since dn_set_model (model updates of the FWI
+// loop, probes), so it is read back.
+int dn_update_means(dn_ctx *c)
+{
+    const size_t n = (size_t)c->g.nx * c->g.ny;
+    std::vector<float> vs(n), rho(n);
+    if (dn_get_model(c, vs.data(), rho.data())) return -1;
+    model_means(vs.data(), rho.data(), c->g.nx, c->g.ny, c->C_vs, c->C_rho);
     return 0;
 }
 
@@ -1031,6 +1049,7 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
 {
     if (!c->d_obs) return dn_fail(c, "gradient: observed data not set");
     if (!c->have_C) return dn_fail(c, "gradient: model not set");
+    if (dn_update_means(c)) return -1;
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     if (!accumulate) {
         HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));

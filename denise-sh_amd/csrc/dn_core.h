@@ -114,6 +114,7 @@ int dn_fail(dn_ctx *c, const char *fmt, ...);
         if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
     } while (0)
 int dn_prepare_model(dn_ctx *c);
+int dn_update_means(dn_ctx *c);   // C_vs, C_rho from the current device model
 int dn_ensure_batch(dn_ctx *c, int want, bool gradient);
 int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record);
 // residual of the batch slots of the last forward pass: resid + per-shot l2/energy

@@ -160,8 +160,10 @@ const char *dn_last_error(const dn_ctx *ctx);
 
 /* Model: vs, rho and (visco) qs, host [ny][nx].  Runs matcopy (periodic
  * halo), av_mue, av_tau and prepare_update_s on the device
- * (denise.c:726-743).  The first call fixes C_vs and C_rho (means of the
- * starting model, denise.c:778-816). */
+ * (denise.c:726-743).  The gradient normalisation means C_vs and C_rho are
+ * those of the model a gradient is taken at: every gradient recomputes them
+ * from the current model, as every FWI iteration of the reference does
+ * (denise.c:773-816). */
 int dn_set_model(dn_ctx *ctx, const float *vs, const float *rho, const float *qs);
 int dn_get_model(dn_ctx *ctx, float *vs, float *rho);
 

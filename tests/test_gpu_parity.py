@@ -385,3 +385,23 @@ def test_shot_group_layout_equals_strip_groups(snap):
     assert np.abs(a[1]).max() > 0
     ctx.close()
 
+
+
+def test_gradient_means_follow_the_current_model():
+    """The normalisation means C_vs, C_rho are those of the model the
+    gradient is taken at (denise.c:773-816 recomputes them every iteration):
+    setting another model first, as when observed data are modelled from a
+    true model, leaves the gradient at the starting model unchanged."""
+    nx, ny, nt = 100, 44, 400
+    ctx, cfg, d = setup_case(nx, ny, 4, 0, 1, 10, nt, 60.0, nshots=1)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 1)
+    ctx.set_model(d["vs"] * np.float32(1.2), d["rho"] * np.float32(0.9))
+    ctx.set_model(d["vs"], d["rho"])
+    ctx.set_observed(obs)
+    ctx.gradient()
+    gvs, grho, _, _ = ctx.get_gradient()
+    r = oracle("f32").gradient(cfg, d["vs"], d["rho"], None, d["src_i"], d["src_j"], d["sig"], d["rec_i"],
+                               d["rec_j"], obs)
+    say("  grad_vs rel L2 %.3e, grad_rho rel L2 %.3e" % (rel_l2(gvs, r["grad_vs"]), rel_l2(grho, r["grad_rho"])))
+    assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
+    ctx.close()
