@@ -293,23 +293,23 @@ extern "C" int dn_set_model(dn_ctx *c, const float *vs, const float *rho, const 
     if (upload_padded(c, c->d_vs, vs) || upload_padded(c, c->d_rho, rho)) return -1;
     if (L > 0 && upload_padded(c, c->d_taus, taus.data())) return -1;
     if (dn_prepare_model(c)) return -1;
-    model_means(vs, rho, nx, ny, c->C_vs, c->C_rho);
-    c->have_C = true;
+    c->have_model = true;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
-// The means of the current model that normalise the gradient and the L-BFGS
-// vectors, recomputed where the reference recomputes them: at every FWI
-// iteration, before the gradient (denise.c:773-816).  The model on the
-// device may have changed since dn_set_model (model updates of the FWI
-// loop, probes), so it is read back.
+// The means that normalise the gradient and the L-BFGS vectors.  The
+// reference takes them once, from the model of its first iteration
+// (denise.c:746-819, inside `if (iter_true == 1)`), and keeps them for the
+// run; here they are taken from the device model at the first gradient of a
+// context and at dn_fwi_start, and kept until then.
 int dn_update_means(dn_ctx *c)
 {
     const size_t n = (size_t)c->g.nx * c->g.ny;
     std::vector<float> vs(n), rho(n);
     if (dn_get_model(c, vs.data(), rho.data())) return -1;
     model_means(vs.data(), rho.data(), c->g.nx, c->g.ny, c->C_vs, c->C_rho);
+    c->C_fixed = true;
     return 0;
 }
 
@@ -465,7 +465,7 @@ extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int
 {
     if (!c) return -1;
     if (nshots < 1 || nshots > c->nshots || nsteps < 1) return dn_fail(c, "dn_autotune: bad arguments");
-    if (!c->have_C) return dn_fail(c, "dn_autotune: model not set");
+    if (!c->have_model) return dn_fail(c, "dn_autotune: model not set");
     if (dn_ensure_batch(c, nshots, false)) return -1;
     const int n = std::min(nshots, c->batch_cap);
     std::vector<int> shots(n);
@@ -1048,8 +1048,8 @@ extern "C" int dn_forward(dn_ctx *c, int shot0, int nshots, float *seis)
 int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
 {
     if (!c->d_obs) return dn_fail(c, "gradient: observed data not set");
-    if (!c->have_C) return dn_fail(c, "gradient: model not set");
-    if (dn_update_means(c)) return -1;
+    if (!c->have_model) return dn_fail(c, "gradient: model not set");
+    if (!c->C_fixed && dn_update_means(c)) return -1;
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     if (!accumulate) {
         HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));
@@ -1107,7 +1107,7 @@ extern "C" int dn_hessian_probe(dn_ctx *c, int shot0, int nshots, int nprobes, f
     if (!c) return -1;
     if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots || nprobes < 1 || !(eps > 0.0f) || !diag_vs)
         return dn_fail(c, "dn_hessian_probe: bad arguments");
-    if (!c->have_C || !c->d_obs) return dn_fail(c, "dn_hessian_probe: model and observed data must be set");
+    if (!c->have_model || !c->d_obs) return dn_fail(c, "dn_hessian_probe: model and observed data must be set");
     const size_t P = (size_t)c->g.plane, nxny = (size_t)c->g.nx * c->g.ny;
     float *vs0 = nullptr, *g0 = nullptr, *diag = nullptr;
     auto cleanup = [&]() {
@@ -1151,7 +1151,7 @@ extern "C" int dn_pseudo_hessian(dn_ctx *c, int shot0, int nshots, float *h_vs, 
 {
     if (!c) return -1;
     if (shot0 < 0 || nshots < 1 || shot0 + nshots > c->nshots) return dn_fail(c, "dn_pseudo_hessian: bad shot range");
-    if (!c->have_C) return dn_fail(c, "dn_pseudo_hessian: model not set");
+    if (!c->have_model) return dn_fail(c, "dn_pseudo_hessian: model not set");
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     if (!c->d_hu) HIPCHK(c, dalloc(&c->d_hu, nxny));
     if (!c->d_hr) HIPCHK(c, dalloc(&c->d_hr, nxny));

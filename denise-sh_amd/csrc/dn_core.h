@@ -32,7 +32,8 @@ struct dn_ctx {
     float *d_vs = nullptr, *d_rho = nullptr, *d_taus = nullptr, *d_uip = nullptr, *d_ujp = nullptr, *d_b = nullptr;
     float *d_tausipjp = nullptr, *d_fipjp = nullptr, *d_f = nullptr, *d_dip = nullptr, *d_d = nullptr;
     float C_vs = 0.0f, C_rho = 0.0f;
-    bool have_C = false;
+    bool have_model = false;
+    bool C_fixed = false;            // C_vs, C_rho taken (first gradient or dn_fwi_start)
     // gradient sums [ny][nx] and outputs after the chain rule
     float *d_gu = nullptr, *d_grho = nullptr, *d_gvs_out = nullptr, *d_grho_out = nullptr;
     unsigned int *d_maxbits = nullptr;
@@ -114,7 +115,7 @@ int dn_fail(dn_ctx *c, const char *fmt, ...);
         if (e_ != hipSuccess) return dn_fail((ctx), "%s failed: %s", #expr, hipGetErrorString(e_));        \
     } while (0)
 int dn_prepare_model(dn_ctx *c);
-int dn_update_means(dn_ctx *c);   // C_vs, C_rho from the current device model
+int dn_update_means(dn_ctx *c);   // C_vs, C_rho from the current device model, then fixed
 int dn_ensure_batch(dn_ctx *c, int want, bool gradient);
 int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool store, bool record);
 // residual of the batch slots of the last forward pass: resid + per-shot l2/energy

@@ -267,9 +267,11 @@ static int misfit_of(dn_ctx *c, const std::vector<int> &shots, double *l2, doubl
 extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
 {
     if (!c || !o) return dn_fail(c, "dn_fwi_start: NULL argument");
-    if (!c->have_C || c->nshots < 1 || !c->d_obs) return dn_fail(c, "dn_fwi_start: set model, shots and data first");
+    if (!c->have_model || c->nshots < 1 || !c->d_obs) return dn_fail(c, "dn_fwi_start: set model, shots and data first");
     if (o->grad_method != 1 && o->grad_method != 2) return dn_fail(c, "GRAD_METHOD must be 1 (PCG) or 2 (L-BFGS)");
     if (o->grad_method == 2 && (o->nlbfgs < 1 || o->nlbfgs > 64)) return dn_fail(c, "NLBFGS outside 1..64");
+    // C_vs, C_rho of the starting model, kept for the run (denise.c:746-819)
+    if (dn_update_means(c)) return -1;
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     dn_fwi_state *f = c->fwi;
     const bool fresh = (f == nullptr);

@@ -161,9 +161,9 @@ const char *dn_last_error(const dn_ctx *ctx);
 /* Model: vs, rho and (visco) qs, host [ny][nx].  Runs matcopy (periodic
  * halo), av_mue, av_tau and prepare_update_s on the device
  * (denise.c:726-743).  The gradient normalisation means C_vs and C_rho are
- * those of the model a gradient is taken at: every gradient recomputes them
- * from the current model, as every FWI iteration of the reference does
- * (denise.c:773-816). */
+ * taken from the model of the first gradient of the context (and again at
+ * dn_fwi_start) and kept: the reference computes them in its first FWI
+ * iteration only (denise.c:746-819). */
 int dn_set_model(dn_ctx *ctx, const float *vs, const float *rho, const float *qs);
 int dn_get_model(dn_ctx *ctx, float *vs, float *rho);
 

@@ -6,8 +6,8 @@ loop (dn_fwi_iteration).  Never by the product path.
 
 One iteration follows denise.c:695-2018:
   * gradient of all shots at the current model with the normalisation means
-    C_vs, C_rho of the current model, recomputed every iteration
-    (denise.c:773-816), after eprecond1 and the chain rule (orc_gradient);
+    C_vs, C_rho of the starting model (denise.c:745-816), after eprecond1
+    and the chain rule (orc_gradient);
   * L2t[1] = L2t[4] = sum of the test shots' misfits / their energies
     (calc_res.c:255-257, calc_energy.c:57-61, denise.c:1715-1723);
   * smooth_grad / taper_grad when switched on (denise.c:1818-1847;
@@ -158,10 +158,6 @@ class FwiOracle:
             self.ptr += 1
         if self.ptr > o["nlbfgs"]:
             self.ptr = 1
-        # denise.c:773-816: the means of the current model, every iteration
-        self.cfg.c_vs = float(float_mean(self.vs))
-        self.cfg.c_rho = float(float_mean(self.rho))
-        self.C_vs, self.C_rho = np.float32(self.cfg.c_vs), np.float32(self.cfg.c_rho)
         g = O.gradient(self.cfg, self.vs, self.rho, self.qs, self.src_i, self.src_j, self.sig, self.rec_i,
                        self.rec_j, self.obs)
         gvs = np.asarray(g["grad_vs"], np.float32)

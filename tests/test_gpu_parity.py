@@ -387,21 +387,33 @@ def test_shot_group_layout_equals_strip_groups(snap):
 
 
 
-def test_gradient_means_follow_the_current_model():
-    """The normalisation means C_vs, C_rho are those of the model the
-    gradient is taken at (denise.c:773-816 recomputes them every iteration):
-    setting another model first, as when observed data are modelled from a
-    true model, leaves the gradient at the starting model unchanged."""
+def test_gradient_means_are_those_of_the_first_gradient():
+    """The normalisation means C_vs, C_rho come from the model of the first
+    gradient and stay fixed (denise.c:746-819 computes them in the first
+    iteration only): another model set before it, as when observed data are
+    modelled from a true model, does not count; a later gradient at another
+    model keeps the first means."""
     nx, ny, nt = 100, 44, 400
     ctx, cfg, d = setup_case(nx, ny, 4, 0, 1, 10, nt, 60.0, nshots=1)
     obs = _obs_for(ctx, cfg, d, nx, ny, 1)
+    o32 = oracle("f32")
     ctx.set_model(d["vs"] * np.float32(1.2), d["rho"] * np.float32(0.9))
     ctx.set_model(d["vs"], d["rho"])
     ctx.set_observed(obs)
     ctx.gradient()
     gvs, grho, _, _ = ctx.get_gradient()
-    r = oracle("f32").gradient(cfg, d["vs"], d["rho"], None, d["src_i"], d["src_j"], d["sig"], d["rec_i"],
-                               d["rec_j"], obs)
-    say("  grad_vs rel L2 %.3e, grad_rho rel L2 %.3e" % (rel_l2(gvs, r["grad_vs"]), rel_l2(grho, r["grad_rho"])))
+    r = o32.gradient(cfg, d["vs"], d["rho"], None, d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    say("  first gradient: grad_vs rel L2 %.3e" % rel_l2(gvs, r["grad_vs"]))
     assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
+    # a second gradient at another model: the means of the first
+    from fwi_oracle import float_mean
+    vs2 = d["vs"] * np.float32(1.05)
+    ctx.set_model(vs2, d["rho"])
+    ctx.gradient()
+    g2, gr2, _, _ = ctx.get_gradient()
+    cfg.c_vs, cfg.c_rho = float(float_mean(d["vs"])), float(float_mean(d["rho"]))
+    r2 = o32.gradient(cfg, vs2, d["rho"], None, d["src_i"], d["src_j"], d["sig"], d["rec_i"], d["rec_j"], obs)
+    cfg.c_vs = cfg.c_rho = 0.0
+    say("  second gradient, first means: grad_vs rel L2 %.3e" % rel_l2(g2, r2["grad_vs"]))
+    assert np.array_equal(g2, r2["grad_vs"]) and np.array_equal(gr2, r2["grad_rho"])
     ctx.close()
