@@ -116,6 +116,36 @@ def measured_traffic(cells_per_launch):
     return None
 
 
+def fwi_run(ctx, W, args):
+    """Plain L-BFGS FWI of the workload (GRAD_METHOD 2, NLBFGS 20, EPRECOND 1,
+    vs only, test shots 1, 11, 21, 31 for the line search) from the 1-D
+    starting model: wall time of the iterations and the misfit after each,
+    with the distance of the model to the true one.  The library's calls
+    return with their work done (they synchronise their own stream)."""
+    ctx.fwi_start(testshot_start=1, testshot_end=args.shots, testshot_incr=10, eps_scale=0.01, stepmax=6,
+                  vs_lower=50.0, vs_upper=600.0)
+    hist = []
+    t0 = time.perf_counter()
+    for _ in range(args.fwi):
+        r = ctx.fwi_iteration()
+        hist.append(dict(misfit=round(float(r["L2t"][3]), 6), step=round(float(r["opteps_vs"]), 6),
+                         test_forwards=int(r["test_forwards"])))
+        print("fwi iteration %d: L2t %.6g, step %.4g, %d test forwards, %.1f s" % (
+            len(hist), hist[-1]["misfit"], hist[-1]["step"], hist[-1]["test_forwards"],
+            time.perf_counter() - t0), file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    vs, _ = ctx.get_model()
+    err0 = float(np.linalg.norm(W["vs_start"] - W["vs_true"]) / np.linalg.norm(W["vs_true"]))
+    err = float(np.linalg.norm(vs - W["vs_true"]) / np.linalg.norm(W["vs_true"]))
+    print(json.dumps({"metric": "FWI wall time, plain L-BFGS", "value": round(wall, 3), "unit": "s",
+                      "higher_is_better": False, "iterations": args.fwi, "s_per_iteration": round(wall / args.fwi, 3),
+                      "misfit_L2t": [h["misfit"] for h in hist], "steps": [h["step"] for h in hist],
+                      "test_forwards": [h["test_forwards"] for h in hist],
+                      "vs_rel_error": {"start": round(err0, 5), "end": round(err, 5)},
+                      "config": {"workload": "visco SH FWI 1200x300 FDORDER8 L2, %d shots x %d steps, DTINV %d"
+                                             % (args.shots, args.nt, W["dtinv"]), "n_gpus": 1}}))
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """Plain device-to-device copy of known size (torch's copy kernel): the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
@@ -148,6 +178,8 @@ def main():
                     help="workgroups of adjacent strips (0) or of the shots of one strip (1)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
+    ap.add_argument("--fwi", type=int, default=0,
+                    help="instead of the gradient metric: time this many L-BFGS FWI iterations of the workload")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -176,6 +208,10 @@ def main():
     obs = ctx.forward()
     ctx.set_observed(obs)
     ctx.set_model(W["vs_start"], W["rho"], W["qs"])
+    if args.fwi:
+        fwi_run(ctx, W, args)
+        ctx.close()
+        return
     tuned = None
     if args.autotune:
         tuned = ctx.autotune(args.shots, min(args.nt, 256))
