@@ -1,7 +1,7 @@
 """CPU restatement of DENISE-SH's FWI iteration with L-BFGS (GRAD_METHOD 2)
 on top of the oracle library (oracle/sh_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: used by tests/ as the checker of the device FWI
+TEST INFRASTRUCTURE ONLY: used by tests/ to check the device FWI
 loop (dn_fwi_iteration).  Never by the product path.
 
 One iteration follows denise.c:695-2018:

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/sh_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  Never by the product path.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg.  Never by the product path.
 
 `Oracle("f32")` reproduces the reference's FP32 arithmetic, `Oracle("f64")`
 is the FP64 oracle.  Arrays follow sh_oracle.c: padded fields of shape

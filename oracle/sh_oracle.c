@@ -1,7 +1,7 @@
 /*
  * sh_oracle.c -- CPU restatement of DENISE-SH's SH time-stepping and
- * adjoint-gradient path.  TEST INFRASTRUCTURE ONLY: this file is the checker
- * for the HIP product path.  Only tests/, __graft_entry__.smoke() and the
+ * adjoint-gradient path.  TEST INFRASTRUCTURE ONLY: this file checks the
+ * HIP product path.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load the libraries built from it.
  *
  * The file is compiled twice (see oracle/Makefile):

@@ -1,7 +1,7 @@
 """Restatement of stf.c (source time function by Wiener deconvolution) and
 of the trace normalisations of util.c:489-579 in numpy.
 
-TEST INFRASTRUCTURE ONLY: the checker of dn_stf / dn_stf_shot in tests/.
+TEST INFRASTRUCTURE ONLY: checks dn_stf / dn_stf_shot in tests/.
 
 stf.c:38-250 for one shot: killed / muted traces are zeroed, NORMALIZE 1
 divides samples 2..ns of each trace by the trace's largest |sample| over

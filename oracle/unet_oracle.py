@@ -1,6 +1,6 @@
 """CPU restatement of the learned preconditioner's U-Net (numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker of the product's HIP kernels
+TEST INFRASTRUCTURE ONLY: checks the product's HIP kernels
 (denise-sh_amd/csrc/dn_unet.hip, dn_unet.cpp); only tests/ may use it.  The
 network is not part of the reference.
 
