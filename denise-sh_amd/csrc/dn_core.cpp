@@ -650,16 +650,26 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
 static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
 {
     if (c->rows_req > 0) return std::min(c->rows_req, c->g.ny);
-    const int target = 9216;
+    const int ny = c->g.ny, target = 9216;
+    auto waves = [&](int R) { return (long long)nshots * nstrips * ((ny + R - 1) / R); };
     const int cand[] = {1 << 20, 512, 256, 128, 96, 64, 48, 32, 24, 16};
     int best = 16;
     for (int R : cand) {
-        const int Rr = std::min(R, c->g.ny);
-        const long long waves = (long long)nshots * nstrips * ((c->g.ny + Rr - 1) / Rr);
-        best = Rr;
-        if (waves >= target) break;
+        best = std::min(R, ny);
+        if (waves(best) >= target) break;
     }
-    return std::max(std::min(best, c->g.ny), std::min(4 * c->K, c->g.ny));
+    best = std::max(best, std::min(4 * c->K, ny));
+    // small batches (the line-search trials of a few test shots) would
+    // leave most of the chip idle even at 16 rows: they go down to 12 or 8
+    // rows (not below 2K), more recomputed rows for more waves
+    if (waves(best) < target / 2)
+        for (int R : {12, 8}) {
+            const int Rr = std::min(R, ny);
+            if (Rr < std::min(2 * c->K, ny) || Rr >= best) continue;
+            best = Rr;
+            if (waves(best) >= target / 2) break;
+        }
+    return best;
 }
 
 void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)

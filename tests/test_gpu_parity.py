@@ -417,3 +417,26 @@ def test_gradient_means_are_those_of_the_first_gradient():
     say("  second gradient, first means: grad_vs rel L2 %.3e" % rel_l2(g2, r2["grad_vs"]))
     assert np.array_equal(g2, r2["grad_vs"]) and np.array_equal(gr2, r2["grad_rho"])
     ctx.close()
+
+
+def test_segment_heights_give_identical_results():
+    """Rows per segment 8, 12, 16, 24 and one segment for the whole grid:
+    seismograms and gradients identical (the segments recompute 2K-1 rows
+    of the velocity at their edges; small batches default to short
+    segments, the measured workload to 16 rows)."""
+    nx, ny, nt = 130, 56, 300
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, dtinv=3)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 3)
+    ctx.set_observed(obs)
+    out = {}
+    for rows in (8, 12, 16, 24, ny):
+        ctx.set_batch(0, rows)
+        seis = ctx.forward()
+        ctx.gradient()
+        gvs, grho, _, _ = ctx.get_gradient()
+        out[rows] = (seis, gvs, grho)
+    a = out[ny]
+    for rows, b in out.items():
+        say("  rows %d: equal %s" % (rows, all(np.array_equal(x, y) for x, y in zip(a, b))))
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    ctx.close()
