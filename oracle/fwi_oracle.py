@@ -12,9 +12,14 @@ One iteration follows denise.c:695-2018:
     (calc_res.c:255-257, calc_energy.c:57-61, denise.c:1715-1723);
   * smooth_grad / taper_grad when switched on (denise.c:1818-1847;
     orc_gauss_filt_var, orc_taper_grad);
-  * store_LBFGS_SH.c:39-120: waveconv *= C, y/s pairs of [rho; u] in the
-    slot of LBFGS_pointer; LBFGS.c:26-95 two-loop in slot order (orc_lbfgs);
-    extract_LBFGS_SH.c:31-64: direction = C * r for iter > 1, else C * g;
+  * GRAD_METHOD 2, store_LBFGS_SH.c:39-120: waveconv *= C, y/s pairs of
+    [rho; u] in the slot of LBFGS_pointer; LBFGS.c:26-95 two-loop in slot
+    order (orc_lbfgs); extract_LBFGS_SH.c:31-64: direction = C * r for
+    iter > 1, else C * g;
+  * GRAD_METHOD 1, denise.c:1850-1870: descent.c negates the gradient,
+    store_PCG_SH.c stacks [u; rho], PCG.c forms the Polak-Ribiere direction
+    per class (orc_pcg, beta reset at < 0), extract_PCG_SH.c and descent.c
+    give it back as a gradient, the negated gradient is kept as PCG_old;
   * line_search.c:60-380: trial models by calc_mat_change_test.c (itest 1)
     at eps_scale, forward of the test shots, the step1/step2/step3 state
     machine, calc_opt_step.c:13-109 for the final step;
@@ -56,7 +61,7 @@ class FwiOracle:
         o = dict(nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1, testshot_end=1,
                  testshot_incr=1, vs_lower=0.0, vs_upper=1e9, rho_lower=0.0, rho_upper=1e9, inv_vs_iter=0,
                  inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0, spatfilter=0, wd_damp=0.0, wd_damp1=0.0,
-                 fc_end=0.0, taper_vert=0, gradt=(0, 0, 0, 0))
+                 fc_end=0.0, taper_vert=0, gradt=(0, 0, 0, 0), grad_method=2)
         o.update(opts)
         self.o = o
         cfg.c_vs = float(float_mean(self.vs))
@@ -69,6 +74,8 @@ class FwiOracle:
         self.s = np.zeros((NL, 2 * n), np.float32)
         self.ptr = 1
         self.iter = 1
+        self.pcg_old = np.zeros((2, n), np.float32)      # [u; rho], zero_PCG
+        self.pcg_dir = np.zeros((2, n), np.float32)
         self.min_iter_help = 0           # denise.c:138
         self.L2_hist = {}
         self.gold = self.mold = None
@@ -173,6 +180,13 @@ class FwiOracle:
             kw = dict(gradt=o["gradt"], src_i=self.src_i, src_j=self.src_j, rec_i=self.rec_i, rec_j=self.rec_j)
             gvs, _ = O.taper_grad(gvs, 1, self.cfg.dh, **kw)
             grho, _ = O.taper_grad(grho, 1, self.cfg.dh, **kw)
+        if o["grad_method"] == 1:
+            new = np.stack([-gvs.ravel(), -grho.ravel()]).astype(np.float32)
+            self.pcg_dir = np.asarray(O.pcg(new, self.pcg_old, self.pcg_dir), np.float32)
+            self.pcg_old = new
+            du = (-self.pcg_dir[0]).reshape(self.vs.shape)
+            dr = (-self.pcg_dir[1]).reshape(self.vs.shape)
+            return self._finish(du, dr, L2t, epst1, energy)
         # store_LBFGS_SH.c: normalisation
         gu = (self.C_vs * gvs).astype(np.float32).ravel()
         gr = (self.C_rho * grho).astype(np.float32).ravel()
@@ -189,6 +203,11 @@ class FwiOracle:
             gr = (np.asarray(r[:n], np.float32) * self.C_rho).astype(np.float32)
             gu = (np.asarray(r[n:], np.float32) * self.C_vs).astype(np.float32)
         du, dr = gu.reshape(self.vs.shape), gr.reshape(self.vs.shape)
+        return self._finish(du, dr, L2t, epst1, energy)
+
+    def _finish(self, du, dr, L2t, epst1, energy):
+        """Line search, model update and the stage rule of one iteration."""
+        o = self.o
         eps_scale, step1, step3 = self._line_search(du, dr, L2t, epst1, energy)
         if self.iter > self.min_iter_help and step1 == 0:
             eps_scale = np.float32(0.0)
@@ -207,5 +226,11 @@ class FwiOracle:
                     self.min_iter_help = self.iter + o["min_iter"]
                     self.iter = 0
                     rec["next_stage"] = 1
+                    # denise.c:1998-2005: zero_PCG, or zero_LBFGS and LBFGS_pointer = 1
+                    self.pcg_old[:] = 0.0
+                    self.pcg_dir[:] = 0.0
+                    self.y[:] = 0.0
+                    self.s[:] = 0.0
+                    self.ptr = 1
         self.iter += 1
         return rec

@@ -1,7 +1,7 @@
 """The device FWI loop (dn_fwi_iteration: gradient, L-BFGS, line search,
 model update, stage rule) against the oracle's restatement of denise.c's
 iteration (oracle/fwi_oracle.py) on the same inputs, with and without the
-gradient post-processing of denise.c:1818-1847.
+gradient post-processing of denise.c:1818-1847, with L-BFGS and with PCG.
 
 The first iteration is bit-identical.  From the second on, the product's
 FP64 L-BFGS dot products and the reference's (and the oracle's) float ones
@@ -30,6 +30,7 @@ def say(*a):
 
 CASES = [
     ("viscoelastic L-BFGS vs+rho", dict(L=2, opts=dict(inv_rho_iter=0))),
+    ("viscoelastic PCG vs+rho", dict(L=2, opts=dict(inv_rho_iter=0, grad_method=1))),
     ("elastic L-BFGS with smoothing and taper",
      dict(L=0, opts=dict(inv_rho_iter=1000, spatfilter=4, wd_damp=0.25, wd_damp1=0.25, fc_end=40.0,
                          taper_vert=1, gradt=(3, 10, 86, 96)))),
