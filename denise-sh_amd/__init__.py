@@ -314,7 +314,7 @@ class Context:
         self._chk(lib().dn_set_wave_shots(self._h, n))
 
     def set_group_layout(self, mode):
-        """0: workgroups of four adjacent strips (default); 1: of one strip of four shots."""
+        """0: workgroups of eight adjacent strips (default); 1: of one strip of eight shots."""
         lib().dn_set_group_layout.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_set_group_layout(self._h, mode))
 

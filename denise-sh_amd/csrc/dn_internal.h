@@ -8,7 +8,7 @@
 namespace dn {
 
 constexpr int kWave = 64;      // CDNA wavefront width: one wave per strip
-constexpr int kStripsPerGroup = 4;   // waves (adjacent strips) per workgroup of the time step
+constexpr int kStripsPerGroup = 8;   // waves (adjacent strips) per workgroup of the time step
 constexpr int kMaxL = 4;       // relaxation mechanisms supported
 // Zero frame around the interior of every padded field.  The fused step
 // reads up to 2K-1 rows above/below the interior, 2K-1 columns left of it

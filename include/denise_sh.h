@@ -224,7 +224,7 @@ int dn_set_exec(dn_ctx *ctx, int graph_steps);
  * wave).  Results are identical; dn_autotune picks the faster. */
 int dn_set_wave_shots(dn_ctx *ctx, int shots_per_wave);
 /* Workgroup layout of the time-step kernel (one shot per wave): 0 (default)
- * four adjacent strips of one shot; 1 the same strip of four consecutive
+ * eight adjacent strips of one shot; 1 the same strip of eight consecutive
  * shots of the batch, whose loads of the static model fields meet in the
  * compute unit's cache.  Results are identical; dn_autotune picks the
  * faster.  No counterpart in the reference (a launch-shape choice). */
