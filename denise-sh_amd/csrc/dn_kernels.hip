@@ -96,8 +96,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // The waves of a workgroup step through the rows together (one
     // s_barrier per row).  Strip groups (default): they take adjacent strips
     // of one shot, so their loads of a row arrive at the memory as one
-    // contiguous run per field; spare waves of the last group re-read the
-    // last strip and store nothing.  Shot groups (one shot per wave only):
+    // contiguous run per field; spare waves of the last group only keep the
+    // row barriers.  Shot groups (one shot per wave only):
     // they take the same strip of consecutive shots, so their loads of the
     // static model rows meet in the CU's L1; a wave without an active shot
     // fetches only those rows and stores nothing.
@@ -255,6 +255,12 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     float sq[NSH][K + 1];   // old sxy at the lane's column, slot t <-> row jv-K+t
     const int jstart = j0 - K + 1;
     const int jend = j1 + K;
+    if (!live) {
+        // a spare wave of the last strip group: it only keeps the workgroup's
+        // per-row barriers (one per iteration of the row loop below)
+        for (int jv = jstart; jv <= jend; ++jv) __builtin_amdgcn_s_barrier();
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < NSH; ++u) {
 #pragma unroll
