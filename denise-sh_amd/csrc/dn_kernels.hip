@@ -75,14 +75,14 @@ __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_
 template <int K, int L, bool ADJ, int EX, int NSH>
 __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs a)
 {
-    // EX: this launch may store snapshots / sum the pseudo-Hessian (forward)
-    // or correlate (adjoint); the launches without (most steps) carry none
-    // of that code and its pointers.  NSH: shots per wave.  With two, a wave
-    // advances the same strip of two shots of the batch: the static model
-    // fields and the receiver table are loaded once for both, and the two
-    // shots' independent arithmetic hides each other's latency.
-    // EX: 0 none; 1 with FP32 snapshots (and every forward EX launch); 2
-    // adjoint correlation with 16-bit snapshots (decoded where used).
+    // EX: 0 for the plain steps (most steps), which carry none of the
+    // snapshot code and its pointers; 1 stores snapshots / sums the
+    // pseudo-Hessian (forward) or correlates with FP32 snapshots (adjoint);
+    // 2 correlates with 16-bit snapshots, decoded where used (adjoint).
+    // NSH: shots per wave.  With two, a wave advances the same strip of two
+    // shots of the batch: the static model fields and the receiver table are
+    // loaded once for both, and the two shots' independent arithmetic hides
+    // each other's latency.
     static_assert(NSH == 1 || (NSH == 2 && !EX), "two shots per wave only on steps without EX work");
     static_assert(EX >= 0 && EX <= 2 && (EX < 2 || ADJ), "EX variant");
     constexpr bool S16 = ADJ && EX == 2;
