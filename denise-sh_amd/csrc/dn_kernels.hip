@@ -409,6 +409,9 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src[u] = true;
             }
     }
+    // per-snapshot max |x| of the values this wave stages for the 16-bit
+    // codec (store_mode 2), kept while they are written
+    float smx0 = 0.0f, smx1 = 0.0f, smx2 = 0.0f;
     // one iteration: velocity row jv and stress row jv - K, from inputs
     // fetched earlier (cm, cr)
     auto row = [&](const int jv, const RowM &cm, const RowS (&cr)[NSH]) {
@@ -460,10 +463,12 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
             if (EX && !ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
-                if (a.store_mode == 1)
+                if (a.store_mode == 1) {
                     snw[ci] = acc;
-                else
+                } else {
                     stg[ci] = acc;
+                    smx0 = fmaxf(smx0, fabsf(acc));
+                }
             }
             if (EX && ADJ && corr && own_v) {
                 // vyp1 = old vy (update_v_PML.c:189); denise.c:1589
@@ -585,6 +590,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                             float *b0 = stg + ci;
                             b0[nxny] = vyy;
                             b0[2 * nxny] = vyx;
+                            smx1 = fmaxf(smx1, fabsf(vyy));
+                            smx2 = fmaxf(smx2, fabsf(vyx));
                         }
                         wab[ci] = cr[u].wa + vc * vc;   // eprecond.c:11
                     }
@@ -630,20 +637,10 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
 
     // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
     if (EX && !ADJ && fstore && a.store_mode == 2 && live) {
-        // recomputed from the staged values this wave wrote (owned cells only)
-        float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
-        if (own_x) {
-            const float *st = a.stage + (long long)s * 3 * nxny;
-            for (int j = j0; j <= j1; ++j) {
-                const long long ci = (long long)(j - 1) * g.nx + (c - 1);
-                m0 = fmaxf(m0, fabsf(st[ci]));
-                m1 = fmaxf(m1, fabsf(st[nxny + ci]));
-                m2 = fmaxf(m2, fabsf(st[2 * nxny + ci]));
-            }
-        }
-        m0 = wave_max(m0);
-        m1 = wave_max(m1);
-        m2 = wave_max(m2);
+        // the maxima over the cells this wave staged (owned cells only)
+        const float m0 = wave_max(smx0);
+        const float m1 = wave_max(smx1);
+        const float m2 = wave_max(smx2);
         if (lane == 0) {
             atomicMax(&a.stage_max[s * 3 + 0], __float_as_uint(m0));
             atomicMax(&a.stage_max[s * 3 + 1], __float_as_uint(m1));
@@ -1032,9 +1029,19 @@ __global__ void k_snap_pack(int nxny, const float *stage, const unsigned int *st
     if (blockIdx.x == 0 && threadIdx.x == 0) snap_scale[((long long)s * nsnap + slot) * 3 + f] = dec;
     const float *src = stage + ((long long)s * 3 + f) * nxny;
     uint16_t *dst = snap16 + (long long)s * shot_stride16 + ((long long)slot * 3 + f) * nxny;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x) {
-        const float x = src[t] * enc;
-        dst[t] = (kind == 1) ? f2bf16(x) : f2h(x);
+    auto code = [&](float v) -> unsigned { return (kind == 1) ? f2bf16(v * enc) : f2h(v * enc); };
+    if ((nxny & 3) == 0) {
+        // four values per lane: 16-byte loads and 8-byte stores (every plane
+        // starts at a multiple of 4 elements of 16-byte aligned buffers)
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        uint2 *d4 = reinterpret_cast<uint2 *>(dst);
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < (nxny >> 2); t += gridDim.x * blockDim.x) {
+            const float4 x = s4[t];
+            d4[t] = make_uint2(code(x.x) | (code(x.y) << 16), code(x.z) | (code(x.w) << 16));
+        }
+    } else {
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x)
+            dst[t] = (uint16_t)code(src[t]);
     }
 }
 
@@ -1043,7 +1050,8 @@ hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, cons
                             const int *nt_base, int nt, int dtinv, int kind, hipStream_t st)
 {
     const int nxny = nx * ny;
-    hipLaunchKernelGGL(k_snap_pack, dim3(std::min(1024, (nxny + 255) / 256), 3, nshots), dim3(256), 0, st, nxny,
+    const int per = (nxny & 3) == 0 ? nxny / 4 : nxny;   // work items per plane
+    hipLaunchKernelGGL(k_snap_pack, dim3(std::min(1024, (per + 255) / 256), 3, nshots), dim3(256), 0, st, nxny,
                        stage, stage_max, snap16, snap_scale, shot_stride16, nsnap, nt_base, nt, dtinv, kind);
     return hipGetLastError();
 }
