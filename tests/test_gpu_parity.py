@@ -209,10 +209,13 @@ def test_autotuner_keeps_results():
 
 
 @pytest.mark.parametrize("kind,tol", [(1, 1e-2), (2, 2e-3)], ids=["bf16", "fp16"])
-def test_reduced_precision_snapshots(kind, tol):
+@pytest.mark.parametrize("nx,ny", [(100, 44), (97, 43)], ids=["grid4400", "grid4171"])
+def test_reduced_precision_snapshots(kind, tol, nx, ny):
     """Gradient with BF16 / FP16 snapshots (per-snapshot max-abs scaling, FP32
-    accumulation): error against FP32 storage and the FP64 oracle stated."""
-    nx, ny, nt = 100, 44, 800
+    accumulation): error against FP32 storage and the FP64 oracle stated.  A
+    plane of 4400 cells is encoded four values per lane, one of 4171 cells
+    one value per lane."""
+    nt = 800
     say("case snapshots", "bf16" if kind == 1 else "fp16")
     ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=2, dtinv=2)
     obs = _obs_for(ctx, cfg, d, nx, ny, 2)
