@@ -443,3 +443,40 @@ def test_segment_heights_give_identical_results():
         say("  rows %d: equal %s" % (rows, all(np.array_equal(x, y) for x, y in zip(a, b))))
         assert all(np.array_equal(x, y) for x, y in zip(a, b))
     ctx.close()
+
+
+@pytest.mark.parametrize("L", [0, 2])
+def test_cpml_with_k_max_above_one(L):
+    """K_MAX_PML = 2 (CPML stretching K > 1, divided through in the CPML
+    cells; every other test uses the default K = 1), top CPML on:
+    seismograms and gradients against the FP32 oracle."""
+    from pyoracle import make_cfg
+    nx, ny, nt, fc = 110, 48, 400, 60.0
+    dn = product()
+    vs, rho, qs = model(nx, ny, DH, seed=8, qs=20.0 if L else None)
+    fl = [6.25, 50.0][:L]
+    kw = dict(fdorder=8, maxrelerror=1, fw=10, free_surf=0, L=L, dt=DT, dh=DH, damping=300.0, fpml=fc, dtinv=2)
+    ctx = dn.Context(nx, ny, fl=fl, k_max_pml=2.0, **kw)
+    cfg = make_cfg(nx, ny, nt=nt, fl=fl, k_max_pml=2.0, **kw)
+    rec_i = np.arange(13, nx - 12, 3, dtype=np.int32)
+    rec_j = np.full_like(rec_i, 5)
+    src_i = np.array([30, 80], np.int32)
+    src_j = np.array([6, 40], np.int32)
+    sig = np.stack([ricker(nt, DT, fc) for _ in range(2)])
+    ctx.set_receivers(rec_i, rec_j)
+    ctx.set_shots(src_i, src_j, sig)
+    o32 = oracle("f32")
+    obs = np.stack([o32.forward(cfg, vs * np.float32(1.05), rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k],
+                                rec_i, rec_j)[0] for k in range(2)]).astype(np.float32)
+    ctx.set_model(vs, rho, qs)
+    seis = ctx.forward()
+    ref = np.stack([o32.forward(cfg, vs, rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k], rec_i, rec_j)[0]
+                    for k in range(2)])
+    ctx.set_observed(obs)
+    ctx.gradient()
+    gvs, grho, _, _ = ctx.get_gradient()
+    r = o32.gradient(cfg, vs, rho, qs, src_i, src_j, sig, rec_i, rec_j, obs)
+    say("  L %d: seismograms rel L2 %.2e, grad_vs rel L2 %.2e" % (L, rel_l2(seis, ref), rel_l2(gvs, r["grad_vs"])))
+    assert np.array_equal(seis, ref)
+    assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
+    ctx.close()
