@@ -174,6 +174,7 @@ def main():
     ap.add_argument("--wave-shots", type=int, default=1, choices=[1, 2],
                     help="shots per wavefront on the plain steps (--autotune chooses)")
     ap.add_argument("--rows", type=int, default=0, help="rows per segment of the step kernel (0: automatic)")
+    ap.add_argument("--max-batch", type=int, default=0, help="most shots advanced together (0: from free HBM)")
     ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
                     help="workgroups of adjacent strips (0) or of the shots of one strip (1)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
@@ -197,8 +198,8 @@ def main():
                      snap_precision=prec)
     ctx.set_exec(-1 if args.exec == "graph" else 0)
     ctx.set_wave_shots(args.wave_shots)
-    if args.rows:
-        ctx.set_batch(0, args.rows)
+    if args.rows or args.max_batch:
+        ctx.set_batch(args.max_batch, args.rows)
     ctx.set_group_layout(args.layout)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
