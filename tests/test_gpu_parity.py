@@ -480,3 +480,54 @@ def test_cpml_with_k_max_above_one(L):
     assert np.array_equal(seis, ref)
     assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
     ctx.close()
+
+
+# nx, ny, fdorder, L, free_surf, fw: grids smaller than a strip group or a
+# segment, rows fewer than the stencil reach, no absorbing frame at all
+EDGE_GRIDS = [
+    (20, 12, 8, 2, 1, 3),
+    (65, 9, 4, 0, 0, 2),
+    (130, 16, 12, 1, 1, 4),
+    (57, 57, 2, 0, 1, 0),
+    (71, 33, 6, 2, 0, 0),
+]
+
+
+@pytest.mark.parametrize("case", EDGE_GRIDS, ids=["%dx%d_fd%d_L%d_fs%d_fw%d" % c for c in EDGE_GRIDS])
+def test_edge_grids_match_oracle(case):
+    """Seismograms and gradients bit for bit on small and awkward grids, with
+    receivers on the first and last interior rows and columns and a source
+    in a corner cell."""
+    from pyoracle import make_cfg
+    nx, ny, fdorder, L, fs, fw = case
+    nt, fc = 200, 60.0
+    dn = product()
+    vs, rho, qs = model(nx, ny, DH, seed=nx + ny, qs=20.0 if L else None)
+    fl = [6.25, 50.0, 12.0][:L]
+    kw = dict(fdorder=fdorder, maxrelerror=1, fw=fw, free_surf=fs, L=L, dt=DT, dh=DH, damping=300.0, fpml=fc,
+              dtinv=2)
+    ctx = dn.Context(nx, ny, fl=fl, **kw)
+    cfg = make_cfg(nx, ny, nt=nt, fl=fl, **kw)
+    rec_i = np.array([1, nx, nx // 2, 1, nx], np.int32)
+    rec_j = np.array([1, 1, ny // 2, ny, ny], np.int32)
+    src_i = np.array([1, nx // 2], np.int32)
+    src_j = np.array([1, max(1, ny // 3)], np.int32)
+    sig = np.stack([ricker(nt, DT, fc) for _ in range(2)])
+    ctx.set_receivers(rec_i, rec_j)
+    ctx.set_shots(src_i, src_j, sig)
+    o32 = oracle("f32")
+    obs = np.stack([o32.forward(cfg, vs * np.float32(1.05), rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k],
+                                rec_i, rec_j)[0] for k in range(2)]).astype(np.float32)
+    ref = np.stack([o32.forward(cfg, vs, rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k], rec_i, rec_j)[0]
+                    for k in range(2)])
+    ctx.set_model(vs, rho, qs)
+    seis = ctx.forward()
+    ctx.set_observed(obs)
+    ctx.gradient()
+    gvs, grho, _, _ = ctx.get_gradient()
+    r = o32.gradient(cfg, vs, rho, qs, src_i, src_j, sig, rec_i, rec_j, obs)
+    say("  %s: seismograms rel L2 %.2e, grad_vs rel L2 %.2e" % (case, rel_l2(seis, ref), rel_l2(gvs, r["grad_vs"])))
+    assert np.all(np.isfinite(seis)) and np.max(np.abs(ref)) > 0
+    assert np.array_equal(seis, ref)
+    assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
+    ctx.close()
