@@ -272,6 +272,8 @@ def main():
         out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
     elif args.rows:
         out["config"]["rows_per_segment"] = args.rows
+    if args.max_batch:
+        out["config"]["max_batch"] = args.max_batch
     out["config"]["exec"] = args.exec
     out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
     out["config"]["group_layout"] = "tuned" if args.autotune else ("shots" if args.layout else "strips")
