@@ -72,9 +72,8 @@ class Survey:
 
     def setup(self, vs_true, vs_start, rho, qs):
         """Observed data from the true model; the context is left at the
-        starting model (which also fixes the normalisation means C_vs, C_rho
-        to the starting model, as the reference's first iteration does)."""
-        self.ctx.set_model(vs_start, rho, qs)
+        starting model (the normalisation means C_vs, C_rho are taken at the
+        first gradient or at dn_fwi_start, from the starting model)."""
         self.ctx.set_model(vs_true, rho, qs)
         obs = self.ctx.forward()
         self.ctx.set_observed(obs)
