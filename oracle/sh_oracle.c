@@ -1126,8 +1126,10 @@ int F(orc_adjoint)(const orc_cfg *c, const float *vs, const float *rho, const fl
  * Pinned (with orc_adjoint) by central differences of the misfit on smooth
  * bumps with EPRECOND = 0: the density correlation equals c*dJ/drho|mu and
  * the shear correlation c*DTF*mu*dJ/dmu (DTF = DT elastic, 1 visco) with one
- * constant c, within 8% / 25%; eprecond1 and the C_vs/C_rho means are
- * restated from denise.c and pinned only through the GPU parity.
+ * constant c, within 8% / 25%.  eprecond1 and the C_vs/C_rho means are
+ * pinned by the ratio of the EPRECOND 0 and 1 gradients: (mean vs / mean
+ * rho)^2 between the parameters, the relative floor, invariance under a
+ * joint scaling of signals and data, and per-shot weights.
  * Full gradient over nshots (denise.c:746-816, 875-1694, 1743-1813):
  *   C_vs, C_rho = float means of the model (i outer, j inner, Q14);
  *   per shot: forward, residual, adjoint, eprecond1 (Q16), scaling;

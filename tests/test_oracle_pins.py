@@ -23,6 +23,10 @@ What pins what:
                    approximation of exp(-t/tau_sigma).
   residual/energy  time reversal identity and closed-form energy of a
                    constant trace.
+  eprecond1/means  EPRECOND 0 / EPRECOND 1 ratio fields: (mean vs / mean rho)^2
+                   between the parameters, the 0.005 max floor, invariance
+                   under a joint scaling of signals and data (square root),
+                   per-shot weights (two shots = sum of one-shot gradients).
   lbfgs/pcg/opt    secant condition and textbook two-loop agreement;
                    A-conjugacy of PR directions with exact line search;
                    vertex of an exact parabola; relative step rule.
@@ -624,6 +628,72 @@ def test_gradient_is_misfit_derivative(L):
     assert np.all(np.abs(r[:, 0] / cst - 1) < 0.08)
     assert np.all((r[:, 1] / cst > 0.75) & (r[:, 1] / cst < 1.12))
     np.testing.assert_allclose(r[:, 2], r[:, 1], rtol=0.02)
+
+
+def test_eprecond1_scaling_and_normalisation_means():
+    """eprecond1.c:15-70 and denise.c:746-819, 1671-1694 by what they fix,
+    from the oracle's EPRECOND 0 and 1 gradients of the same shots:
+
+      per shot  g1 = g0 / (We * C^2),  We = sqrt(Ws Wr) + 0.005 max sqrt(Ws Wr),
+                C_vs, C_rho = the means of the vs and rho models;
+
+    so for one shot g0_u / g1_u = We C_vs^2 and g0_rho / g1_rho = We C_rho^2:
+    (a) their ratio is (mean vs / mean rho)^2 in every cell (one We for both
+        parameters, the means of the right models, not swapped);
+    (b) We = g0_u / (g1_u C_vs^2) is nowhere below the floor
+        0.005 / 1.005 * max We (the regularisation is relative to the maximum);
+    (c) source signals and observed data scaled together by a = 2 (exact in
+        floating point) scale every field energy and the correlation by a^2:
+        the EPRECOND 1 gradient stays the same only if We is of degree one in
+        Ws Wr (the square root);
+    (d) the two-shot gradient is the sum of the one-shot gradients (We is
+        taken per shot, not from energies summed over shots)."""
+    nx, ny, nt, dh, dt = 60, 40, 500, 0.25, 1e-4
+    kw = dict(nx=nx, ny=ny, fdorder=4, maxrelerror=1, fw=10, free_surf=1, L=0, nt=nt, dtinv=2, dt=dt, dh=dh,
+              damping=300.0, fpml=30.0)
+    c0, c1 = cfg(eprecond=0, **kw), cfg(eprecond=1, **kw)
+    rng = np.random.default_rng(7)
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    vs = (180.0 + 60.0 * jj / ny + 8.0 * rng.standard_normal((ny, nx))).astype(np.float32)
+    rho = (1500.0 + 400.0 * ii / nx + 30.0 * rng.standard_normal((ny, nx))).astype(np.float32)
+    t = np.arange(1, nt + 1) * dt
+    tau = np.pi * (t - 1.5 / 30.0) / (1.5 / 30.0)
+    w = ((1 - 4 * tau ** 2) * np.exp(-2 * tau ** 2)).astype(np.float32)
+    si, sj = np.array([20, 41], np.int32), np.array([3, 3], np.int32)
+    ri = np.arange(12, 50, 3, dtype=np.int32)
+    rj = np.full_like(ri, 2)
+    vt = vs.copy()
+    vt[12:22, 22:40] *= np.float32(1.07)
+    obs = np.stack([O64.forward(c0, vt, rho, None, si[k:k + 1], sj[k:k + 1], w, ri, rj)[0] for k in range(2)])
+    obs = obs.astype(np.float32)
+    sig = np.stack([w, w])
+
+    def grad(c, k, a=1.0):
+        s = slice(k, k + 1) if k is not None else slice(0, 2)
+        return O64.gradient(c, vs, rho, None, si[s], sj[s], (a * sig[s]).astype(np.float32), ri, rj,
+                            (a * obs[s]).astype(np.float32))
+
+    g0, g1 = grad(c0, 0), grad(c1, 0)
+    cvs, crho = float(np.mean(vs.astype(np.float64))), float(np.mean(rho.astype(np.float64)))
+    m = (np.abs(g0["raw_u"]) > 1e-6 * np.abs(g0["raw_u"]).max()) & \
+        (np.abs(g0["raw_rho"]) > 1e-6 * np.abs(g0["raw_rho"]).max())
+    ru = g0["raw_u"][m] / g1["raw_u"][m]
+    rr = g0["raw_rho"][m] / g1["raw_rho"][m]
+    print("cells %d, C_vs %.6g, C_rho %.6g" % (m.sum(), cvs, crho))
+    np.testing.assert_allclose(ru / rr, (cvs / crho) ** 2, rtol=1e-9)                    # (a)
+    We = ru / cvs ** 2
+    floor = 0.005 / 1.005 * We.max()
+    print("min We / floor = %.6f" % (We.min() / floor))
+    assert We.min() >= floor * (1 - 1e-9)                                                 # (b)
+    g1a = grad(c1, 0, a=2.0)
+    g0a = grad(c0, 0, a=2.0)
+    assert rel_l2(g0a["raw_u"], 4.0 * g0["raw_u"]) < 1e-13
+    for key in ("raw_u", "raw_rho", "grad_vs", "grad_rho"):                               # (c)
+        assert rel_l2(g1a[key], g1[key]) < 1e-13, key
+    g12 = grad(c1, None)                                                                  # (d)
+    g1b = grad(c1, 1)
+    for key in ("raw_u", "raw_rho", "grad_vs", "grad_rho"):
+        assert rel_l2(g12[key], g1[key] + g1b[key]) < 1e-12, key
 
 
 def test_pseudo_hessian_is_sum_of_squared_snapshots():
