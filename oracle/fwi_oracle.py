@@ -33,8 +33,12 @@ and x-major in the reference; the two-loop result does not depend on the
 order beyond rounding.
 
 Parity: pinned through its parts (each oracle function is pinned in
-tests/test_oracle_pins.py); the composition itself is parity unpinned
-against the reference, whose FWI loop needs MPI and cannot be built here.
+tests/test_oracle_pins.py) and, as a composition, by
+tests/test_fwi_oracle_pins.py: iteration 1 steps along minus the normalised
+gradient, iteration 2 along the one-pair BFGS (or Polak-Ribiere) direction
+built from independently computed gradients, and the misfit falls.  It is
+not run against the reference itself, whose FWI loop needs MPI and cannot
+be built here.
 """
 import ctypes as C
 
