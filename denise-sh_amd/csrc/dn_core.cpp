@@ -81,6 +81,14 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
         dn_fail(nullptr, "dn_create: the CPML frames of width FW=%d overlap on a %dx%d grid", p->fw, p->nx, p->ny);
         return nullptr;
     }
+    {
+        // the fused step addresses one shot's fields with 32-bit byte offsets
+        const Geo t = make_geo(p->nx, p->ny, p->fw, p->free_surf, p->L);
+        if ((double)t.plane * 4.0 * (double)std::max(4, 2 * std::max(1, p->L)) >= 2147483648.0) {
+            dn_fail(nullptr, "dn_create: a %dx%d grid is too large for one device context", p->nx, p->ny);
+            return nullptr;
+        }
+    }
     if (p->dtinv < 1) {
         dn_fail(nullptr, "dn_create: DTINV must be >= 1");
         return nullptr;
@@ -141,6 +149,7 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
               dalloc(&c->d_tausipjp, P) == hipSuccess && dalloc(&c->d_fipjp, P) == hipSuccess &&
               dalloc(&c->d_f, P) == hipSuccess && dalloc(&c->d_dip, P * std::max(1, p->L)) == hipSuccess &&
               dalloc(&c->d_d, P * std::max(1, p->L)) == hipSuccess &&
+              dalloc(&c->d_spk, spk_floats(P, p->L)) == hipSuccess &&
               dalloc(&c->d_gu, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_grho, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_gvs_out, (size_t)p->nx * p->ny) == hipSuccess &&
@@ -157,6 +166,7 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
     for (float *z : zs) (void)launch_fill(z, P, 0.0f, c->stream);
     (void)launch_fill(c->d_dip, P * std::max(1, p->L), 0.0f, c->stream);
     (void)launch_fill(c->d_d, P * std::max(1, p->L), 0.0f, c->stream);
+    (void)launch_fill(c->d_spk, spk_floats(P, p->L), 0.0f, c->stream);
     (void)launch_fill(c->d_gu, (size_t)p->nx * p->ny, 0.0f, c->stream);
     (void)launch_fill(c->d_grho, (size_t)p->nx * p->ny, 0.0f, c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -205,6 +215,7 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_pml);
     dfree_t(c->d_vs); dfree_t(c->d_rho); dfree_t(c->d_taus); dfree_t(c->d_uip); dfree_t(c->d_ujp);
     dfree_t(c->d_b); dfree_t(c->d_tausipjp); dfree_t(c->d_fipjp); dfree_t(c->d_f); dfree_t(c->d_dip);
+    dfree_t(c->d_spk);
     dfree_t(c->d_d); dfree_t(c->d_gu); dfree_t(c->d_grho); dfree_t(c->d_gvs_out); dfree_t(c->d_grho_out);
     dfree_t(c->d_maxbits);
     dfree_t(c->d_trace_flags);
@@ -255,6 +266,7 @@ int dn_prepare_model(dn_ctx *c)
     a.fcoef = c->d_f;
     a.dip = c->d_dip;
     a.dcoef = c->d_d;
+    a.spk = c->d_spk;
     a.sumu = c->sumu;
     for (int l = 0; l < kMaxL; ++l) a.eta[l] = (l < c->p.L) ? c->eta[l] : 0.0f;
     HIPCHK(c, launch_model_prep(a, c->stream));
@@ -707,6 +719,7 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a.fcoef = c->d_f;
     a.dip = c->d_dip;
     a.dcoef = c->d_d;
+    a.spk = c->d_spk;
     for (int l = 0; l < kMaxL; ++l) {
         a.bip[l] = c->bip[l];
         a.bjm[l] = c->bjm[l];

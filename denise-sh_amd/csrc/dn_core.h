@@ -31,6 +31,7 @@ struct dn_ctx {
     // model (padded)
     float *d_vs = nullptr, *d_rho = nullptr, *d_taus = nullptr, *d_uip = nullptr, *d_ujp = nullptr, *d_b = nullptr;
     float *d_tausipjp = nullptr, *d_fipjp = nullptr, *d_f = nullptr, *d_dip = nullptr, *d_d = nullptr;
+    float *d_spk = nullptr;   // the stress coefficients packed per cell (see StepArgs::spk)
     float C_vs = 0.0f, C_rho = 0.0f;
     bool have_model = false;
     bool C_fixed = false;            // C_vs, C_rho taken (first gradient or dn_fwi_start)

@@ -48,6 +48,9 @@ inline Geo make_geo(int nx, int ny, int fw, int free_surf, int L)
     return g;
 }
 
+// floats of the packed stress-coefficient planes (StepArgs::spk)
+inline size_t spk_floats(size_t plane, int L) { return L > 0 ? plane * (size_t)(4 + 2 * (L - 1)) : plane * 2; }
+
 __host__ __device__ inline long long gidx(const Geo &g, int j, int i)
 {
     return (long long)(j - 1 + g.yoff) * g.pitch + (long long)(i - 1 + g.xoff);
@@ -82,6 +85,11 @@ struct StepArgs {
     const float *__restrict__ fcoef;
     const float *__restrict__ dip;      // [L][plane]
     const float *__restrict__ dcoef;    // [L][plane]
+    // the same stress coefficients packed per cell, read by the fused step
+    // with one wide load instead of one load per field: elastic one float2
+    // plane {uip, ujp}; viscoelastic one float4 plane {fipjp, f, dip[0],
+    // d[0]} followed by L-1 float2 planes {dip[l], d[l]}
+    const float *__restrict__ spk;
     float bip[kMaxL], bjm[kMaxL], cip[kMaxL], cjm[kMaxL];
     // per-shot dynamic state; shot stride = plane (L*plane for r, q)
     const float *vy_in, *sxy_in, *syz_in;
@@ -131,6 +139,7 @@ struct ModelPrepArgs {
     float *vs, *rho, *taus;                 // padded, halo filled by prep
     float *uip, *ujp, *tausipjp, *bfield;
     float *fipjp, *fcoef, *dip, *dcoef;
+    float *spk;                             // packed copy (StepArgs::spk)
     float sumu;                             // prepare_update_s.c:30-33 (host)
     float eta[kMaxL];                       // readmod.c:33 (host)
 };

@@ -33,6 +33,14 @@ __device__ __forceinline__ float ldf(const float *base, unsigned boff)
     return *(const float *)((const char *)base + boff);
 }
 __device__ __forceinline__ void stf(float *base, unsigned boff, float v) { *(float *)((char *)base + boff) = v; }
+__device__ __forceinline__ float2 ld2(const float *base, unsigned boff)
+{
+    return *(const float2 *)((const char *)base + boff);
+}
+__device__ __forceinline__ float4 ld4(const float *base, unsigned boff)
+{
+    return *(const float4 *)((const char *)base + boff);
+}
 
 __device__ __forceinline__ uint16_t f2bf16(float x)
 {
@@ -319,16 +327,20 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         m.rrow = (jv >= 1 && jv <= g.ny) ? a.rec_row[jv] : -1;
         if (!sneed) {
         } else if (!VISCO) {
-            m.s1 = ldf(a.uip, gs);
-            m.s2 = ldf(a.ujp, gs);
+            const float2 t = ld2(a.spk, gs * 2u);
+            m.s1 = t.x;
+            m.s2 = t.y;
         } else {
-            m.s1 = ldf(a.fipjp, gs);
-            m.s2 = ldf(a.fcoef, gs);
+            const float4 t = ld4(a.spk, gs * 4u);
+            m.s1 = t.x;
+            m.s2 = t.y;
+            m.dp[0] = t.z;
+            m.dd[0] = t.w;
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-                const unsigned o4 = gs + (unsigned)(l * plane) * 4u;
-                m.dp[l] = ldf(a.dip, o4);
-                m.dd[l] = ldf(a.dcoef, o4);
+            for (int l = 1; l < L; ++l) {
+                const float2 u = ld2(a.spk + 4 * g.plane + 2 * (long long)(l - 1) * g.plane, gs * 2u);
+                m.dp[l] = u.x;
+                m.dd[l] = u.y;
             }
         }
         const int ykv = ykof(jv), yks = sneed ? ykof(jsr) : 0;
@@ -780,6 +792,10 @@ __global__ void k_model_coef(ModelPrepArgs a)
     if ((u1 == 0.0f) || (u3 == 0.0f)) ujp = 0.0f;
     a.uip[o] = uip;
     a.ujp[o] = ujp;
+    if (a.L == 0) {
+        a.spk[2 * o] = uip;
+        a.spk[2 * o + 1] = ujp;
+    }
     // update_v_PML.c:181-182
     const float rv = a.rho[o];
     float b = (float)(1.0 / (double)rv);
@@ -794,11 +810,19 @@ __global__ void k_model_coef(ModelPrepArgs a)
         // prepare_update_s.c:45-57
         const float mujp = (float)((double)ujp / (1.0 + (double)(a.sumu * ptaus)));
         const float muip = (float)((double)uip / (1.0 + (double)(a.sumu * tip)));
-        a.fipjp[o] = (float)((double)(muip * a.DT) * (1.0 + (double)((float)a.L * tip)));
-        a.fcoef[o] = (float)((double)(mujp * a.DT) * (1.0 + (double)((float)a.L * ptaus)));
+        const float fipjp = (float)((double)(muip * a.DT) * (1.0 + (double)((float)a.L * tip)));
+        const float f = (float)((double)(mujp * a.DT) * (1.0 + (double)((float)a.L * ptaus)));
+        a.fipjp[o] = fipjp;
+        a.fcoef[o] = f;
+        a.spk[4 * o] = fipjp;
+        a.spk[4 * o + 1] = f;
         for (int l = 0; l < a.L; ++l) {
-            a.dip[(long long)l * g.plane + o] = muip * a.eta[l] * tip;
-            a.dcoef[(long long)l * g.plane + o] = mujp * a.eta[l] * ptaus;
+            const float dp = muip * a.eta[l] * tip, dd = mujp * a.eta[l] * ptaus;
+            a.dip[(long long)l * g.plane + o] = dp;
+            a.dcoef[(long long)l * g.plane + o] = dd;
+            float *pk = (l == 0) ? a.spk + 4 * o + 2 : a.spk + 4 * g.plane + 2 * ((long long)(l - 1) * g.plane + o);
+            pk[0] = dp;
+            pk[1] = dd;
         }
     }
 }
