@@ -84,7 +84,7 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
     {
         // the fused step addresses one shot's fields with 32-bit byte offsets
         const Geo t = make_geo(p->nx, p->ny, p->fw, p->free_surf, p->L);
-        if ((double)t.plane * 4.0 * (double)std::max(4, 2 * std::max(1, p->L)) >= 2147483648.0) {
+        if ((double)t.plane * (double)(16 + 8 * p->L) >= 2147483648.0) {
             dn_fail(nullptr, "dn_create: a %dx%d grid is too large for one device context", p->nx, p->ny);
             return nullptr;
         }
@@ -195,7 +195,7 @@ static void free_batch(dn_ctx *c)
         dfree_t(c->d_psx[k]);
         dfree_t(c->d_psy[k]);
     }
-    dfree_t(c->d_r); dfree_t(c->d_q); dfree_t(c->d_pvx); dfree_t(c->d_pvy);
+    dfree_t(c->d_rq); dfree_t(c->d_pvx); dfree_t(c->d_pvy);
     dfree_t(c->d_seis); dfree_t(c->d_resid); dfree_t(c->d_snap); dfree_t(c->d_snap16); dfree_t(c->d_snap_scale);
     dfree_t(c->d_stage); dfree_t(c->d_stage_max);
     dfree_t(c->d_Ws); dfree_t(c->d_Wr); dfree_t(c->d_gu_shot); dfree_t(c->d_grho_shot); dfree_t(c->d_part);
@@ -627,8 +627,7 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
         HIPCHK(c, dalloc(&c->d_psx[k], S * fw2 * g.ny));
         HIPCHK(c, dalloc(&c->d_psy[k], S * fw2 * g.nx));
     }
-    HIPCHK(c, dalloc(&c->d_r, S * P * std::max(1, g.L)));
-    HIPCHK(c, dalloc(&c->d_q, S * P * std::max(1, g.L)));
+    HIPCHK(c, dalloc(&c->d_rq, S * P * 2 * std::max(1, g.L)));
     HIPCHK(c, dalloc(&c->d_pvx, S * fw2 * g.ny));
     HIPCHK(c, dalloc(&c->d_pvy, S * fw2 * g.nx));
     HIPCHK(c, dalloc(&c->d_seis, S * c->ntr * c->nt_max));
@@ -726,8 +725,7 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
         a.cip[l] = c->cip[l];
         a.cjm[l] = c->cjm[l];
     }
-    a.r = c->d_r;
-    a.q = c->d_q;
+    a.rq = c->d_rq;
     a.pvx = c->d_pvx;
     a.pvy = c->d_pvy;
     a.shot_map = c->d_shot_map;
@@ -755,8 +753,7 @@ int dn_zero_state(dn_ctx *c, int n)
         HIPCHK(c, launch_fill(c->d_psy[k], S * fw2 * g.nx, 0.0f, c->stream));
     }
     if (g.L > 0) {
-        HIPCHK(c, launch_fill(c->d_r, S * P * g.L, 0.0f, c->stream));
-        HIPCHK(c, launch_fill(c->d_q, S * P * g.L, 0.0f, c->stream));
+        HIPCHK(c, launch_fill(c->d_rq, S * P * 2 * g.L, 0.0f, c->stream));
     }
     HIPCHK(c, launch_fill(c->d_pvx, S * fw2 * g.ny, 0.0f, c->stream));
     HIPCHK(c, launch_fill(c->d_pvy, S * fw2 * g.nx, 0.0f, c->stream));

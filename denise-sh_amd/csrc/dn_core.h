@@ -52,7 +52,7 @@ struct dn_ctx {
     bool batch_has_snap = false;
     float *d_vy[2] = {nullptr, nullptr}, *d_sxy[2] = {nullptr, nullptr}, *d_syz[2] = {nullptr, nullptr};
     float *d_psx[2] = {nullptr, nullptr}, *d_psy[2] = {nullptr, nullptr};
-    float *d_r = nullptr, *d_q = nullptr, *d_pvx = nullptr, *d_pvy = nullptr;
+    float *d_rq = nullptr, *d_pvx = nullptr, *d_pvy = nullptr;
     float *d_seis = nullptr, *d_resid = nullptr, *d_snap = nullptr;
     uint16_t *d_snap16 = nullptr;
     float *d_snap_scale = nullptr, *d_stage = nullptr;

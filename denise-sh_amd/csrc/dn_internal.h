@@ -91,10 +91,14 @@ struct StepArgs {
     // d[0]} followed by L-1 float2 planes {dip[l], d[l]}
     const float *__restrict__ spk;
     float bip[kMaxL], bjm[kMaxL], cip[kMaxL], cjm[kMaxL];
-    // per-shot dynamic state; shot stride = plane (L*plane for r, q)
+    // per-shot dynamic state; shot stride = plane (2*L*plane for rq)
     const float *vy_in, *sxy_in, *syz_in;
     float *vy_out, *sxy_out, *syz_out;
-    float *r, *q;
+    // the relaxation memory variables r[L], q[L] of a shot, interleaved per
+    // cell so that one wide access moves a cell's pairs: float4 planes
+    // {r[l], q[l], r[l+1], q[l+1]} for l = 0, 2, ..., then a float2 plane
+    // {r[L-1], q[L-1]} when L is odd
+    float *rq;
     const float *psx_in;  float *psx_out;   // psi_sxy_x [s][ny][2fw] (ping-pong)
     const float *psy_in;  float *psy_out;   // psi_syy_y [s][2fw][nx] (ping-pong)
     float *pvx, *pvy;                       // psi_vyx [s][ny][2fw], psi_vyy [s][2fw][nx]

@@ -33,6 +33,8 @@ __device__ __forceinline__ float ldf(const float *base, unsigned boff)
     return *(const float *)((const char *)base + boff);
 }
 __device__ __forceinline__ void stf(float *base, unsigned boff, float v) { *(float *)((char *)base + boff) = v; }
+__device__ __forceinline__ void st2(float *base, unsigned boff, float2 v) { *(float2 *)((char *)base + boff) = v; }
+__device__ __forceinline__ void st4(float *base, unsigned boff, float4 v) { *(float4 *)((char *)base + boff) = v; }
 __device__ __forceinline__ float2 ld2(const float *base, unsigned boff)
 {
     return *(const float2 *)((const char *)base + boff);
@@ -186,8 +188,10 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     float *__restrict__ vyo = a.vy_out + s0 * g.plane;
     float *__restrict__ sxo_ = a.sxy_out + s0 * g.plane;
     float *__restrict__ szo = a.syz_out + s0 * g.plane;
-    float *__restrict__ rb = VISCO ? a.r + s0 * L * g.plane : nullptr;
-    float *__restrict__ qb = VISCO ? a.q + s0 * L * g.plane : nullptr;
+    // r, q interleaved: float4 planes of pairs (l, l+1), then a float2 plane
+    // when L is odd; rqb(k) is the base of plane k
+    float *__restrict__ rqb = VISCO ? a.rq + s0 * 2 * L * g.plane : nullptr;
+    auto rqp = [&](int k) -> float * { return rqb + (long long)k * 4 * g.plane; };
     const float *__restrict__ pxi = a.psx_in + s0 * g.ny * fw2;
     float *__restrict__ pxo = a.psx_out + s0 * g.ny * fw2;
     const float *__restrict__ pyi = a.psy_in + s0 * fw2 * g.nx;
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
 #pragma unroll
     for (int u = 0; u < NSH; ++u) {
         sof[u] = (unsigned)u * (unsigned)plane * 4u;
-        sor[u] = (unsigned)u * (unsigned)(L > 0 ? L : 1) * (unsigned)plane * 4u;
+        sor[u] = (unsigned)u * (unsigned)(L > 0 ? 2 * L : 1) * (unsigned)plane * 4u;
         spx[u] = (unsigned)u * (unsigned)(g.ny * fw2) * 4u;
         spy[u] = (unsigned)u * (unsigned)(fw2 * g.nx) * 4u;
     }
@@ -371,10 +375,17 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             if (yks > 0 && in_x) o[u].qy = ldf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u);
             if (VISCO && sneed) {
 #pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    const unsigned o4 = sor[u] + gs + (unsigned)(l * plane) * 4u;
-                    o[u].r[l] = ldf(rb, o4);
-                    o[u].q[l] = ldf(qb, o4);
+                for (int l = 0; l + 1 < L; l += 2) {
+                    const float4 t = ld4(rqp(l / 2), sor[u] + gs * 4u);
+                    o[u].r[l] = t.x;
+                    o[u].q[l] = t.y;
+                    o[u].r[l + 1] = t.z;
+                    o[u].q[l + 1] = t.w;
+                }
+                if (L & 1) {
+                    const float2 t = ld2(rqp(L / 2), sor[u] + gs * 2u);
+                    o[u].r[L - 1] = t.x;
+                    o[u].q[L - 1] = t.y;
                 }
             }
         }
@@ -564,7 +575,6 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 } else {
                     float rr[L > 0 ? L : 1], qq[L > 0 ? L : 1];
                     float sumr = 0.0f, sumq = 0.0f;
-                    const unsigned gs4 = sor[u] + gs * 4u;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
                         rr[l] = cr[u].r[l];
@@ -580,11 +590,13 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                     for (int l = 0; l < L; ++l) {
                         rr[l] = a.bip[l] * (rr[l] * a.cip[l] - (cm.dp[l] * vyx));
                         qq[l] = a.bjm[l] * (qq[l] * a.cjm[l] - (cm.dd[l] * vyy));
-                        stf(rb, gs4 + (unsigned)(l * plane) * 4u, rr[l]);
-                        stf(qb, gs4 + (unsigned)(l * plane) * 4u, qq[l]);
                         sumr += rr[l];
                         sumq += qq[l];
                     }
+#pragma unroll
+                    for (int l = 0; l + 1 < L; l += 2)
+                        st4(rqp(l / 2), sor[u] + gs * 16u, make_float4(rr[l], qq[l], rr[l + 1], qq[l + 1]));
+                    if (L & 1) st2(rqp(L / 2), sor[u] + gs * 8u, make_float2(rr[L - 1], qq[L - 1]));
                     sxn = sxn + (a.dthalbe * sumr);
                     syn = syn + (a.dthalbe * sumq);
                 }
