@@ -152,7 +152,10 @@ typedef struct dn_params {
     int eprecond;        /* EPRECOND: 1 energy preconditioning (eprecond1.c), 0 none */
 } dn_params;
 
-/* Context on the current HIP device; NULL on failure (dn_last_error(NULL)). */
+/* Context on the current HIP device; NULL on failure (dn_last_error(NULL)).
+ * The fused step addresses one shot's fields with 32-bit byte offsets: one
+ * padded plane ((NX+80 rounded up to 32) x (NY+24) floats) times 16 + 8L
+ * bytes must stay below 2 GiB (about 7 500 x 7 500 cells at L = 2). */
 dn_ctx *dn_create(const dn_params *p);
 void dn_destroy(dn_ctx *ctx);
 /* Error text of the last failing call on ctx (or of dn_create when NULL). */
