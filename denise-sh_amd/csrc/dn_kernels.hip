@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     float *__restrict__ sxo_ = a.sxy_out + s0 * g.plane;
     float *__restrict__ szo = a.syz_out + s0 * g.plane;
     // r, q interleaved: float4 planes of pairs (l, l+1), then a float2 plane
-    // when L is odd; rqb(k) is the base of plane k
+    // when L is odd; rqp(k) is the base of plane k
     float *__restrict__ rqb = VISCO ? a.rq + s0 * 2 * L * g.plane : nullptr;
     auto rqp = [&](int k) -> float * { return rqb + (long long)k * 4 * g.plane; };
     const float *__restrict__ pxi = a.psx_in + s0 * g.ny * fw2;
