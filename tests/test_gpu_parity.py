@@ -331,11 +331,15 @@ def test_gradient_after_forward_on_same_shots():
         assert np.array_equal(a, b)
 
 
-def test_two_shots_per_wave_equals_one():
+@pytest.mark.parametrize("L", [0, 2, 3])
+def test_two_shots_per_wave_equals_one(L):
     """dn_set_wave_shots(2) and (1) (the default) give identical seismograms
-    and gradients, with an odd shot count and uneven record lengths."""
+    and gradients, with an odd shot count and uneven record lengths; elastic
+    (packed float2 coefficients), L = 2 (float4 memory variables) and odd L
+    (the float2 tail plane of the second shot of a wave)."""
     nx, ny, nt = 110, 48, 300
-    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=3, nts=[300, 260, 290])
+    say("  L %d" % L)
+    ctx, cfg, d = setup_case(nx, ny, 8, L, 1, 10, nt, 60.0, nshots=3, nts=[300, 260, 290])
     obs = _obs_for(ctx, cfg, d, nx, ny, 3)
     for s, n in enumerate([300, 260, 290]):
         obs[s, :, n:] = 0.0
