@@ -1,23 +1,35 @@
 """Benchmark of the SH adjoint-gradient hot path on one MI355X.
 
-Workload (viscoelastic FWI configuration): NX=1200, NY=300, DH=0.2 m,
-FDORDER=8 (Holberg, MAXRELERROR 1), L=2 relaxation mechanisms (Q=20,
-FL = 6.25/50 Hz), 20-point CPML, free surface on top, DT=1e-4 s, NT=10000
-(1 s records), 40 shots along the surface, 25 Hz Ricker, 240 receivers at
-1 m spacing, snapshots every DTINV=4 steps.  True model: vs gradient
-100->350 m/s with a +-10% checkerboard of 8 m cells; the gradient is taken
-at the 1-D starting model; rho 1700->2000 kg/m^3.  Observed data are
-synthesised on the device from the true model before timing (synthetic,
-seeded).
+Two workloads (viscoelastic SH, FDORDER 8 Holberg with MAXRELERROR 1, L = 2
+relaxation mechanisms with Q = 20 and FL = 6.25 / 50 Hz, 20-point CPML on
+the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
 
-One step = one misfit gradient over all 40 shots: forward modelling with
-snapshot storage, residuals, adjoint modelling with the zero-lag
-correlation, energy preconditioning and the chain rule.  The metric counts
-cell-updates of the forward and the adjoint sweeps (NX*NY*NT*2 per shot).
+* ``c4`` (default, the headline): NX = 4000, NY = 800, DH = 0.1 m,
+  DT = 5e-5 s, NT = 12 000; vs in four layers 80 / 150 / 250 / 400 m/s with
+  twelve seeded Gaussian lenses (+-15 %, sigma 3 m) in the true model, the
+  plain layers as the model the gradient is taken at; rho 1800 -> 2000;
+  32 shots evenly spread from 10 m to 390 m along the row j = 5 (0.5 m
+  deep), 60 Hz Ricker, 394
+  receivers every 1 m on the same row.  A shot stores 3000 snapshots of
+  three fields (115 GB in FP32), so HBM holds two shots at a time: one step
+  is the gradient of one such batch (forward with snapshot storage,
+  residuals, adjoint with correlation, energy preconditioning, chain rule),
+  the steps walk through the 16 batches of the 32 shots in turn.
+* ``c2``: NX = 1200, NY = 300, DH = 0.2 m, DT = 1e-4 s, NT = 10 000; vs
+  100 -> 350 m/s with depth times a +-10 % checkerboard of 8 m cells (true)
+  and the plain gradient (start); rho 1700 -> 2000; 40 shots, 25 Hz Ricker,
+  240 receivers at 1 m spacing.  One step is the gradient over all 40
+  shots (batches of as many shots as fit in HBM).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT]
-       [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu] [--autotune]
-       python bench.py --copy-probe      (copy kernel that scales HBM counters)
+Observed data are synthesised on the device from the true model before
+timing (synthetic, seeded).  The metric counts cell-updates of the forward
+and adjoint sweeps (NX*NY*NT*2 per shot).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2]
+       [--nt NT] [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu]
+       [--rows R] [--max-batch B] [--autotune] [--exec graph|eager]
+       python bench.py --config c2 --fwi N     (N plain L-BFGS iterations, wall time)
+       python bench.py --copy-probe            (copy kernel that scales HBM counters)
 """
 import argparse
 import json
@@ -43,31 +55,85 @@ def _package():
     return mod
 
 
-def workload(nt):
-    nx, ny, dh = 1200, 300, 0.2
+def layered_lenses(nx, ny, dh, seed):
+    """vs layers 80 / 150 / 250 / 400 m/s with depth and twelve seeded
+    Gaussian lenses (+-15 %, sigma 3 m) in the true model; the plain layers
+    as the starting model; rho 1800 -> 2000; Q 20."""
+    j = np.arange(ny)[:, None]
+    i = np.arange(nx)[None, :]
+    layer = np.minimum(4 * j // ny, 3)
+    vs0 = np.array([80.0, 150.0, 250.0, 400.0])[layer] + 0.0 * i
+    rng = np.random.default_rng(seed)
+    pert = np.zeros((ny, nx))
+    for _ in range(12):
+        cx, cy = rng.uniform(0, nx * dh), rng.uniform(0.2 * ny * dh, ny * dh)
+        pert += rng.choice([-1.0, 1.0]) * 0.15 * np.exp(-(((i * dh - cx) ** 2 + (j * dh - cy) ** 2) / (2 * 3.0 ** 2)))
+    rho = (1800.0 + 200.0 * j / (ny - 1) + 0.0 * i).astype(np.float32)
+    return (vs0 * (1 + pert)).astype(np.float32), vs0.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)
+
+
+def checkerboard(nx, ny, dh):
+    """vs 100 -> 350 m/s with depth times a +-10 % checkerboard of 8 m cells
+    (true) and the plain gradient (start); rho 1700 -> 2000; Q 20."""
     j = np.arange(ny, dtype=np.float64)[:, None]
     i = np.arange(nx, dtype=np.float64)[None, :]
     vs1d = 100.0 + 250.0 * j / (ny - 1) + 0.0 * i
-    cell = 8.0 / dh  # 8 m checkerboard cells
+    cell = 8.0 / dh
     cb = np.where(((np.floor(i / cell) + np.floor(j / cell)) % 2) == 0, 1.0, -1.0)
     vs_true = (vs1d * (1.0 + 0.1 * cb)).astype(np.float32)
-    vs_start = vs1d.astype(np.float32)
     rho = (1700.0 + 300.0 * j / (ny - 1) + 0.0 * i).astype(np.float32)
-    qs = np.full((ny, nx), 20.0, np.float32)
-    nshots = 40
-    src_x = np.linspace(3.0, nx * dh - 3.0, nshots)
+    return vs_true, vs1d.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)
+
+
+def workload(config="c4", nt=None):
+    """Model, acquisition and solver settings of a workload (see the module
+    docstring).  nt overrides the record length."""
+    if config == "c2":
+        nx, ny, dh, dt = 1200, 300, 0.2, 1e-4
+        vs_true, vs_start, rho, qs = checkerboard(nx, ny, dh)
+        nshots, nrec, fc = 40, 240, 25.0
+        src_x = np.linspace(3.0, nx * dh - 3.0, nshots)
+        rec_x = 0.5 + np.arange(nrec) * 1.0
+        row = int(np.floor(0.5 / dh + 0.5))
+        nt = 10000 if nt is None else nt
+        damping = 350.0
+    elif config == "c4":
+        nx, ny, dh, dt = 4000, 800, 0.1, 5e-5
+        vs_true, vs_start, rho, qs = layered_lenses(nx, ny, dh, seed=5)
+        nshots, nrec, fc = 32, 394, 60.0
+        src_x = np.linspace(10.0, nx * dh - 10.0, nshots)
+        rec_x = 3.0 + np.arange(nrec) * 1.0
+        row = 5
+        nt = 12000 if nt is None else nt
+        damping = 350.0
+    else:
+        raise ValueError("unknown workload %r" % config)
     src_i = np.floor(src_x / dh + 0.5).astype(np.int32)          # iround(x/DH), splitsrc.c
-    src_j = np.full(nshots, int(np.floor(0.5 / dh + 0.5)), np.int32)
-    rec_x = 0.5 + np.arange(240) * 1.0
+    src_j = np.full(nshots, row, np.int32)
     rec_i = np.floor(rec_x / dh + 0.5).astype(np.int32)          # receiver.c:40
-    rec_j = np.full(240, int(np.floor(0.5 / dh + 0.5)), np.int32)
-    return dict(nx=nx, ny=ny, dh=dh, dt=1e-4, nt=nt, vs_true=vs_true, vs_start=vs_start, rho=rho, qs=qs,
-                src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j, fdorder=8, L=2, fl=[6.25, 50.0], fw=20,
-                fc=25.0, dtinv=4)
+    rec_j = np.full(nrec, row, np.int32)
+    return dict(config=config, nx=nx, ny=ny, dh=dh, dt=dt, nt=nt, vs_true=vs_true, vs_start=vs_start, rho=rho,
+                qs=qs, src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j, fdorder=8, L=2, fl=[6.25, 50.0], fw=20,
+                fc=fc, dtinv=4, damping=damping)
+
+
+def describe(W, shots, snap):
+    return ("visco SH gradient %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d, %s snapshots"
+            % (W["nx"], W["ny"], shots, W["nt"], W["dtinv"], snap))
 
 
 def ricker(dn, nt, dt, fc):
     return dn.wavelet(1, nt, dt, 0.0, fc, 1.0)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def cpu_baseline(W, seconds):
@@ -80,7 +146,7 @@ def cpu_baseline(W, seconds):
 
     def run(nt_sample):
         cfg = make_cfg(W["nx"], W["ny"], fdorder=W["fdorder"], fw=W["fw"], free_surf=1, L=W["L"], nt=nt_sample,
-                       dtinv=W["dtinv"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], fl=W["fl"])
+                       dtinv=W["dtinv"], dt=W["dt"], dh=W["dh"], damping=W["damping"], fpml=W["fc"], fl=W["fl"])
         sig = o.wavelet(1, nt_sample, W["dt"], 0.0, W["fc"], 1.0).astype(np.float32)[None]
         obs = np.zeros((1, len(W["rec_i"]), nt_sample), np.float32)
         t0 = time.perf_counter()
@@ -91,26 +157,35 @@ def cpu_baseline(W, seconds):
     probe = 40
     t = run(probe)
     nt_sample = int(min(W["nt"], max(probe, probe * seconds / max(t, 1e-3))))
+    nt_sample -= nt_sample % W["dtinv"]
     sec = run(nt_sample)
     cells = 2.0 * W["nx"] * W["ny"] * nt_sample
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=cells / sec / 1e9, unit="Gcell-updates/s", cores=threads, kind="port",
-                sample="1 of %d shots, forward+adjoint gradient over %d of %d steps on the same grid (%.1f s)"
-                       % (len(W["src_i"]), nt_sample, W["nt"], sec))
+                sample="1 of %d shots, forward+adjoint gradient over %d of %d steps on the same %dx%d grid "
+                       "(%.1f s, %d OpenMP threads on %s)"
+                       % (len(W["src_i"]), nt_sample, W["nt"], W["nx"], W["ny"], sec, threads, cpu_model()))
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_visco_gradient_1200x300_hbm_traffic.md")
+def traffic_file(W):
+    return os.path.join(ROOT, "profiles", "k_step_visco_gradient_%dx%d_hbm_traffic.md" % (W["nx"], W["ny"]))
 
 
-def measured_traffic(cells_per_launch):
-    """HBM bytes per launch of the step kernel on this workload (mean over
-    the forward and adjoint launches of a gradient), from the FETCH_SIZE /
-    WRITE_SIZE profile (scaled by the copy-probe factors) kept in profiles/:
-    bytes per cell-update x cell-updates per launch."""
+def measured_traffic(W, batch, cells_per_launch):
+    """HBM bytes per launch of the step kernel (mean over the forward and
+    adjoint launches of a gradient), from the FETCH_SIZE / WRITE_SIZE
+    profile of this grid and batch shape kept in profiles/ (scaled by the
+    copy-probe factors): bytes per cell-update x cell-updates per launch.
+    None when no profile of this shape exists."""
     try:
-        for line in open(TRAFFIC_FILE):
+        per, shots = None, None
+        for line in open(traffic_file(W)):
             if line.startswith("traffic_bytes_per_cell_update_gradient:"):
-                return round(float(line.split(":")[1]) * cells_per_launch)
+                per = float(line.split(":")[1])
+            if line.startswith("shots_per_launch:"):
+                shots = int(line.split(":")[1])
+        if per is not None and (shots is None or shots == batch):
+            return round(per * cells_per_launch)
     except OSError:
         pass
     return None
@@ -118,11 +193,12 @@ def measured_traffic(cells_per_launch):
 
 def fwi_run(ctx, W, args):
     """Plain L-BFGS FWI of the workload (GRAD_METHOD 2, NLBFGS 20, EPRECOND 1,
-    vs only, test shots 1, 11, 21, 31 for the line search) from the 1-D
-    starting model: wall time of the iterations and the misfit after each,
-    with the distance of the model to the true one.  The library's calls
-    return with their work done (they synchronise their own stream)."""
-    ctx.fwi_start(testshot_start=1, testshot_end=args.shots, testshot_incr=10, eps_scale=0.01, stepmax=6,
+    vs only, test shots 1, 11, 21, 31 for the line search) from the starting
+    model: wall time of the iterations and the misfit after each, with the
+    distance of the model to the true one.  The library's calls return with
+    their work done (they synchronise their own stream)."""
+    nshots = len(W["src_i"]) if args.shots <= 0 else args.shots
+    ctx.fwi_start(testshot_start=1, testshot_end=nshots, testshot_incr=10, eps_scale=0.01, stepmax=6,
                   vs_lower=50.0, vs_upper=600.0)
     hist = []
     t0 = time.perf_counter()
@@ -142,8 +218,8 @@ def fwi_run(ctx, W, args):
                       "misfit_L2t": [h["misfit"] for h in hist], "steps": [h["step"] for h in hist],
                       "test_forwards": [h["test_forwards"] for h in hist],
                       "vs_rel_error": {"start": round(err0, 5), "end": round(err, 5)},
-                      "config": {"workload": "visco SH FWI 1200x300 FDORDER8 L2, %d shots x %d steps, DTINV %d"
-                                             % (args.shots, args.nt, W["dtinv"]), "n_gpus": 1}}))
+                      "config": {"workload": "visco SH FWI %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d"
+                                             % (W["nx"], W["ny"], nshots, W["nt"], W["dtinv"]), "n_gpus": 1}}))
 
 
 def copy_probe(nbytes=1 << 30, reps=4):
@@ -162,10 +238,11 @@ def copy_probe(nbytes=1 << 30, reps=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nt", type=int, default=10000)
-    ap.add_argument("--shots", type=int, default=40)
+    ap.add_argument("--config", default="c4", choices=["c4", "c2"])
+    ap.add_argument("--nt", type=int, default=0, help="record length in steps (0: the workload's)")
+    ap.add_argument("--shots", type=int, default=0, help="shots of the acquisition used (0: all)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -191,10 +268,11 @@ def main():
         sys.exit(3)
 
     dn = _package()
-    W = workload(args.nt)
+    W = workload(args.config, args.nt or None)
+    nshots = len(W["src_i"]) if args.shots <= 0 else min(args.shots, len(W["src_i"]))
     prec = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snap]
     ctx = dn.Context(W["nx"], W["ny"], fdorder=W["fdorder"], maxrelerror=1, fw=W["fw"], free_surf=1, L=W["L"],
-                     fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=350.0, fpml=W["fc"], dtinv=W["dtinv"],
+                     fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=W["damping"], fpml=W["fc"], dtinv=W["dtinv"],
                      snap_precision=prec)
     ctx.set_exec(-1 if args.exec == "graph" else 0)
     ctx.set_wave_shots(args.wave_shots)
@@ -202,28 +280,41 @@ def main():
         ctx.set_batch(args.max_batch, args.rows)
     ctx.set_group_layout(args.layout)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
-    sig = np.tile(ricker(dn, args.nt, W["dt"], W["fc"]), (args.shots, 1))
-    ctx.set_shots(W["src_i"][:args.shots], W["src_j"][:args.shots], sig)
+    sig = np.tile(ricker(dn, W["nt"], W["dt"], W["fc"]), (nshots, 1))
+    ctx.set_shots(W["src_i"][:nshots], W["src_j"][:nshots], sig)
     # observed data from the true model (untimed setup, on the device)
+    print("bench: %s, observed data from the true model" % describe(W, nshots, args.snap), file=sys.stderr,
+          flush=True)
     ctx.set_model(W["vs_true"], W["rho"], W["qs"])
     obs = ctx.forward()
-    ctx.set_observed(obs)
     ctx.set_model(W["vs_start"], W["rho"], W["qs"])
+    ctx.set_observed(obs)
+    del obs
     if args.fwi:
         fwi_run(ctx, W, args)
         ctx.close()
         return
     tuned = None
     if args.autotune:
-        tuned = ctx.autotune(args.shots, min(args.nt, 256))
-    for _ in range(args.warmup):
-        ctx.gradient()
+        tuned = ctx.autotune(nshots, min(W["nt"], 256))
+    # one step: the gradient of one batch (c4) or of all shots (c2)
+    if args.config == "c4":
+        batch = ctx.batch_shots(True)
+        batches = [(s0, min(batch, nshots - s0)) for s0 in range(0, nshots, batch)]
+    else:
+        batch = ctx.batch_shots(True)
+        batches = [(0, nshots)]
+    print("bench: %d shots per batch, %d step(s) per pass over the shots" % (batch, len(batches)), file=sys.stderr,
+          flush=True)
+    for k in range(args.warmup):
+        ctx.gradient(*batches[k % len(batches)])
     ms = {0: 0.0, 1: 0.0}
     la = {0: 0, 1: 0}
     ce = {0: 0, 1: 0}
+    l2 = 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        l2, en = ctx.gradient()
+    for k in range(args.steps):
+        l2, en = ctx.gradient(*batches[(args.warmup + k) % len(batches)])
         for p in (0, 1):
             m, l, c = ctx.pass_timing(p)
             ms[p] += m
@@ -244,7 +335,8 @@ def main():
     b_adj = base + (36.0 - (6.0 if prec != 0 else 0.0)) / W["dtinv"]
     bytes_per_cell = (ce[0] * b_fwd + ce[1] * b_adj) / max(ce[0] + ce[1], 1)
     achieved = units_per_launch * bytes_per_cell / (per_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(units_per_launch) if (prec == 0 and args.nt >= 1000) else None
+    full = W["nt"] == (12000 if args.config == "c4" else 10000)
+    traffic = measured_traffic(W, batch, units_per_launch) if (prec == 0 and full) else None
     out = {
         "metric": "Gcell-updates/s fwd+adj",
         "value": round(value, 3),
@@ -257,10 +349,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded checkerboard true model, observed data modelled on the device)",
-        "config": {"workload": "visco SH gradient 1200x300 FDORDER8 L2, %d shots x %d steps, DTINV %d, %s snapshots"
-                               % (args.shots, args.nt, W["dtinv"], args.snap),
-                   "nx": W["nx"], "ny": W["ny"], "shots": args.shots, "nt": args.nt, "fdorder": 8, "L": 2},
+        "data": "synthetic (seeded %s true model, observed data modelled on the device)"
+                % ("layered-lens" if args.config == "c4" else "checkerboard"),
+        "config": {"workload": describe(W, nshots, args.snap), "nx": W["nx"], "ny": W["ny"], "shots": nshots,
+                   "nt": W["nt"], "fdorder": 8, "L": 2, "dtinv": W["dtinv"], "shots_per_batch": batch,
+                   "step": ("gradient of one %d-shot batch" % batch) if args.config == "c4"
+                   else "gradient over all %d shots" % nshots},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step<4,visco> (forward and adjoint launches of the gradient)",

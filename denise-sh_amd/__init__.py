@@ -28,7 +28,7 @@ EXPORTS = [
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
     "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
-    "dn_set_group_layout",
+    "dn_set_group_layout", "dn_batch_shots",
 ]
 
 
@@ -128,6 +128,7 @@ def lib():
         L.dn_set_shots.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, fp]
         L.dn_set_observed.argtypes = [C.c_void_p, fp]
         L.dn_set_batch.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.dn_batch_shots.argtypes = [C.c_void_p, C.c_int, ip]
         L.dn_forward.argtypes = [C.c_void_p, C.c_int, C.c_int, fp]
         L.dn_gradient.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
@@ -325,6 +326,12 @@ class Context:
 
     def set_batch(self, max_shots=0, rows_per_segment=0):
         self._chk(lib().dn_set_batch(self._h, max_shots, rows_per_segment))
+
+    def batch_shots(self, gradient=True):
+        """Shots advanced together in one batch (allocates the batch workspace)."""
+        n = C.c_int()
+        self._chk(lib().dn_batch_shots(self._h, 1 if gradient else 0, C.byref(n)))
+        return n.value
 
     def forward(self, shot0=0, nshots=None, want_seis=True):
         nshots = self.nshots - shot0 if nshots is None else nshots

@@ -363,7 +363,8 @@ extern "C" int dn_set_receivers(dn_ctx *c, int ntr, const int *ri, const int *rj
     c->rec_j.assign(rj, rj + ntr);
     c->dc = dn_ctx::DataCond{};   // conditioning refers to receivers: reset
     c->dc_dirty = true;
-    free_batch(c);  // seismogram buffers depend on ntr
+    dfree_t(c->d_obs);  // observed data are [shot][ntr][nt_max]: set them again
+    free_batch(c);      // seismogram buffers depend on ntr
     return 0;
 }
 
@@ -443,6 +444,14 @@ extern "C" int dn_set_batch(dn_ctx *c, int max_shots, int rows_per_segment)
     c->batch_req = std::max(0, max_shots);
     c->rows_req = std::max(0, rows_per_segment);
     free_batch(c);
+    return 0;
+}
+
+extern "C" int dn_batch_shots(dn_ctx *c, int gradient, int *nshots)
+{
+    if (!c || !nshots) return dn_fail(c, "dn_batch_shots: NULL argument");
+    if (dn_ensure_batch(c, c->nshots, gradient != 0)) return -1;
+    *nshots = std::min(c->batch_cap, c->nshots);
     return 0;
 }
 

@@ -197,6 +197,12 @@ int dn_set_observed(dn_ctx *ctx, const float *obs);
  * and rows per workgroup segment of the time-step kernel (0 = automatic). */
 int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
 
+/* Shots one batch advances together (gradient != 0: with snapshot storage),
+ * sized from free HBM and the dn_set_batch cap; the batch workspace is
+ * allocated by this call.  The reference runs one shot at a time per rank
+ * (denise.c:875); no counterpart. */
+int dn_batch_shots(dn_ctx *ctx, int gradient, int *nshots);
+
 /* Source- and receiver-side energy maps of the last gradient, summed over
  * its shots: sum_s Ws, sum_s Wr (eprecond.c), host [ny][nx]. */
 int dn_get_energy_maps(dn_ctx *ctx, float *ws, float *wr);

@@ -103,7 +103,7 @@ def test_visco_1200x300_one_shot_full_record():
     sys.path.insert(0, ROOT)
     import bench
     dn = product()
-    W = bench.workload(10000)
+    W = bench.workload("c2", 10000)
     nx, ny, nt = W["nx"], W["ny"], W["nt"]
     k = 20                                                            # a shot in the middle of the line
     src_i, src_j = W["src_i"][k:k + 1], W["src_j"][k:k + 1]
@@ -185,26 +185,12 @@ def test_batched_uneven_stages_2000x400_equals_one_at_a_time():
     ctx.close()
 
 
-def _layered_lenses(nx, ny, dh, seed):
-    """vs layers 80 / 150 / 250 / 400 m/s with depth and seeded Gaussian
-    lenses (+-15 %, sigma 3 m) in the true model; rho 1800 -> 2000; Q 20."""
-    j = np.arange(ny)[:, None]
-    i = np.arange(nx)[None, :]
-    layer = np.minimum(4 * j // ny, 3)
-    vs0 = np.array([80.0, 150.0, 250.0, 400.0])[layer] + 0.0 * i
-    rng = np.random.default_rng(seed)
-    pert = np.zeros((ny, nx))
-    for _ in range(12):
-        cx, cy = rng.uniform(0, nx * dh), rng.uniform(0.2 * ny * dh, ny * dh)
-        pert += rng.choice([-1.0, 1.0]) * 0.15 * np.exp(-(((i * dh - cx) ** 2 + (j * dh - cy) ** 2) / (2 * 3.0 ** 2)))
-    rho = (1800.0 + 200.0 * j / (ny - 1) + 0.0 * i).astype(np.float32)
-    return (vs0 * (1 + pert)).astype(np.float32), vs0.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)
-
-
 def test_reduced_precision_snapshots_4000x800():
+    sys.path.insert(0, ROOT)
+    import bench
     dn = product()
     nx, ny, dh, dt, nt, fc = 4000, 800, 0.1, 5e-5, 12000, 60.0
-    vs_true, vs0, rho, qs = _layered_lenses(nx, ny, dh, seed=5)
+    vs_true, vs0, rho, qs = bench.layered_lenses(nx, ny, dh, seed=5)
     ns = 2
     src_i = np.array([1000, 3000], np.int32)
     src_j = np.full(ns, 5, np.int32)
