@@ -28,7 +28,9 @@ EXPORTS = [
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
     "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
-    "dn_set_group_layout", "dn_batch_shots",
+    "dn_set_group_layout", "dn_batch_shots", "dn_calc_res",
+    "dn_set_time_filt", "zero_fdveps", "zero_fdveps_visc", "dn_dropin_resident",
+    "dn_dropin_sync", "dn_fwi_save_state", "dn_fwi_load_state",
 ]
 
 
@@ -76,7 +78,8 @@ class StfOpts(C.Structure):
 
 class FwiFiles(C.Structure):
     _fields_ = [("misfit_log", C.c_char_p), ("time_filt", C.c_int), ("fc_start", C.c_float), ("fc", C.c_float),
-                ("jacobian", C.c_char_p), ("inv_modelfile", C.c_char_p), ("write_it", C.c_int)]
+                ("jacobian", C.c_char_p), ("inv_modelfile", C.c_char_p), ("write_it", C.c_int),
+                ("lbfgs_old", C.c_int)]
 
 
 class FwiOpts(C.Structure):
@@ -87,7 +90,8 @@ class FwiOpts(C.Structure):
                 ("inv_rho_iter", C.c_int), ("scalerho", C.c_float), ("min_iter", C.c_int), ("pro", C.c_float),
                 ("spatfilter", C.c_int), ("wd_damp", C.c_float), ("wd_damp1", C.c_float), ("fc_end", C.c_float),
                 ("taper_vert", C.c_int), ("taper_hor", C.c_int), ("taper_sources", C.c_int),
-                ("taper_file", C.c_int), ("taper", Taper)]
+                ("taper_file", C.c_int), ("taper", Taper), ("time_filt", C.c_int), ("fc_start", C.c_float),
+                ("order", C.c_int)]
 
 
 class FwiRecord(C.Structure):
@@ -101,7 +105,8 @@ class FwiRecord(C.Structure):
 FWI_DEFAULTS = dict(grad_method=2, nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1,
                     testshot_end=1, testshot_incr=1, vs_lower=0.0, vs_upper=1e9, rho_lower=0.0, rho_upper=1e9,
                     inv_vs_iter=0, inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0, spatfilter=0, wd_damp=0.0,
-                    wd_damp1=0.0, fc_end=0.0, taper_vert=0, taper_hor=0, taper_sources=0, taper_file=0)
+                    wd_damp1=0.0, fc_end=0.0, taper_vert=0, taper_hor=0, taper_sources=0, taper_file=0,
+                    time_filt=0, fc_start=0.0, order=0)
 # taper parameters of fwi_start (go into FwiOpts.taper)
 TAPER_KEYS = ("gradt", "srtshape", "srtradius", "filtsize", "taper_map")
 
@@ -253,6 +258,12 @@ class Context:
         assert obs.shape == (self.nshots, self.ntr, self.nt_max)
         self._chk(lib().dn_set_observed(self._h, _f(obs)))
 
+    def set_time_filt(self, time_filt, fc_start=0.0, fc=0.0, order=0):
+        """Stage frequency filter of the signals and observed data (from
+        the copies last set; time_filt 0 restores them)."""
+        lib().dn_set_time_filt.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int]
+        self._chk(lib().dn_set_time_filt(self._h, time_filt, fc_start, fc, order))
+
     # ---- FWI (dn_fwi_*) ----
     def fwi_start(self, **opts):
         o = FwiOpts()
@@ -273,7 +284,7 @@ class Context:
         return out
 
     def fwi_write_files(self, rec, misfit_log=None, time_filt=0, fc_start=0.0, fc=0.0, jacobian=None,
-                        inv_modelfile=None, write_it=0):
+                        inv_modelfile=None, write_it=0, lbfgs_old=0):
         """Writes the reference's per-iteration files for the record rec of
         fwi_iteration()."""
         r = FwiRecord()
@@ -284,9 +295,19 @@ class Context:
             else:
                 setattr(r, k, rec[k])
         enc = lambda x: None if x is None else x.encode()
-        f = FwiFiles(enc(misfit_log), time_filt, fc_start, fc, enc(jacobian), enc(inv_modelfile), write_it)
+        f = FwiFiles(enc(misfit_log), time_filt, fc_start, fc, enc(jacobian), enc(inv_modelfile), write_it,
+                     lbfgs_old)
         lib().dn_fwi_write_files.argtypes = [C.c_void_p, C.POINTER(FwiRecord), C.POINTER(FwiFiles)]
         self._chk(lib().dn_fwi_write_files(self._h, C.byref(r), C.byref(f)))
+
+    def fwi_save_state(self, path):
+        lib().dn_fwi_save_state.argtypes = [C.c_void_p, C.c_char_p]
+        self._chk(lib().dn_fwi_save_state(self._h, path.encode()))
+
+    def fwi_load_state(self, path, jacobian=None, name_fix=1):
+        lib().dn_fwi_load_state.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]
+        self._chk(lib().dn_fwi_load_state(self._h, path.encode(), None if jacobian is None else jacobian.encode(),
+                                          name_fix))
 
     def fwi_model(self):
         vs = np.zeros((self.ny, self.nx), np.float32)
@@ -604,6 +625,23 @@ def time_window(data, dt, timewin, gamma, twlength_plus=0.0, picks=None):
     if L.dn_time_window(ntr, ns, dt, timewin, gamma, twlength_plus, _f(p), _f(d)) != 0:
         raise DeniseError(L.dn_last_error(None).decode())
     return d
+
+
+def calc_res(syn, obs, dt):
+    """calc_res.c + calc_energy.c on one shot's host seismograms [ntr][ns]
+    (runs on the GPU): returns (time-reversed residual, l2, energy, obs with
+    obs[:, 0] = syn[:, 0] as the reference leaves it)."""
+    s = np.ascontiguousarray(syn, np.float32)
+    o = np.ascontiguousarray(obs, np.float32).copy()
+    ntr, ns = s.shape
+    diff = np.zeros_like(s)
+    l2, en = C.c_double(), C.c_double()
+    L = lib()
+    L.dn_calc_res.argtypes = [C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                              C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    if L.dn_calc_res(ntr, ns, dt, _f(s), _f(o), _f(diff), C.byref(l2), C.byref(en)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return diff, l2.value, en.value, o
 
 
 def stf(obs, syn, signal, mute=None, eps_stf=0.05, normalize=0):

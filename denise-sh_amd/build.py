@@ -37,6 +37,7 @@ SOURCES = [
     ("dn_unet.cpp", False),
     ("dn_gradproc.hip", True),
     ("dn_stf.hip", True),
+    ("dn_misfit.hip", True),
 ]
 
 

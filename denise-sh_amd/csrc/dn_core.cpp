@@ -6,6 +6,7 @@
 // shots), its residuals, its adjoint loop, then the per-shot preconditioning
 // and accumulation in shot order.
 #include "dn_core.h"
+#include "dn_setup.h"
 
 #include <algorithm>
 #include <cmath>
@@ -13,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace dn;
@@ -364,6 +366,7 @@ extern "C" int dn_set_receivers(dn_ctx *c, int ntr, const int *ri, const int *rj
     c->dc = dn_ctx::DataCond{};   // conditioning refers to receivers: reset
     c->dc_dirty = true;
     dfree_t(c->d_obs);  // observed data are [shot][ntr][nt_max]: set them again
+    c->h_obs.clear();
     free_batch(c);      // seismogram buffers depend on ntr
     return 0;
 }
@@ -407,6 +410,9 @@ extern "C" int dn_set_shots_multi(dn_ctx *c, int nshots, int nt_max, const int *
     c->shot_l2.assign(nshots, 0.0);
     c->shot_energy.assign(nshots, 0.0);
     c->cur_map.clear();
+    c->h_sig.assign(signals, signals + (size_t)nsrc * nt_max);
+    c->h_obs.clear();
+    c->filt_sig = c->filt_obs = dn_ctx::TimeFilt{};
     free_batch(c);
     return 0;
 }
@@ -425,6 +431,8 @@ extern "C" int dn_set_signals(dn_ctx *c, const float *signals)
     if (!c || !signals || c->nshots < 1) return dn_fail(c, "dn_set_signals: set shots first");
     const size_t n = c->src_i.size() * (size_t)c->nt_max;
     HIPCHK(c, hipMemcpy(c->d_sig, signals, n * sizeof(float), hipMemcpyHostToDevice));
+    c->h_sig.assign(signals, signals + n);
+    c->filt_sig = dn_ctx::TimeFilt{};
     return 0;
 }
 
@@ -435,6 +443,64 @@ extern "C" int dn_set_observed(dn_ctx *c, const float *obs)
     const size_t n = (size_t)c->nshots * c->ntr * c->nt_max;
     if (!c->d_obs) HIPCHK(c, dalloc(&c->d_obs, n));
     HIPCHK(c, hipMemcpy(c->d_obs, obs, n * sizeof(float), hipMemcpyHostToDevice));
+    c->h_obs.assign(obs, obs + n);
+    c->filt_obs = dn_ctx::TimeFilt{};
+    return 0;
+}
+
+// Filters copies of [rows][nt_max] traces whose first ns(row) samples carry
+// the record, as timedomain_filt.c does per trace (low-pass at fc, then for
+// TIME_FILT 2 high-pass at fc_start), spread over host threads.
+static void filter_rows(std::vector<float> &out, const std::vector<float> &in, int rows, int nt_max,
+                        const std::vector<int> &row_ns, const dn_ctx::TimeFilt &f, float dt)
+{
+    out = in;
+    if (f.time_filt == 0) return;
+    const int nth = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+        pool.emplace_back([&, t]() {
+            for (int r = t; r < rows; r += nth) {
+                float *row = out.data() + (size_t)r * nt_max;
+                dn::timedomain_filt(row, 1, row_ns[r], dt, f.fc, f.order, 1);
+                if (f.time_filt == 2) dn::timedomain_filt(row, 1, row_ns[r], dt, f.fc_start, f.order, 2);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+extern "C" int dn_set_time_filt(dn_ctx *c, int time_filt, float fc_start, float fc, int order)
+{
+    if (!c) return -1;
+    if (time_filt < 0 || time_filt > 2) return dn_fail(c, "dn_set_time_filt: TIME_FILT must be 0, 1 or 2");
+    if (time_filt && (order < 1 || !(fc > 0.0f) || (time_filt == 2 && !(fc_start > 0.0f))))
+        return dn_fail(c, "dn_set_time_filt: need ORDER >= 1 and positive corner frequencies");
+    if (c->nshots < 1) return dn_fail(c, "dn_set_time_filt: set shots first");
+    dn_ctx::TimeFilt want;
+    want.time_filt = time_filt;
+    if (time_filt) {
+        want.order = order;
+        want.fc = fc;
+        want.fc_start = time_filt == 2 ? fc_start : 0.0f;
+    }
+    std::vector<float> buf;
+    if (!(c->filt_sig == want)) {
+        // sources of shot s: src_off[s] .. src_off[s+1]-1, each nt(s) samples
+        std::vector<int> ns(c->src_i.size());
+        for (int s = 0; s < c->nshots; ++s)
+            for (int l = c->src_off[s]; l < c->src_off[s + 1]; ++l) ns[l] = c->shot_nt[s];
+        filter_rows(buf, c->h_sig, (int)ns.size(), c->nt_max, ns, want, c->p.dt);
+        if (!buf.empty()) HIPCHK(c, hipMemcpy(c->d_sig, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice));
+        c->filt_sig = want;
+    }
+    if (!c->h_obs.empty() && c->d_obs && !(c->filt_obs == want)) {
+        std::vector<int> ns((size_t)c->nshots * c->ntr);
+        for (int s = 0; s < c->nshots; ++s)
+            for (int r = 0; r < c->ntr; ++r) ns[(size_t)s * c->ntr + r] = c->shot_nt[s];
+        filter_rows(buf, c->h_obs, (int)ns.size(), c->nt_max, ns, want, c->p.dt);
+        HIPCHK(c, hipMemcpy(c->d_obs, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice));
+        c->filt_obs = want;
+    }
     return 0;
 }
 

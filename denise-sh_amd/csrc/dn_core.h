@@ -106,6 +106,21 @@ struct dn_ctx {
     bool dc_dirty = false;
     unsigned char *d_trace_flags = nullptr;
     std::vector<int> src_off;        // host copy of the per-shot source ranges
+    // the signals and observed data as the caller set them (host copies):
+    // a stage's frequency filter is applied to these afresh, never on top of
+    // an earlier stage's (denise.c:1099-1111, 1381-1394 filter the freshly
+    // made wavelet and the data read from disk each time)
+    struct TimeFilt {
+        int time_filt = 0, order = 0;
+        float fc_start = 0.0f, fc = 0.0f;
+        bool operator==(const TimeFilt &o) const
+        {
+            return time_filt == o.time_filt && (time_filt == 0 || (order == o.order && fc == o.fc &&
+                                                                 (time_filt != 2 || fc_start == o.fc_start)));
+        }
+    };
+    std::vector<float> h_sig, h_obs;
+    TimeFilt filt_sig, filt_obs;     // the filter the device signals / observed data carry
 };
 
 int dn_fail(dn_ctx *c, const char *fmt, ...);

@@ -270,12 +270,15 @@ extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
     if (!c->have_model || c->nshots < 1 || !c->d_obs) return dn_fail(c, "dn_fwi_start: set model, shots and data first");
     if (o->grad_method != 1 && o->grad_method != 2) return dn_fail(c, "GRAD_METHOD must be 1 (PCG) or 2 (L-BFGS)");
     if (o->grad_method == 2 && (o->nlbfgs < 1 || o->nlbfgs > 64)) return dn_fail(c, "NLBFGS outside 1..64");
-    // C_vs, C_rho of the starting model, kept for the run (denise.c:746-819)
-    if (dn_update_means(c)) return -1;
+    // the stage's frequency filter on the wavelets and observed data
+    if (dn_set_time_filt(c, o->time_filt, o->fc_start, o->fc_end, o->order)) return -1;
     const size_t nxny = (size_t)c->g.nx * c->g.ny;
     dn_fwi_state *f = c->fwi;
     const bool fresh = (f == nullptr);
     if (fresh) {
+        // C_vs, C_rho of the starting model, kept for the whole run over all
+        // stages (denise.c:746-819, at iter_true == 1)
+        if (dn_update_means(c)) return -1;
         f = new dn_fwi_state();
         c->fwi = f;
         f->nxny = nxny;
@@ -809,7 +812,10 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
         }
     }
     f->iter++;
-    f->iter_true++;
+    // denise.c:2014-2019: the stage change leaves the iteration loop with
+    // break, before iter_true++ (the next stage's first iteration carries the
+    // same iter_true)
+    if (!R.next_stage) f->iter_true++;
     if (rec) *rec = R;
     return 0;
 }
@@ -837,6 +843,23 @@ extern "C" int dn_fwi_write_files(dn_ctx *c, const dn_fwi_record *rec, const dn_
     if (F->jacobian) {
         HIPCHK(c, hipMemcpy(a.data(), c->fwi->gu, n * sizeof(float), hipMemcpyDeviceToHost));
         if (write(std::string(F->jacobian) + "_grad_vs.bin", a.data())) return -1;
+    }
+    if (F->jacobian && F->lbfgs_old && c->fwi->o.grad_method == 2) {
+        // extract_LBFGS_SH.c:68-178: the model the iteration started from,
+        // the normalised gradient there and the direction (H^-1 g), each as
+        // the rank file <JACOBIAN>_<name>.old.<POS[1]>.<POS[2]> and the
+        // merged <JACOBIAN>_<name>.old (mergemod.c; one domain: POS = 0, 0)
+        const dn_fwi_state *f = c->fwi;
+        const std::string jac = F->jacobian;
+        const struct {
+            const char *name;
+            const float *v;
+        } files[] = {{"_p_vs.old", f->mold + n}, {"_p_u.old", f->gold + n}, {"_c_u.old", f->gu},
+                     {"_p_mrho.old", f->mold}, {"_p_rho.old", f->gold}, {"_c_rho.old", f->grho}};
+        for (const auto &fl : files) {
+            HIPCHK(c, hipMemcpy(a.data(), fl.v, n * sizeof(float), hipMemcpyDeviceToHost));
+            if (write(jac + fl.name + ".0.0", a.data()) || write(jac + fl.name, a.data())) return -1;
+        }
     }
     if (F->inv_modelfile) {
         const std::string base = F->inv_modelfile;
@@ -872,5 +895,134 @@ extern "C" int dn_fwi_get_direction(dn_ctx *c, float *gvs, float *grho)
     const size_t n = c->fwi->nxny * sizeof(float);
     if (gvs) HIPCHK(c, hipMemcpy(gvs, c->fwi->gu, n, hipMemcpyDeviceToHost));
     if (grho) HIPCHK(c, hipMemcpy(grho, c->fwi->grho, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Optimiser state on disk.  The reference keeps the L-BFGS ring and the PCG
+// vectors in memory only (a restarted denise.c begins with an empty ring) and
+// reads the previous model and normalised gradient back from the files of
+// extract_LBFGS_SH.c (store_LBFGS_SH.c:78-112).  dn_fwi_save_state writes
+// everything an FWI run needs to continue; dn_fwi_load_state restores it and
+// can take the previous model and gradient from those reference-named files.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr char kStateMagic[8] = {'D', 'N', 'F', 'W', 'I', 'S', '0', '1'};
+
+struct StateHead {
+    char magic[8];
+    int32_t nx, ny, grad_method, nlbfgs, iter, iter_true, stage, min_iter_help, lbfgs_ptr, nhist;
+    float C_vs, C_rho;
+    double last_plain_drop;
+};
+
+// the device arrays of the state, in file order
+std::vector<std::pair<float *, size_t>> state_arrays(dn_fwi_state *f)
+{
+    const size_t n = f->nxny, v2 = 2 * n;
+    std::vector<std::pair<float *, size_t>> v = {{f->m_vs, n},     {f->m_rho, n},    {f->m0_vs, n},
+                                                 {f->m0_rho, n},   {f->gold, v2},    {f->mold, v2},
+                                                 {f->pcg_old, v2}, {f->pcg_dir, v2}};
+    if (f->o.grad_method == 2) {
+        v.push_back({f->y, (size_t)f->o.nlbfgs * v2});
+        v.push_back({f->s, (size_t)f->o.nlbfgs * v2});
+    }
+    return v;
+}
+}  // namespace
+
+extern "C" int dn_fwi_save_state(dn_ctx *c, const char *path)
+{
+    if (!c || !c->fwi || !path) return dn_fail(c, "dn_fwi_save_state: no FWI state or no path");
+    dn_fwi_state *f = c->fwi;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StateHead h{};
+    std::memcpy(h.magic, kStateMagic, 8);
+    h.nx = c->g.nx;
+    h.ny = c->g.ny;
+    h.grad_method = f->o.grad_method;
+    h.nlbfgs = f->o.nlbfgs;
+    h.iter = f->iter;
+    h.iter_true = f->iter_true;
+    h.stage = f->stage;
+    h.min_iter_help = f->min_iter_help;
+    h.lbfgs_ptr = f->lbfgs_ptr;
+    h.nhist = (int32_t)std::min<size_t>(f->L2_hist.size(), (size_t)std::max(0, f->iter) + 1);
+    h.C_vs = c->C_vs;
+    h.C_rho = c->C_rho;
+    h.last_plain_drop = f->last_plain_drop;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return dn_fail(c, "Could not open %s", path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1 &&
+              std::fwrite(f->L2_hist.data(), sizeof(float), (size_t)h.nhist, fp) == (size_t)h.nhist;
+    std::vector<float> buf;
+    for (const auto &a : state_arrays(f)) {
+        if (!ok) break;
+        buf.resize(a.second);
+        if (hipMemcpy(buf.data(), a.first, a.second * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+            std::fclose(fp);
+            return dn_fail(c, "dn_fwi_save_state: device copy failed");
+        }
+        ok = std::fwrite(buf.data(), sizeof(float), a.second, fp) == a.second;
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? 0 : dn_fail(c, "Could not write %s", path);
+}
+
+extern "C" int dn_fwi_load_state(dn_ctx *c, const char *path, const char *jacobian, int name_fix)
+{
+    if (!c || !c->fwi || !path) return dn_fail(c, "dn_fwi_load_state: call dn_fwi_start first, and give a path");
+    dn_fwi_state *f = c->fwi;
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return dn_fail(c, "Could not open %s", path);
+    StateHead h{};
+    if (std::fread(&h, sizeof(h), 1, fp) != 1 || std::memcmp(h.magic, kStateMagic, 8) != 0) {
+        std::fclose(fp);
+        return dn_fail(c, "%s is not an FWI state file", path);
+    }
+    if (h.nx != c->g.nx || h.ny != c->g.ny || h.grad_method != f->o.grad_method || h.nlbfgs != f->o.nlbfgs ||
+        h.nhist < 0 || h.nhist > (int)f->L2_hist.size()) {
+        std::fclose(fp);
+        return dn_fail(c, "%s: grid %dx%d, GRAD_METHOD %d, NLBFGS %d do not match this run", path, h.nx, h.ny,
+                       h.grad_method, h.nlbfgs);
+    }
+    bool ok = std::fread(f->L2_hist.data(), sizeof(float), (size_t)h.nhist, fp) == (size_t)h.nhist;
+    std::vector<float> buf;
+    for (const auto &a : state_arrays(f)) {
+        if (!ok) break;
+        buf.resize(a.second);
+        ok = std::fread(buf.data(), sizeof(float), a.second, fp) == a.second &&
+             hipMemcpy(a.first, buf.data(), a.second * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    std::fclose(fp);
+    if (!ok) return dn_fail(c, "%s: truncated FWI state file", path);
+    if (jacobian) {
+        // store_LBFGS_SH.c:78-112 reads the rank files as
+        // <JACOBIAN>_<name>.old.%i%i, extract_LBFGS_SH.c writes them as
+        // .old.%i.%i: name_fix reads what was written, 0 the reference's name
+        const std::string sfx = name_fix ? ".0.0" : ".00";
+        const size_t n = f->nxny;
+        const struct {
+            const char *name;
+            float *v;
+        } files[] = {{"_p_rho.old", f->gold}, {"_p_mrho.old", f->mold}, {"_p_u.old", f->gold + n},
+                     {"_p_vs.old", f->mold + n}};
+        buf.resize(n);
+        for (const auto &fl : files) {
+            const std::string p = std::string(jacobian) + fl.name + sfx;
+            if (dn_read_model_file(p.c_str(), c->g.nx, c->g.ny, buf.data()))
+                return dn_fail(c, "dn_fwi_load_state: cannot read %s", p.c_str());
+            HIPCHK(c, hipMemcpy(fl.v, buf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    f->iter = h.iter;
+    f->iter_true = h.iter_true;
+    f->stage = h.stage;
+    f->min_iter_help = h.min_iter_help;
+    f->lbfgs_ptr = h.lbfgs_ptr;
+    f->last_plain_drop = h.last_plain_drop;
+    c->C_vs = h.C_vs;
+    c->C_rho = h.C_rho;
+    c->C_fixed = true;
     return 0;
 }

@@ -89,6 +89,37 @@ void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvy, float **vy,
  * centred time derivative of the signal (one-sided at nt = 1 and NT). */
 void psource(int nt, float **sxx, float **syy, float **srcpos_loc, float **signals, int nsrc);
 
+/* zero_fdveps.c:11-13 / zero_fdveps_visc.c:11-14 (fd.h:447-454): zero the
+ * wavefields, the CPML memory and (visco) the memory variables as the
+ * reference does, and, in resident mode, their device copies too. */
+void zero_fdveps(int ny1, int ny2, int nx1, int nx2, float **vy, float **syy, float **sxy, float **vym1, float **vyp1,
+                 float **psi_sxy_x, float **psi_vyx, float **psi_syy_y, float **psi_vyy);
+void zero_fdveps_visc(int ny1, int ny2, int nx1, int nx2, float **vy, float **syy, float **sxy, float **vym1,
+                      float **vyp1, float **psi_sxy_x, float **psi_vyx, float **psi_syy_y, float **psi_vyy,
+                      float ***pr, float ***pp, float ***pq);
+
+/* Residency of the drop-ins' arrays (no counterpart in the reference; it
+ * replaces the reference's per-step host arrays with device-resident ones).
+ * mode 0 (default): every drop-in call copies what it reads to the device
+ * and what it writes back, so the host arrays are current after each call.
+ * mode 1: each array a drop-in receives is copied to HBM when its host
+ * pointer is first seen and then stays there; the drop-ins run on the device
+ * copies without host transfers, and the host arrays go stale.  seismo_ssg
+ * then reads the receiver samples back from the device (the one sync per
+ * step) and also refreshes vy[NY/2][NX/2], the cell the reference's NaN
+ * probe reads (denise.c:1183).  mode 2: as 1, but seismo_ssg keeps the
+ * samples on the device until dn_dropin_sync(1) writes them into sectionvy
+ * (no sync per step).  Switching to 0 copies the written arrays back and
+ * frees the device copies. */
+int dn_dropin_resident(int mode);
+/* to_host 1: wait for the device, copy every resident array the kernels
+ * write (wavefields, memory variables, CPML memory) and the deferred
+ * seismograms back into the host arrays, e.g. before the host reads fields
+ * (the snapshot loop of denise.c:1262-1306).  to_host 0: re-read every
+ * resident array from the host, e.g. after the host changed the model
+ * (prepare_update_s after a model update) or a CPML vector. */
+int dn_dropin_sync(int to_host);
+
 /* ------------------------------------------------------------------------
  * Host-side setup routines of the path (no device needed).  The reference
  * versions read globals; these take the same quantities as arguments.
@@ -192,6 +223,17 @@ int dn_set_signals(dn_ctx *ctx, const float *signals);
 
 /* Observed data host [nshots][ntr][nt_max] (samples 1..nt[s] used). */
 int dn_set_observed(dn_ctx *ctx, const float *obs);
+
+/* Frequency filter of a workflow stage (denise.c:1099-1111 on the wavelets,
+ * 1381-1394 and line_search.c:138-145, 258-270 on the observed data):
+ * TIME_FILT 1 low-passes every source signal and observed trace at FC with
+ * timedomain_filt.c (Butterworth of ORDER, libcseife), TIME_FILT 2 then
+ * high-passes at FC_START as well, 0 leaves them unfiltered.  The filter is
+ * applied to the signals and data as last given to dn_set_shots /
+ * dn_set_signals / dn_set_observed (the context keeps host copies), never
+ * on top of an earlier stage's filter, each shot's traces over its own nt
+ * samples.  dn_fwi_start calls it with the stage's values. */
+int dn_set_time_filt(dn_ctx *ctx, int time_filt, float fc_start, float fc, int order);
 
 /* Maximum number of shots advanced together (0 = automatic from free HBM)
  * and rows per workgroup segment of the time-step kernel (0 = automatic). */
@@ -344,6 +386,15 @@ int dn_set_datacond(dn_ctx *ctx, const dn_datacond *dc);
 int dn_time_window(int ntr, int ns, float dt, int timewin, float gamma, float twlength_plus, const float *picks,
                    float *data);
 
+/* calc_res.c:150-257 (LNORM 2, GRAD_FORM 2) and calc_energy.c:53-61 on one
+ * shot's host seismograms syn, obs [ntr][ns]: diff [ntr][ns] receives the
+ * time-reversed residual syn - obs, *l2 the sum of its squares and *energy
+ * the energy of the time-integrated observed traces (sum over samples of
+ * (DT * running sum of obs)^2), both summed in FP64 per trace and then in
+ * trace order.  As in the reference, obs[i][0] is replaced by syn[i][0]
+ * first (on return obs holds that substitution). */
+int dn_calc_res(int ntr, int ns, float dt, const float *syn, float *obs, float *diff, double *l2, double *energy);
+
 /* stf.c:11-250: source time function of one shot by Wiener deconvolution of
  * the observed by the modelled traces, applied to the current signal:
  *   STF = Re IFFT( S sum_i D_i conj(M_i) / (sum_i |M_i|^2 + EPS_STF ntr Ebar) )
@@ -430,6 +481,10 @@ typedef struct dn_fwi_opts {
     float wd_damp, wd_damp1, fc_end;   /* WD_DAMP (y), WD_DAMP1 (x), FC_END           */
     int taper_vert, taper_hor, taper_sources, taper_file;   /* SWS_TAPER_GRAD_*, SWS_TAPER_FILE */
     dn_taper taper;       /* GRADT1..4, SRTSHAPE, SRTRADIUS, FILTSIZE, map (sws ignored) */
+    /* frequency filter of the stage (dn_set_time_filt): low-pass at fc_end */
+    int time_filt;        /* TIME_FILT: 0 none, 1 low-pass, 2 band-pass            */
+    float fc_start;       /* FC_START (FC_low of the workflow): high-pass corner    */
+    int order;            /* ORDER of the Butterworth filters                      */
 } dn_fwi_opts;
 
 /* One line of the reference's misfit log (denise.c:1929-1933) plus extras. */
@@ -473,8 +528,29 @@ typedef struct dn_fwi_files {
     const char *jacobian;     /* JACOBIAN                                           */
     const char *inv_modelfile;/* INV_MODELFILE                                      */
     int write_it;             /* also the per-iteration model files                 */
+    int lbfgs_old;            /* L-BFGS: also the files of extract_LBFGS_SH.c:68-178,
+                                 <JACOBIAN>_p_vs.old, _p_u.old, _c_u.old, _p_mrho.old,
+                                 _p_rho.old, _c_rho.old (model the iteration started
+                                 from, normalised gradient there, direction), each as
+                                 the rank file <name>.0.0 and the merged <name>        */
 } dn_fwi_files;
 int dn_fwi_write_files(dn_ctx *ctx, const dn_fwi_record *rec, const dn_fwi_files *files);
+
+/* The FWI optimiser state on disk (no counterpart in the reference, which
+ * keeps the L-BFGS ring and the PCG vectors in memory and so cannot resume):
+ * dn_fwi_save_state writes iteration, stage and abort counters, the misfit
+ * history, C_vs / C_rho, current and starting model, previous model and
+ * normalised gradient, the PCG vectors and the whole L-BFGS ring.
+ * dn_fwi_load_state restores it into a context after dn_fwi_start with the
+ * same GRAD_METHOD and NLBFGS (model, shots and data set as for the saved
+ * run); the next dn_fwi_iteration continues the saved run.  With jacobian
+ * non-NULL the previous model and normalised gradient are then read from the
+ * reference-named files written with lbfgs_old (store_LBFGS_SH.c:78-112
+ * reads them): name_fix 1 reads <JACOBIAN>_p_rho.old.0.0 etc., the names
+ * extract_LBFGS_SH.c writes; name_fix 0 reads <JACOBIAN>_p_rho.old.00 etc.,
+ * the names store_LBFGS_SH.c asks for (absent unless renamed: an error). */
+int dn_fwi_save_state(dn_ctx *ctx, const char *path);
+int dn_fwi_load_state(dn_ctx *ctx, const char *path, const char *jacobian, int name_fix);
 
 /* Learned preconditioner for L-BFGS (NULL: off).  The network's
  * prediction H of the vs diagonal Hessian gives the initial L-BFGS Hessian

@@ -62,6 +62,7 @@ class FwiOracle:
         self.sig = np.ascontiguousarray(signals, np.float32)
         self.rec_i, self.rec_j = np.asarray(rec_i, np.int32), np.asarray(rec_j, np.int32)
         self.obs = np.ascontiguousarray(obs, np.float32)
+        self.sig0, self.obs0 = self.sig.copy(), self.obs.copy()
         o = dict(nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1, testshot_end=1,
                  testshot_incr=1, vs_lower=0.0, vs_upper=1e9, rho_lower=0.0, rho_upper=1e9, inv_vs_iter=0,
                  inv_rho_iter=0, scalerho=0.5, min_iter=0, pro=0.0, spatfilter=0, wd_damp=0.0, wd_damp1=0.0,
@@ -86,6 +87,29 @@ class FwiOracle:
         inc = max(1, o["testshot_incr"])
         self.tshots = [k - 1 for k in range(o["testshot_start"], min(o["testshot_end"], len(self.src_i)) + 1, inc)
                        if k >= 1]
+
+    def start_stage(self, time_filt=0, fc_start=0.0, order=0, **opts):
+        """A new workflow stage (denise.c:682-689 and the optimiser reset of
+        denise.c:1998-2005): the stage's options replace the old ones, and
+        the wavelets and observed data are the originals low-passed at the
+        stage's FC = FC_END with ORDER (TIME_FILT 1), then high-passed at
+        FC_START (TIME_FILT 2), by timedomain_filt (denise.c:1099-1111,
+        1381-1394)."""
+        self.o.update(opts)
+        self.sig, self.obs = self.sig0.copy(), self.obs0.copy()
+        if time_filt:
+            fc = self.o["fc_end"]
+            self.sig = self.O.timedomain_filt(self.sig, self.cfg.dt, fc, order, 1)
+            self.obs = np.stack([self.O.timedomain_filt(o, self.cfg.dt, fc, order, 1) for o in self.obs])
+            if time_filt == 2:
+                self.sig = self.O.timedomain_filt(self.sig, self.cfg.dt, fc_start, order, 2)
+                self.obs = np.stack([self.O.timedomain_filt(o, self.cfg.dt, fc_start, order, 2) for o in self.obs])
+        self.y[:] = 0.0
+        self.s[:] = 0.0
+        self.ptr = 1
+        self.iter = 1
+        self.pcg_old[:] = 0.0
+        self.pcg_dir[:] = 0.0
 
     # test-shot misfit of a model (forward + residual, FP64 sums)
     def _misfit(self, vs, rho):

@@ -2,12 +2,13 @@
 update_s_elastic_PML, update_s_visc_PML) called with the reference's
 Numerical-Recipes float** arrays, against the oracle's single sweeps."""
 import ctypes as C
+import os
 import sys
 
 import numpy as np
 import pytest
 
-from common import model, oracle, product
+from common import ROOT, model, oracle, product
 
 pytestmark = pytest.mark.gpu
 
@@ -139,3 +140,91 @@ def test_update_s_PML(fdorder, free_surf, L):
         d = np.max(np.abs(got.astype(np.float64) - ref))
         say("  %s max |diff| %.3e" % (name, d))
         assert np.array_equal(got, ref.astype(np.float32)), name
+
+
+# --------------------------------------------------------------------------
+# the drop-ins driven from C (tests/dropin_caller.c) like denise.c's loop
+# --------------------------------------------------------------------------
+CALLER = os.path.join(ROOT, "tests", "dropin_caller")
+
+
+def _caller_input(path, nx, ny, fdorder, fw, free_surf, L, nt, mode, seed=3):
+    from pyoracle import make_cfg
+    o = oracle("f32")
+    fl = [6.25, 50.0, 12.0, 30.0][:L]
+    cfg = make_cfg(nx, ny, fdorder=fdorder, fw=fw, free_surf=free_surf, L=L, nt=nt, dt=1e-4, dh=0.25,
+                   damping=300.0, fpml=40.0, fl=fl)
+    vs, rho, q = model(nx, ny, 0.25, seed=seed, qs=20.0 if L else None)
+    P = o.model_setup(cfg, vs, rho, q)
+    nd = fdorder // 2 + 1
+    I = (slice(nd, nd + ny), slice(nd, nd + nx))
+    hc = o.holberg(fdorder, 1).astype(np.float32)
+    pml = o.pml(cfg)
+    sig = o.wavelet(1, nt, 1e-4, 0.0, 40.0, 1.0).astype(np.float32)
+    rec_i = np.arange(fw + 3, nx - fw - 2, 7, dtype=np.int32)
+    rec_j = np.full_like(rec_i, 4)
+    src = (nx // 3, 5)
+    with open(path, "wb") as f:
+        np.array([nx, ny, fdorder, fw, free_surf, L, nt, len(rec_i), mode, src[0], src[1]], np.int32).tofile(f)
+        np.concatenate([[1e-4, 0.25], hc]).astype(np.float32).tofile(f)
+        for k in ("K_x", "a_x", "b_x", "K_x_half", "a_x_half", "b_x_half", "K_y", "a_y", "b_y", "K_y_half",
+                  "a_y_half", "b_y_half"):
+            pml[k][:2 * fw + 1].astype(np.float32).tofile(f)
+        planes = [P["rho"]] + ([P["uip"], P["ujp"]] if L == 0 else [P["fipjp"], P["f"]] + [P["dip"][l] for l in range(L)]
+                                + [P["d"][l] for l in range(L)])
+        for a in planes:
+            np.ascontiguousarray(a[I], np.float32).tofile(f)
+        if L:
+            for k in ("bip", "bjm", "cip", "cjm"):
+                P[k][:L].astype(np.float32).tofile(f)
+        sig.tofile(f)
+        rec_i.tofile(f)
+        rec_j.tofile(f)
+    seis = o.forward(cfg, vs, rho, q, np.array([src[0]], np.int32), np.array([src[1]], np.int32), sig[None],
+                     rec_i, rec_j)[0]
+    return seis.astype(np.float32), len(rec_i)
+
+
+def _run_caller(tmp, nx, ny, nt, ntr, mode):
+    import subprocess
+    out = os.path.join(tmp, "out%d.bin" % mode)
+    r = subprocess.run([CALLER, os.path.join(tmp, "in%d.bin" % mode), out], capture_output=True, text=True,
+                       timeout=300)
+    say("  " + r.stdout.strip() + (" | " + r.stderr.strip() if r.stderr.strip() else ""))
+    assert r.returncode == 0, r.stderr
+    a = np.fromfile(out, np.float32)
+    ms = float(r.stdout.split(",")[-1].split()[0])
+    return a[:ntr * nt].reshape(ntr, nt), a[ntr * nt:].reshape(ny, nx), ms
+
+
+@pytest.mark.parametrize("L,fdorder,free_surf", [(0, 4, 1), (2, 8, 0)])
+def test_c_caller_all_modes_bit_equal_to_oracle(tmp_path, L, fdorder, free_surf):
+    """denise.c's forward loop in C through the drop-ins: copy, resident and
+    resident with deferred seismograms give seismograms bit-equal to the
+    oracle's forward and the same final wavefield."""
+    assert os.path.exists(CALLER), "tests/dropin_caller not built (python -c 'import __graft_entry__ as g; g.build()')"
+    nx, ny, nt, fw = 90, 50, 300, 10
+    say("C caller L", L, "FDORDER", fdorder, "FREE_SURF", free_surf)
+    finals = []
+    for mode in (0, 1, 2):
+        seis, ntr = _caller_input(os.path.join(tmp_path, "in%d.bin" % mode), nx, ny, fdorder, fw, free_surf, L, nt,
+                                  mode)
+        got, vy, _ = _run_caller(str(tmp_path), nx, ny, nt, ntr, mode)
+        assert np.array_equal(got, seis), "mode %d" % mode
+        finals.append(vy)
+    assert np.array_equal(finals[0], finals[1]) and np.array_equal(finals[0], finals[2])
+
+
+def test_c_caller_resident_speed(tmp_path):
+    """The same loop at 1200 x 300 (visco L = 2, FDORDER 8): step time of the
+    three modes, the seismograms of the resident modes bit-equal to the
+    oracle's; resident steps run without the per-call host copies."""
+    assert os.path.exists(CALLER)
+    nx, ny, nt, fw = 1200, 300, 200, 20
+    ms = {}
+    for mode in (0, 1, 2):
+        seis, ntr = _caller_input(os.path.join(tmp_path, "in%d.bin" % mode), nx, ny, 8, fw, 1, 2, nt, mode)
+        got, _, ms[mode] = _run_caller(str(tmp_path), nx, ny, nt, ntr, mode)
+        assert np.array_equal(got, seis), "mode %d" % mode
+    say("  ms per step: copy %.3f, resident %.3f, deferred %.3f" % (ms[0], ms[1], ms[2]))
+    assert ms[1] < ms[0] and ms[2] < ms[0]
