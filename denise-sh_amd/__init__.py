@@ -29,6 +29,7 @@ EXPORTS = [
     "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host",
     "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
     "dn_set_group_layout", "dn_batch_shots", "dn_calc_res",
+    "dn_calc_res_noalloc",
     "dn_set_time_filt", "zero_fdveps", "zero_fdveps_visc", "dn_dropin_resident",
     "dn_dropin_sync", "dn_fwi_save_state", "dn_fwi_load_state",
 ]

@@ -38,6 +38,7 @@ SOURCES = [
     ("dn_gradproc.hip", True),
     ("dn_stf.hip", True),
     ("dn_misfit.hip", True),
+    ("dn_misfit_noalloc.hip", True),
 ]
 
 

@@ -394,6 +394,20 @@ int dn_time_window(int ntr, int ns, float dt, int timewin, float gamma, float tw
  * trace order.  As in the reference, obs[i][0] is replaced by syn[i][0]
  * first (on return obs holds that substitution). */
 int dn_calc_res(int ntr, int ns, float dt, const float *syn, float *obs, float *diff, double *l2, double *energy);
+/* No counterpart in the reference: dn_calc_res for callers that cannot
+ * allocate device memory (code under stream capture, serving loops, harnesses
+ * that forbid hipMalloc).  syn, obs, diff [ntr][ns] and l2_energy[2] (l2,
+ * energy) are device arrays; obs is read only (the obs[i][0] substitution is
+ * applied in the sums, not written back).  Work is queued on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting.
+ * The per-trace partial sums live in a fixed device array of the library,
+ * one per GPU, kept until the process ends: ntr up to
+ * DN_CALC_RES_NOALLOC_MAX_TRACES (1 MiB); larger ntr is rejected.  Calls
+ * that overlap on one GPU (other streams or threads) share that array and
+ * race. */
+#define DN_CALC_RES_NOALLOC_MAX_TRACES 65536
+int dn_calc_res_noalloc(int ntr, int ns, float dt, const float *syn, const float *obs, float *diff, double *l2_energy,
+                        void *stream);
 
 /* stf.c:11-250: source time function of one shot by Wiener deconvolution of
  * the observed by the modelled traces, applied to the current signal:
