@@ -1,6 +1,6 @@
 """Benchmark of the SH adjoint-gradient hot path on one MI355X.
 
-Two workloads (viscoelastic SH, FDORDER 8 Holberg with MAXRELERROR 1, L = 2
+Three workloads (viscoelastic SH, FDORDER 8 Holberg with MAXRELERROR 1, L = 2
 relaxation mechanisms with Q = 20 and FL = 6.25 / 50 Hz, 20-point CPML on
 the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
 
@@ -20,12 +20,18 @@ the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
   and the plain gradient (start); rho 1700 -> 2000; 40 shots, 25 Hz Ricker,
   240 receivers at 1 m spacing.  One step is the gradient over all 40
   shots (batches of as many shots as fit in HBM).
+* ``c3``: the batched workload: NX = 2000, NY = 400, DH = 0.2 m, DT = 1e-4 s,
+  the c2 model family; 64 shots with seeded record lengths of 0.4-1.2 s and
+  wavelets low-passed at 15, 30 or 60 Hz (Butterworth order 4, shot s in
+  stage s mod 3), all mixed in the same batches; 390 receivers at 1 m
+  spacing.  One step is the gradient over all 64 shots;
+  --compare-sequential also times it with one shot per batch.
 
 Observed data are synthesised on the device from the true model before
 timing (synthetic, seeded).  The metric counts cell-updates of the forward
 and adjoint sweeps (NX*NY*NT*2 per shot).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3] [--compare-sequential]
        [--nt NT] [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu]
        [--rows R] [--max-batch B] [--autotune] [--exec graph|eager]
        python bench.py --config c2 --fwi N     (N plain L-BFGS iterations, wall time)
@@ -97,6 +103,18 @@ def workload(config="c4", nt=None):
         row = int(np.floor(0.5 / dh + 0.5))
         nt = 10000 if nt is None else nt
         damping = 350.0
+    elif config == "c3":
+        nx, ny, dh, dt = 2000, 400, 0.2, 1e-4
+        vs_true, vs_start, rho, qs = checkerboard(nx, ny, dh)
+        nshots, nrec, fc = 64, 390, 25.0
+        src_x = np.linspace(3.0, nx * dh - 3.0, nshots)
+        rec_x = 5.0 + np.arange(nrec) * 1.0
+        row = int(np.floor(0.5 / dh + 0.5))
+        rng = np.random.default_rng(3)
+        nts = np.floor(rng.uniform(0.4, 1.2, nshots) / dt + 0.5).astype(np.int32)
+        nt = int(nts.max()) if nt is None else nt
+        nts = np.minimum(nts, nt)
+        damping = 350.0
     elif config == "c4":
         nx, ny, dh, dt = 4000, 800, 0.1, 5e-5
         vs_true, vs_start, rho, qs = layered_lenses(nx, ny, dh, seed=5)
@@ -112,14 +130,37 @@ def workload(config="c4", nt=None):
     src_j = np.full(nshots, row, np.int32)
     rec_i = np.floor(rec_x / dh + 0.5).astype(np.int32)          # receiver.c:40
     rec_j = np.full(nrec, row, np.int32)
-    return dict(config=config, nx=nx, ny=ny, dh=dh, dt=dt, nt=nt, vs_true=vs_true, vs_start=vs_start, rho=rho,
-                qs=qs, src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j, fdorder=8, L=2, fl=[6.25, 50.0], fw=20,
-                fc=fc, dtinv=4, damping=damping)
+    W = dict(config=config, nx=nx, ny=ny, dh=dh, dt=dt, nt=nt, vs_true=vs_true, vs_start=vs_start, rho=rho,
+             qs=qs, src_i=src_i, src_j=src_j, rec_i=rec_i, rec_j=rec_j, fdorder=8, L=2, fl=[6.25, 50.0], fw=20,
+             fc=fc, dtinv=4, damping=damping)
+    if config == "c3":
+        # uneven record lengths and three low-pass stages (Butterworth order
+        # 4 at 15 / 30 / 60 Hz, shot s in stage s mod 3) mixed in one batch
+        W["nts"] = nts
+        W["stages"] = [15.0, 30.0, 60.0]
+    return W
 
 
 def describe(W, shots, snap):
+    if "nts" in W:
+        nts = W["nts"][:shots]
+        return ("visco SH gradient %dx%d FDORDER8 L2, %d shots x %d-%d steps (uneven), low-pass stages %s Hz "
+                "mixed, DTINV %d, %s snapshots" % (W["nx"], W["ny"], shots, int(nts.min()), int(nts.max()),
+                                                   "/".join("%g" % f for f in W["stages"]), W["dtinv"], snap))
     return ("visco SH gradient %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d, %s snapshots"
             % (W["nx"], W["ny"], shots, W["nt"], W["dtinv"], snap))
+
+
+def signals_of(dn, W, nshots):
+    """Source signals [nshots][nt]: the Ricker wavelet, for the batched
+    workload low-passed per stage and cut at each shot's record length."""
+    base = ricker(dn, W["nt"], W["dt"], W["fc"])
+    sig = np.tile(base, (nshots, 1)).astype(np.float32)
+    if "nts" in W:
+        for s in range(nshots):
+            sig[s] = dn.timedomain_filt(base[None], W["dt"], W["stages"][s % 3], 4, 1)[0]
+            sig[s, W["nts"][s]:] = 0.0
+    return sig
 
 
 def ricker(dn, nt, dt, fc):
@@ -240,7 +281,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c4", choices=["c4", "c2"])
+    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3"])
+    ap.add_argument("--compare-sequential", action="store_true",
+                    help="also time the same gradient with one shot per batch (the per-shot sequential loop)")
     ap.add_argument("--nt", type=int, default=0, help="record length in steps (0: the workload's)")
     ap.add_argument("--shots", type=int, default=0, help="shots of the acquisition used (0: all)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"])
@@ -280,8 +323,8 @@ def main():
         ctx.set_batch(args.max_batch, args.rows)
     ctx.set_group_layout(args.layout)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
-    sig = np.tile(ricker(dn, W["nt"], W["dt"], W["fc"]), (nshots, 1))
-    ctx.set_shots(W["src_i"][:nshots], W["src_j"][:nshots], sig)
+    sig = signals_of(dn, W, nshots)
+    ctx.set_shots(W["src_i"][:nshots], W["src_j"][:nshots], sig, W["nts"][:nshots] if "nts" in W else None)
     # observed data from the true model (untimed setup, on the device)
     print("bench: %s, observed data from the true model" % describe(W, nshots, args.snap), file=sys.stderr,
           flush=True)
@@ -297,7 +340,7 @@ def main():
     tuned = None
     if args.autotune:
         tuned = ctx.autotune(nshots, min(W["nt"], 256))
-    # one step: the gradient of one batch (c4) or of all shots (c2)
+    # one step: the gradient of one batch (c4) or of all shots (c2, c3)
     if args.config == "c4":
         batch = ctx.batch_shots(True)
         batches = [(s0, min(batch, nshots - s0)) for s0 in range(0, nshots, batch)]
@@ -371,6 +414,20 @@ def main():
     out["config"]["exec"] = args.exec
     out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
     out["config"]["group_layout"] = "tuned" if args.autotune else ("shots" if args.layout else "strips")
+    if args.compare_sequential:
+        # the same gradient with one shot per batch: the per-shot sequential
+        # GPU loop the batched launch replaces (the reference's shot loop,
+        # denise.c:875)
+        ctx.set_batch(1, args.rows)
+        t1 = time.perf_counter()
+        ce_s = 0
+        for k in range(max(1, args.steps)):
+            ctx.gradient(*batches[(args.warmup + k) % len(batches)])
+            ce_s += sum(ctx.pass_timing(p)[2] for p in (0, 1))
+        wall_s = time.perf_counter() - t1
+        out["sequential"] = {"value": round(ce_s / wall_s / 1e9, 3), "unit": "Gcell-updates/s",
+                             "ms_per_step": round(wall_s / max(1, args.steps) * 1e3, 3),
+                             "batched_over_sequential": round(value / (ce_s / wall_s / 1e9), 3)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_seconds)
     ctx.close()

@@ -733,28 +733,44 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
 // rows per segment: enough waves to fill 256 CUs several times over (about
 // 36 per CU: the latency of the row loads is hidden by many resident waves
 // rather than by long segments; 1200x300, 20 shots, FDORDER 8 -> 16 rows)
+// Rows per segment of the fused step.  A segment of R rows costs its wave
+// R + 2K - 1 velocity rows (the first 2K - 1 only fill the register
+// windows), and the segments of all (shot, strip) pairs run as workgroups
+// of kStripsPerGroup waves, two per CU at the step kernel's four waves per
+// SIMD, each XCD taking an equal share of them (the kernel's block remap):
+// the launch lasts about as many row times as its fullest CU walks,
+// rounds x (R + 2K - 1) with rounds = the workgroups of an XCD over its
+// slots rounded up.  The smallest such cost wins; a part-filled last round
+// costs as much as a full one.
 static int choose_rows(const dn_ctx *c, int nshots, int nstrips)
 {
     if (c->rows_req > 0) return std::min(c->rows_req, c->g.ny);
-    const int ny = c->g.ny, target = 9216;
-    auto waves = [&](int R) { return (long long)nshots * nstrips * ((ny + R - 1) / R); };
-    const int cand[] = {1 << 20, 512, 256, 128, 96, 64, 48, 32, 24, 16};
-    int best = 16;
-    for (int R : cand) {
-        best = std::min(R, ny);
-        if (waves(best) >= target) break;
+    static int cus = 0;
+    if (cus <= 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
     }
-    best = std::max(best, std::min(4 * c->K, ny));
-    // small batches (the line-search trials of a few test shots) would
-    // leave most of the chip idle even at 16 rows: they go down to 12 or 8
-    // rows (not below 2K), more recomputed rows for more waves
-    if (waves(best) < target / 2)
-        for (int R : {12, 8}) {
-            const int Rr = std::min(R, ny);
-            if (Rr < std::min(2 * c->K, ny) || Rr >= best) continue;
-            best = Rr;
-            if (waves(best) >= target / 2) break;
+    const int ny = c->g.ny, K = c->K;
+    const long long xcd_slots = std::max(1LL, 2LL * cus / 8);   // 8 XCDs, two workgroups per CU
+    const int rmin = std::min(ny, 2 * K);
+    int best = ny;
+    long long best_cost = -1;
+    for (int segs = 1; segs <= (ny + rmin - 1) / rmin; ++segs) {
+        const int R = (ny + segs - 1) / segs;       // the fewest rows giving this many segments
+        if (R < rmin || (ny + R - 1) / R != segs) continue;
+        const long long wgs = ((long long)nshots * nstrips * segs + kStripsPerGroup - 1) / kStripsPerGroup;
+        const long long per_xcd = (wgs + 7) / 8;
+        const long long rounds = (per_xcd + xcd_slots - 1) / xcd_slots;
+        const long long cost = rounds * (R + 2 * K - 1);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = R;
         }
+    }
     return best;
 }
 

@@ -149,16 +149,21 @@ class Oracle:
         rec_i = np.ascontiguousarray(rec_i, np.int32)
         rec_j = np.ascontiguousarray(rec_j, np.int32)
         sig = np.ascontiguousarray(signals, np.float32)
-        obs = np.ascontiguousarray(obs, np.float32)
+        # observed data in this oracle's precision (FP64 oracle: float64 data
+        # keep the whole chain in double), else as float32
+        native = self.kind == "f64" and np.asarray(obs).dtype == np.float64
+        obs = np.ascontiguousarray(obs, np.float64 if native else np.float32)
         gvs = np.zeros((cfg.ny, cfg.nx), self.dt)
         grho = np.zeros((cfg.ny, cfg.nx), self.dt)
         wu = np.zeros((cfg.ny, cfg.nx), self.dt)
         wr = np.zeros((cfg.ny, cfg.nx), self.dt)
         l2 = C.c_double()
         en = C.c_double()
-        rc = self.fn("orc_gradient")(C.byref(cfg), self._f32(vs), self._f32(rho), self._f32(qs), len(src_i),
+        rc = self.fn("orc_gradient_obsreal" if native else "orc_gradient")(
+                                     C.byref(cfg), self._f32(vs), self._f32(rho), self._f32(qs), len(src_i),
                                      self._i32(src_i), self._i32(src_j), self._f32(sig), len(rec_i),
-                                     self._i32(rec_i), self._i32(rec_j), self._f32(obs), self._r(gvs),
+                                     self._i32(rec_i), self._i32(rec_j), self._r(obs) if native else self._f32(obs),
+                                     self._r(gvs),
                                      self._r(grho), C.byref(l2), C.byref(en), self._r(wu), self._r(wr))
         if rc:
             raise RuntimeError("oracle gradient failed (%d)" % rc)

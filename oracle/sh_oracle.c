@@ -1139,10 +1139,35 @@ int F(orc_adjoint)(const orc_cfg *c, const float *vs, const float *rho, const fl
  * Outputs (all [NY][NX] row-major REAL): grad_vs, grad_rho; optional
  * We_last (the last shot's energy weights).  l2/energy in double.
  * ---------------------------------------------------------------------- */
+static int gradient_impl(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
+                         const int *src_i, const int *src_j, const float *signals, int ntr, const int *rec_i,
+                         const int *rec_j, const float *obs, const real *obs_real, real *grad_vs, real *grad_rho,
+                         double *l2, double *energy, real *waveconv_u_out, real *waveconv_rho_out);
+
 int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
                     const int *src_i, const int *src_j, const float *signals, int ntr, const int *rec_i,
                     const int *rec_j, const float *obs, real *grad_vs, real *grad_rho, double *l2, double *energy,
                     real *waveconv_u_out, real *waveconv_rho_out)
+{
+    return gradient_impl(c, vs, rho, qs, nshots, src_i, src_j, signals, ntr, rec_i, rec_j, obs, NULL, grad_vs,
+                         grad_rho, l2, energy, waveconv_u_out, waveconv_rho_out);
+}
+
+/* The same with the observed data in the oracle's own precision (the FP64
+ * oracle's whole chain in double, observed data from orc_forward_f64). */
+int F(orc_gradient_obsreal)(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
+                            const int *src_i, const int *src_j, const float *signals, int ntr, const int *rec_i,
+                            const int *rec_j, const real *obs, real *grad_vs, real *grad_rho, double *l2,
+                            double *energy, real *waveconv_u_out, real *waveconv_rho_out)
+{
+    return gradient_impl(c, vs, rho, qs, nshots, src_i, src_j, signals, ntr, rec_i, rec_j, NULL, obs, grad_vs,
+                         grad_rho, l2, energy, waveconv_u_out, waveconv_rho_out);
+}
+
+static int gradient_impl(const orc_cfg *c, const float *vs, const float *rho, const float *qs, int nshots,
+                         const int *src_i, const int *src_j, const float *signals, int ntr, const int *rec_i,
+                         const int *rec_j, const float *obs, const real *obs_real, real *grad_vs, real *grad_rho,
+                         double *l2, double *energy, real *waveconv_u_out, real *waveconv_rho_out)
 {
     const int NT = c->nt, NX = c->nx, NY = c->ny;
     const size_t P = F(orc_padded_size)(c);
@@ -1179,7 +1204,8 @@ int F(orc_gradient)(const orc_cfg *c, const float *vs, const float *rho, const f
         rc = F(orc_forward)(c, vs, rho, qs, 1, &src_i[ishot], &src_j[ishot], signals + (size_t)ishot * NT, ntr,
                             rec_i, rec_j, seis, snaps, Ws);
         if (rc) break;
-        for (t = 0; t < (size_t)ntr * NT; t++) obsr[t] = (real)obs[(size_t)ishot * ntr * NT + t];
+        for (t = 0; t < (size_t)ntr * NT; t++)
+            obsr[t] = obs ? (real)obs[(size_t)ishot * ntr * NT + t] : obs_real[(size_t)ishot * ntr * NT + t];
         F(orc_residual_cond)(ntr, NT, c->dt, seis, obsr, diff, l2, energy,
                              c->skip_res ? c->skip_res + (size_t)ishot * ntr : NULL,
                              c->skip_en ? c->skip_en + (size_t)ishot * ntr : NULL);

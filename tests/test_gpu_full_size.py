@@ -9,10 +9,12 @@
 * Batched gradient, 2000 x 400: eight shots with record lengths between 0.4
   and 1.2 s and wavelets low-passed at 15, 30 and 60 Hz (Butterworth order 4)
   in one batch give the seismograms, misfit and gradient of the same shots
-  advanced one at a time, bit for bit.
+  advanced one at a time, bit for bit; the shortest of them against the FP32
+  and FP64 oracles.
 * Reduced-precision snapshots, 4000 x 800, 12 000 steps, every fourth step
   stored: BF16 and FP16 storage against FP32 storage, gradient error printed
-  and bounded.
+  and bounded; on a cropped window of the same model with a 6000-step
+  record, FP32, BF16 and FP16 storage against the FP32 and FP64 oracles.
 The oracle runs on the host cores (OpenMP), tens of seconds in all.
 """
 import os
@@ -117,10 +119,14 @@ def test_visco_1200x300_one_shot_full_record():
     o32, o64 = oracle("f32"), oracle("f64")
     say("case visco 1200x300 FD8 L2, 10000 steps, shot %d: oracle forward of the true model" % k)
     obs, _, _ = o32.forward(cfg, W["vs_true"], W["rho"], W["qs"], src_i, src_j, sig[0], W["rec_i"], W["rec_j"])
+    say("  oracle forward f64")
+    obs64, _, _ = o64.forward(cfg, W["vs_true"], W["rho"], W["qs"], src_i, src_j, sig[0], W["rec_i"], W["rec_j"])
     ctx.set_model(W["vs_true"], W["rho"], W["qs"])
     seis = ctx.forward()[0]
-    e32 = rel_l2(seis, obs)
-    say("  seismograms rel L2 vs oracle f32 %.3e" % e32)
+    e32, e64 = rel_l2(seis, obs), rel_l2(seis, obs64)
+    say("  seismograms rel L2 vs oracle f32 %.3e, vs f64 %.3e (oracle f32 vs f64 %.3e)" % (
+        e32, e64, rel_l2(obs, obs64)))
+    assert e64 <= 1e-3
     assert np.max(np.abs(obs)) > 0 and e32 <= SEIS_TOL
     obs = np.ascontiguousarray(obs[None], np.float32)
     ctx.set_model(W["vs_start"], W["rho"], W["qs"])
@@ -183,6 +189,28 @@ def test_batched_uneven_stages_2000x400_equals_one_at_a_time():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     assert a[3] == b[3] and a[4] == b[4]
     ctx.close()
+    # the shortest shot of the batch against the oracles, alone
+    from pyoracle import make_cfg
+    k = int(np.argmin(nts))
+    ntk = int(nts[k])
+    kw = dict(fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=2, dt=dt, dh=dh, damping=350.0, fpml=fc, dtinv=4)
+    c1 = dn.Context(nx, ny, fl=[6.25, 50.0], **kw)
+    c1.set_receivers(rec_i, rec_j)
+    c1.set_shots(src_i[k:k + 1], src_j[k:k + 1], sig[k:k + 1, :ntk])
+    c1.set_model(vs0, rho, qs)
+    ob = np.ascontiguousarray(obs[k:k + 1, :, :ntk], np.float32)
+    c1.set_observed(ob)
+    c1.gradient()
+    gvs, grho, _, _ = c1.get_gradient()
+    cfg = make_cfg(nx, ny, nt=ntk, fl=[6.25, 50.0], **kw)
+    say("  shot %d (%d steps, %g Hz stage) against the oracles f32 and f64" % (k, ntk, stages[k % 3]))
+    o32, o64 = oracle("f32"), oracle("f64")
+    r32 = o32.gradient(cfg, vs0, rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k:k + 1, :ntk], rec_i, rec_j, ob)
+    r64 = o64.gradient(cfg, vs0, rho, qs, src_i[k:k + 1], src_j[k:k + 1], sig[k:k + 1, :ntk], rec_i, rec_j, ob)
+    ev = _report_grad("grad_vs ", gvs, r32["grad_vs"], r64["grad_vs"])
+    er = _report_grad("grad_rho", grho, r32["grad_rho"], r64["grad_rho"])
+    assert ev <= GRAD_TOL and er <= GRAD_TOL
+    c1.close()
 
 
 def test_reduced_precision_snapshots_4000x800():
@@ -224,3 +252,63 @@ def test_reduced_precision_snapshots_4000x800():
         say("  %s vs fp32 storage: grad_vs rel L2 %.3e, grad_rho rel L2 %.3e" % (["", "bf16", "fp16"][prec], ev, er))
         assert ev <= tol and er <= tol
         assert grads[prec][2] == g0[2]          # the misfit comes from the forward pass, not the snapshots
+
+
+def test_4000x800_window_against_fp64_oracle():
+    """The reduced-precision workload's grid spacing, time step and model on
+    a cropped window (600 x 300 cells: x 319-379 m, the top 30 m, over the
+    shallowest lens) with a 6000-step record, one shot, every fourth step
+    stored, FP32 / BF16 / FP16 snapshot storage.  Against the FP32 oracle:
+    FP32 storage within the gradient bar.  Against the FP64 oracle (the
+    observed data modelled in FP64 as well): two residuals.
+    (a) The workload's own: layered start model, the lens the only
+        difference; the residual is ~3e-6 of the data, the size of FP32
+        rounding in a 6000-step simulation, so the FP32 gradient (the
+        reference's arithmetic) differs from the FP64 one by tens of
+        percent -- printed, not bounded.
+    (b) The start model 5 % slower everywhere: a residual of the data's
+        size; the FP32 gradient is within 1e-3 of the FP64 one, BF16 and
+        FP16 storage within 1e-2."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from pyoracle import make_cfg
+    dn = product()
+    dh, dt, nt, fc = 0.1, 5e-5, 6000, 60.0
+    vt, v0, rh, q = bench.layered_lenses(4000, 800, dh, seed=5)
+    win = (slice(0, 300), slice(3190, 3790))
+    vs_true, vs_layer, rho, qs = (np.ascontiguousarray(a[win]) for a in (vt, v0, rh, q))
+    ny, nx = vs_true.shape
+    src_i, src_j = np.array([300], np.int32), np.array([5], np.int32)
+    rec_i = np.arange(30, nx - 30, 4, dtype=np.int32)
+    rec_j = np.full_like(rec_i, 5)
+    sig = dn.wavelet(1, nt, dt, 0.0, fc, 1.0)[None].astype(np.float32)
+    kw = dict(fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=2, dt=dt, dh=dh, damping=350.0, fpml=fc, dtinv=4)
+    cfg = make_cfg(nx, ny, nt=nt, fl=[6.25, 50.0], **kw)
+    o32, o64 = oracle("f32"), oracle("f64")
+    say("case 4000x800 model window %dx%d, 1 shot x %d steps: observed data (oracle f32, f64)" % (nx, ny, nt))
+    obs = np.ascontiguousarray(o32.forward(cfg, vs_true, rho, qs, src_i, src_j, sig[0], rec_i, rec_j)[0][None],
+                               np.float32)
+    obs64 = o64.forward(cfg, vs_true, rho, qs, src_i, src_j, sig[0], rec_i, rec_j)[0][None]
+    for name, vs0 in (("a: layered start model", vs_layer), ("b: start model 5% slow", (0.95 * vs_true).astype(np.float32))):
+        say("  %s: oracle gradients f32, f64" % name)
+        r32 = o32.gradient(cfg, vs0, rho, qs, src_i, src_j, sig, rec_i, rec_j, obs)
+        r64 = o64.gradient(cfg, vs0, rho, qs, src_i, src_j, sig, rec_i, rec_j, obs64)
+        say("    residual l2 %.3e (fp32) %.3e (fp64); data energy %.3e" % (r32["l2"], r64["l2"], r64["energy"]))
+        assert r32["l2"] > 0 and np.all(np.isfinite(r32["grad_vs"])) and np.max(np.abs(r32["grad_vs"])) > 0
+        for prec in (0, 1, 2):
+            ctx = dn.Context(nx, ny, fl=[6.25, 50.0], snap_precision=prec, **kw)
+            ctx.set_receivers(rec_i, rec_j)
+            ctx.set_shots(src_i, src_j, sig)
+            ctx.set_model(vs0, rho, qs)
+            ctx.set_observed(obs)
+            ctx.gradient()
+            gvs, grho, _, _ = ctx.get_gradient()
+            tag = ["fp32", "bf16", "fp16"][prec]
+            ev = _report_grad("%s grad_vs " % tag, gvs, r32["grad_vs"], r64["grad_vs"])
+            er = _report_grad("%s grad_rho" % tag, grho, r32["grad_rho"], r64["grad_rho"])
+            if prec == 0:
+                assert ev <= GRAD_TOL and er <= GRAD_TOL
+            if name.startswith("b"):
+                bound = 1e-3 if prec == 0 else 1e-2
+                assert rel_l2(gvs, r64["grad_vs"]) <= bound and rel_l2(grho, r64["grad_rho"]) <= bound
+            ctx.close()

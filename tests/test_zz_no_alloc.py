@@ -4,7 +4,6 @@ with fewer traces and samples, bit-identical residuals and equal FP64 sums;
 the maximum trace count is enforced.  Device memory comes from the HIP
 runtime the library itself is linked against."""
 import ctypes as C
-import os
 import sys
 
 import numpy as np
@@ -19,8 +18,9 @@ H2D, D2H = 1, 2   # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
 
 def _hip():
     product().lib()    # loads the library and with it its HIP runtime
-    path = "/opt/rocm/lib/libamdhip64.so.7"
-    hip = C.CDLL(path if os.path.exists(path) else "libamdhip64.so.7")
+    # by soname: the runtime copy already loaded for the library (a second
+    # copy opened by path would not share its device state)
+    hip = C.CDLL("libamdhip64.so.7")
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipFree.argtypes = [C.c_void_p]
     hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
