@@ -99,8 +99,8 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
         dn_fail(nullptr, "dn_create: snap_precision must be 0, 1 or 2");
         return nullptr;
     }
-    if (p->eprecond != 0 && p->eprecond != 1) {
-        dn_fail(nullptr, "dn_create: eprecond must be 0 or 1");
+    if (p->eprecond != 0 && p->eprecond != 1 && p->eprecond != 3) {
+        dn_fail(nullptr, "dn_create: eprecond must be 0, 1 or 3");
         return nullptr;
     }
     dn_ctx *c = new dn_ctx();
@@ -158,7 +158,7 @@ extern "C" dn_ctx *dn_create(const dn_params *p)
               dalloc(&c->d_grho_out, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_ws_sum, (size_t)p->nx * p->ny) == hipSuccess &&
               dalloc(&c->d_wr_sum, (size_t)p->nx * p->ny) == hipSuccess &&
-              dalloc(&c->d_maxbits, 4096) == hipSuccess;
+              dalloc(&c->d_maxbits, 4096) == hipSuccess && dalloc(&c->d_status, 4) == hipSuccess;
     if (!ok) {
         dn_fail(nullptr, "dn_create: out of device memory for the model");
         dn_destroy(c);
@@ -220,6 +220,7 @@ extern "C" void dn_destroy(dn_ctx *c)
     dfree_t(c->d_spk);
     dfree_t(c->d_d); dfree_t(c->d_gu); dfree_t(c->d_grho); dfree_t(c->d_gvs_out); dfree_t(c->d_grho_out);
     dfree_t(c->d_maxbits);
+    dfree_t(c->d_status);
     dfree_t(c->d_trace_flags);
     dfree_t(c->d_rec_row); dfree_t(c->d_rec_tab);
     dfree_t(c->d_shot_nt); dfree_t(c->d_src_off); dfree_t(c->d_src_i); dfree_t(c->d_src_j); dfree_t(c->d_sig);
@@ -820,6 +821,7 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a.pvx = c->d_pvx;
     a.pvy = c->d_pvy;
     a.shot_map = c->d_shot_map;
+    a.status = c->d_status;
     a.shot_nt = c->d_shot_nt;
     a.src_off = c->d_src_off;
     a.src_i = c->d_src_i;
@@ -993,6 +995,7 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
         if (!c->d_ntbase) HIPCHK(c, hipMalloc((void **)&c->d_ntbase, 256));
         if (get_graph(c, a, n, adjoint, store, record, pack_mode, nsnap, CH, corr_res, fwd_extra, &exec)) return -1;
     }
+    HIPCHK(c, launch_fill_u32((unsigned int *)c->d_status, 4, 0u, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int cur = 0;
     if (nchunks > 0) {
@@ -1016,14 +1019,14 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     c->launches_pass[adjoint ? 1 : 0] += ntmax;
     c->cells_pass[adjoint ? 1 : 0] += cells;
     c->cur_final = cur;
-    // NaN probe of the centre cell (denise.c:1183-1185)
-    std::vector<float> centre(n);
-    const long long off = gidx(c->g, c->g.ny / 2, c->g.nx / 2);
-    for (int s = 0; s < n; ++s)
-        HIPCHK(c, hipMemcpy(&centre[s], c->d_vy[cur] + (size_t)s * c->g.plane + off, sizeof(float),
-                            hipMemcpyDeviceToHost));
-    for (int s = 0; s < n; ++s)
-        if (std::isnan(centre[s])) return dn_fail(c, "Simulation is unstable ! (shot %d)", shots[s] + 1);
+    // the kernel's latch: NaN at the probe cell after some step
+    // (denise.c:1183-1185 probes that cell before every step), or a shot map
+    // the batch did not upload
+    int status[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpy(status, c->d_status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status[2]) return dn_fail(c, "step kernel: a batch slot held no valid shot (shot map not uploaded)");
+    if (status[0])
+        return dn_fail(c, "Simulation is unstable ! (time step %d, shot %d)", status[0] + 1, status[1]);
     return 0;
 }
 
@@ -1173,7 +1176,7 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
         if (dn_run_pass(c, part, false, true, true)) return -1;
         if (dn_batch_residual(c, part)) return -1;
         if (dn_run_pass(c, part, true, false, false)) return -1;
-        HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->d_Ws, c->d_Wr, c->d_gu_shot,
+        HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr, c->d_gu_shot,
                                              c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits,
                                              c->p.eprecond, c->d_ws_sum, c->d_wr_sum, b0 == 0 && !accumulate,
                                              c->stream));

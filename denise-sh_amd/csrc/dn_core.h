@@ -38,6 +38,7 @@ struct dn_ctx {
     // gradient sums [ny][nx] and outputs after the chain rule
     float *d_gu = nullptr, *d_grho = nullptr, *d_gvs_out = nullptr, *d_grho_out = nullptr;
     unsigned int *d_maxbits = nullptr;
+    int *d_status = nullptr;         // instability / shot-map latch of the step kernel (StepArgs::status)
     // receivers
     int ntr = 0;
     std::vector<int> rec_i, rec_j;
@@ -92,6 +93,7 @@ struct dn_ctx {
     // learned preconditioner (owned by the caller) and the judge interval
     dn_unet *unet = nullptr;
     int judge_every = 5;
+    bool learned_fixed = false;      // D from the first iteration of a stage, kept for the stage
     // gradient smoothing / taper workspace (dn_gradproc.hip)
     dn_gradproc_state *gradproc = nullptr;
     // data conditioning of the residual (dn_set_datacond): parameters and the

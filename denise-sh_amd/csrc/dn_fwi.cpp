@@ -169,6 +169,7 @@ struct dn_fwi_state {
     float *D = nullptr, *r2 = nullptr, *gu_l = nullptr, *grho_l = nullptr, *gu_p = nullptr, *grho_p = nullptr;
     float *feat = nullptr, *pred = nullptr;
     double last_plain_drop = -1.0;
+    bool D_valid = false;      // D holds this stage's learned diagonal (fixed mode)
 };
 
 static void fwi_free(dn_fwi_state *f)
@@ -319,6 +320,7 @@ extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
     for (float *z : zs) HIPCHK(c, launch_fill(z, v2, 0.0f, c->stream));
     f->iter = 1;
     f->lbfgs_ptr = 1;
+    f->D_valid = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -584,6 +586,14 @@ extern "C" int dn_precond_features(dn_ctx *c, float *x)
     return 0;
 }
 
+extern "C" int dn_fwi_set_learned_fixed(dn_ctx *c, int fixed)
+{
+    if (!c) return -1;
+    c->learned_fixed = fixed != 0;
+    if (c->fwi) c->fwi->D_valid = false;
+    return 0;
+}
+
 extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
 {
     if (!c) return -1;
@@ -709,7 +719,10 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
             float **bs[] = {&f->gu_l, &f->grho_l, &f->gu_p, &f->grho_p};
             for (float **b : bs) HIPCHK(c, hipMalloc((void **)b, n * sizeof(float)));
         }
-        if (learned_diagonal(c)) return -1;
+        if (!(c->learned_fixed && f->D_valid)) {
+            if (learned_diagonal(c)) return -1;
+            f->D_valid = true;
+        }
     }
     if (f->o.grad_method == 1) {
         if (pcg_direction(c)) return -1;

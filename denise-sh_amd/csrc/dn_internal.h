@@ -129,6 +129,11 @@ struct StepArgs {
     int hess;                               // forward: accumulate the pseudo-Hessian (hess_save.c, hessian.c)
     int nt;                                 // current step, 1-based (offset within a graph chunk if nt_base)
     const int *nt_base;                     // graph replay: device step counter added to nt, or null
+    // latched by the kernel: [0] the first step after which vy at the probe
+    // cell (NY/2, NX/2) is NaN (denise.c:1183-1185 checks that cell before
+    // every step; 0: none), [1] that shot + 1, [2] nonzero: a shot-map
+    // entry was out of range (the batch slot was skipped)
+    int *status;
 };
 
 // launchers (dn_kernels.hip)
@@ -161,9 +166,9 @@ hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, cons
                            const unsigned char *flags, hipStream_t st);
 
 // energy preconditioning + shot accumulation (eprecond1.c, denise.c:1665-1694)
-// (precond 0: EPRECOND 0, plain sum).  ws_sum/wr_sum collect the energy maps
+// (precond 0: EPRECOND 0, plain sum; 1 and 3: EPRECOND 1 and 3).  ws_sum/wr_sum collect the energy maps
 // over shots (reset when first).
-hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
+hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, float DH, const float *Ws, const float *Wr,
                                       const float *gu_shot, const float *grho_shot, float *gu, float *grho,
                                       float C_vs, float C_rho, unsigned int *maxbits, int precond, float *ws_sum,
                                       float *wr_sum, int first, hipStream_t st);

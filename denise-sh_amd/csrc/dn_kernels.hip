@@ -151,6 +151,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 Sx[u] = S;
                 NTs[u] = a.shot_nt[S];
                 act[u] = nt <= NTs[u];          // finished shots (uneven record lengths) rest
+            } else {
+                a.status[2] = 1;                // reported by the host after the pass
             }
         }
         any = any || act[u];
@@ -512,6 +514,10 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 }
             }
             vnew[u] = vn;
+            // instability latch at the reference's probe cell (row test first:
+            // wave-uniform, true on one row of the grid)
+            if (jv == (g.ny >> 1) && own_v && c == (g.nx >> 1) && !(vn == vn))
+                if (atomicCAS(&a.status[0], 0, nt) == 0) a.status[1] = Sx[u] + 1;
             if (own_v) {
                 stf(vyo, sof[u] + gi * 4u, vn);
                 if (!ADJ && a.seis != nullptr) {
@@ -910,20 +916,37 @@ hipError_t launch_residual(int nshots, int nshots_all, int ntr, int nt_max, cons
 // We += 0.005*max(We), scale the shot gradients and add them to the sums in
 // shot order (deterministic).
 // ---------------------------------------------------------------------------
-__global__ void k_we_max(int nxny, const float *Ws, const float *Wr, unsigned int *maxbits)
+// energy weight of one cell (eprecond1.c:15-50): EPRECOND 1 sqrt(Ws Wr);
+// EPRECOND 3 sqrt(Ws) times the Plessix-Mulder (2004) receiver-side
+// approximation asinh((xmax - x)/y) - asinh((xmin - x)/y) for receivers
+// along the surface from xmin = DH to xmax = NX DH, at x = i DH, y = j DH
+// (float coordinates, double asinh and sqrt as the reference's C calls)
+__device__ __forceinline__ float energy_weight(int precond, float ws, float wr, int t, int nx, float DH)
+{
+    if (precond == 3) {
+        const int i = t % nx + 1, j = t / nx + 1;
+        const float x = (float)i * DH, y = (float)j * DH, xmin = DH, xmax = (float)nx * DH;
+        return (float)(sqrt((double)ws) * (asinh((double)((xmax - x) / y)) - asinh((double)((xmin - x) / y))));
+    }
+    return (float)sqrt((double)(ws * wr));
+}
+
+__global__ void k_we_max(int nxny, int nx, float DH, int precond, const float *Ws, const float *Wr,
+                         unsigned int *maxbits)
 {
     const int s = blockIdx.y;
     float m = 0.0f;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nxny; t += gridDim.x * blockDim.x) {
         const long long o = (long long)s * nxny + t;
-        const float we = (float)sqrt((double)(Ws[o] * Wr[o]));
+        const float we = energy_weight(precond, Ws[o], Wr[o], t, nx, DH);
         m = fmaxf(m, we);
     }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[s], __float_as_uint(m));
 }
 
-__global__ void k_eprecond_acc(int nshots, int nxny, const float *Ws, const float *Wr, const float *gu_shot,
+__global__ void k_eprecond_acc(int nshots, int nxny, int nx, float DH, const float *Ws, const float *Wr,
+                               const float *gu_shot,
                                const float *grho_shot, float *gu, float *grho, float C_vs, float C_rho,
                                const unsigned int *maxbits, int precond, float *ws_sum, float *wr_sum, int first)
 {
@@ -937,7 +960,7 @@ __global__ void k_eprecond_acc(int nshots, int nxny, const float *Ws, const floa
         es = es + Ws[o];
         er = er + Wr[o];
         if (precond) {
-            float we = (float)sqrt((double)(Ws[o] * Wr[o]));
+            float we = energy_weight(precond, Ws[o], Wr[o], t, nx, DH);
             we = we + (EPSILON_WE * __uint_as_float(maxbits[s]));
             const float su = gu_shot[o] / (we * C_vs * C_vs);
             const float sr = grho_shot[o] / (we * C_rho * C_rho);
@@ -954,7 +977,7 @@ __global__ void k_eprecond_acc(int nshots, int nxny, const float *Ws, const floa
     wr_sum[t] = er;
 }
 
-hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *Ws, const float *Wr,
+hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, float DH, const float *Ws, const float *Wr,
                                       const float *gu_shot, const float *grho_shot, float *gu, float *grho,
                                       float C_vs, float C_rho, unsigned int *maxbits, int precond, float *ws_sum,
                                       float *wr_sum, int first, hipStream_t st)
@@ -963,12 +986,12 @@ hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, const float *W
     if (precond) {
         hipError_t e = launch_fill_u32(maxbits, (size_t)nshots, 0u, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, Ws, Wr, maxbits);
+        hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, nx, DH, precond, Ws, Wr, maxbits);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_eprecond_acc, dim3((nxny + 255) / 256), dim3(256), 0, st, nshots, nxny, Ws, Wr, gu_shot,
-                       grho_shot, gu, grho, C_vs, C_rho, maxbits, precond, ws_sum, wr_sum, first);
+    hipLaunchKernelGGL(k_eprecond_acc, dim3((nxny + 255) / 256), dim3(256), 0, st, nshots, nxny, nx, DH, Ws, Wr,
+                       gu_shot, grho_shot, gu, grho, C_vs, C_rho, maxbits, precond, ws_sum, wr_sum, first);
     return hipGetLastError();
 }
 

@@ -180,7 +180,9 @@ typedef struct dn_params {
     float npower, k_max_pml;
     int dtinv;           /* DTINV: snapshot every dtinv-th step            */
     int snap_precision;  /* 0: fp32, 1: bf16, 2: fp16 (per-snapshot scale) */
-    int eprecond;        /* EPRECOND: 1 energy preconditioning (eprecond1.c), 0 none */
+    int eprecond;        /* EPRECOND: 1 energy preconditioning sqrt(Ws Wr), 3 sqrt(Ws)
+                            times the Plessix-Mulder receiver approximation
+                            (eprecond1.c:15-50), 0 none */
 } dn_params;
 
 /* Context on the current HIP device; NULL on failure (dn_last_error(NULL)).
@@ -578,6 +580,12 @@ int dn_fwi_load_state(dn_ctx *ctx, const char *path, const char *jacobian, int n
  * first) and the plain L-BFGS direction, with the same history, after.
  * The network stays owned by the caller.  Not in the reference. */
 int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
+/* fixed 1: the learned diagonal D is predicted on the first iteration of a
+ * stage and kept for the whole stage (a fixed initial Hessian, so the L-BFGS
+ * pairs and the scaling stay consistent); 0 (default): predicted again at
+ * every iteration from the current energy maps and model.  Not in the
+ * reference. */
+int dn_fwi_set_learned_fixed(dn_ctx *ctx, int fixed);
 /* The network's input maps from the last gradient and the current model,
  * host [3][H8][W8] with H8, W8 = ny, nx rounded up to multiples of 8 (edge
  * rows/columns repeated): log10 of sum Ws and sum Wr relative to their

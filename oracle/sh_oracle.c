@@ -66,7 +66,8 @@ typedef struct {
     double dt, dh;
     double damping, fpml, npower, k_max_pml;
     double fl[8];        /* relaxation frequencies FL[1..L] at fl[0..L-1]   */
-    int eprecond;        /* EPRECOND: 1 = eprecond1 energy scaling, 0 = none */
+    int eprecond;        /* EPRECOND: 1 = eprecond1 energy scaling sqrt(Ws Wr), 3 = the receiver
+                            approximation (eprecond1.c:34-50), 0 = none */
     double c_vs, c_rho;  /* > 0: the normalisation means C_vs, C_rho (fixed at
                             the first FWI iteration, denise.c:745-816);
                             0: the means of the model passed in            */
@@ -1219,11 +1220,20 @@ static int gradient_impl(const orc_cfg *c, const float *vs, const float *rho, co
                 }
             continue;
         }
-        /* eprecond1.c:15-70 */
+        /* eprecond1.c:15-70: EPRECOND 1 sqrt(Ws Wr); EPRECOND 3 sqrt(Ws)
+         * times asinh((xmax-x)/y) - asinh((xmin-x)/y) (eprecond1.c:34-50,
+         * float coordinates x = i DH, y = j DH, xmin = DH, xmax = NX DH) */
         maxWe = 0.0;
         for (i = 1; i <= NX; i++)
             for (j = 1; j <= NY; j++) {
-                AT(We, c, j, i) = sqrt(AT(Ws, c, j, i) * AT(Wr, c, j, i));
+                if (c->eprecond == 3) {
+                    const float DHf = (float)c->dh, x = (float)i * DHf, y = (float)j * DHf;
+                    const float xmin = DHf, xmax = (float)NX * DHf;
+                    AT(We, c, j, i) = (real)(sqrt((double)AT(Ws, c, j, i)) *
+                                             (asinh((double)((xmax - x) / y)) - asinh((double)((xmin - x) / y))));
+                } else {
+                    AT(We, c, j, i) = sqrt(AT(Ws, c, j, i) * AT(Wr, c, j, i));
+                }
                 if (AT(We, c, j, i) > maxWe) maxWe = AT(We, c, j, i);
             }
         for (i = 1; i <= NX; i++)

@@ -78,8 +78,10 @@ def test_calls_reject_bad_input_and_the_context_stays_usable():
 
 
 def test_unstable_time_step_is_reported():
-    """DT ten times above the stability limit: the pass ends with a NaN in
-    the centre cell and the call fails with the reference's message."""
+    """DT ten times above the stability limit: the step kernel latches the
+    first step after which the reference's probe cell (NY/2, NX/2) holds a
+    NaN, and the call fails with the reference's message and that step,
+    long before the end of the 3000-step record."""
     dn, ctx = _ctx(dt=2e-3)
     vs, rho, _ = model(80, 40, DH, seed=2)
     ctx.set_model(vs, rho)
@@ -90,4 +92,7 @@ def test_unstable_time_step_is_reported():
         ctx.forward()
     say("  %s" % e.value)
     assert "unstable" in str(e.value)
+    import re
+    step = int(re.search(r"time step (\d+)", str(e.value)).group(1))
+    assert 1 < step < 3000
     ctx.close()

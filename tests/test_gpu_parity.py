@@ -120,14 +120,18 @@ GRAD_CASES = [
     ("el_fd4_dtinv1", 100, 44, 4, 0, 1, 10, 800, 60.0, 1),
     ("visc_fd8_dtinv3", 100, 44, 8, 2, 1, 10, 800, 60.0, 3),
     ("visc_fd8_nofs_dtinv4", 96, 40, 8, 2, 0, 10, 700, 60.0, 4),
+    # EPRECOND 3 (eprecond1.c:34-50): the asinh receiver approximation in double
+    # on the device and in the oracle's C library
+    ("visc_fd8_eprecond3", 100, 44, 8, 2, 1, 10, 800, 60.0, 3, 3),
 ]
 
 
 @pytest.mark.parametrize("case", GRAD_CASES, ids=[c[0] for c in GRAD_CASES])
 def test_gradient(case):
-    name, nx, ny, fdorder, L, fs, fw, nt, fc, dtinv = case
+    name, nx, ny, fdorder, L, fs, fw, nt, fc, dtinv = case[:10]
+    eprecond = case[10] if len(case) > 10 else 1
     say("case", name)
-    ctx, cfg, d = setup_case(nx, ny, fdorder, L, fs, fw, nt, fc, nshots=2, dtinv=dtinv)
+    ctx, cfg, d = setup_case(nx, ny, fdorder, L, fs, fw, nt, fc, nshots=2, dtinv=dtinv, eprecond=eprecond)
     # observed data: the true model is 3% faster with an extra +10% box in vs
     vt = d["vs"] * np.float32(1.03)
     vt[ny // 4: ny // 2, nx // 3: 2 * nx // 3] *= np.float32(1.1)

@@ -696,6 +696,53 @@ def test_eprecond1_scaling_and_normalisation_means():
         assert rel_l2(g12[key], g1[key] + g1b[key]) < 1e-12, key
 
 
+def test_eprecond3_receiver_approximation():
+    """EPRECOND 3 (eprecond1.c:34-50): per shot g3 = g0 / (We3 C^2) with
+    We3 = sqrt(Ws) (asinh((xmax - x)/y) - asinh((xmin - x)/y)) + 0.005 max,
+    the Plessix-Mulder (2004) approximation of the receiver-side energy for
+    receivers along the whole surface (xmin = DH, xmax = NX DH, x = i DH,
+    y = j DH).  Ws from the oracle's own forward, the rest in numpy: the
+    ratio of the EPRECOND 0 and 3 gradients must be We3 C^2 in every cell;
+    receiver energy Wr plays no part (a different receiver line leaves g3 /
+    g0 unchanged)."""
+    nx, ny, nt, dh, dt = 60, 40, 500, 0.25, 1e-4
+    kw = dict(nx=nx, ny=ny, fdorder=4, maxrelerror=1, fw=10, free_surf=1, L=0, nt=nt, dtinv=2, dt=dt, dh=dh,
+              damping=300.0, fpml=30.0)
+    c0, c3 = cfg(eprecond=0, **kw), cfg(eprecond=3, **kw)
+    rng = np.random.default_rng(11)
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    vs = (180.0 + 60.0 * jj / ny + 8.0 * rng.standard_normal((ny, nx))).astype(np.float32)
+    rho = (1500.0 + 400.0 * ii / nx + 30.0 * rng.standard_normal((ny, nx))).astype(np.float32)
+    t = np.arange(1, nt + 1) * dt
+    tau = np.pi * (t - 1.5 / 30.0) / (1.5 / 30.0)
+    w = ((1 - 4 * tau ** 2) * np.exp(-2 * tau ** 2)).astype(np.float32)
+    si, sj = np.array([25], np.int32), np.array([3], np.int32)
+    vt = vs.copy()
+    vt[12:22, 22:40] *= np.float32(1.07)
+    Ws = None
+    ratios = []
+    for ri in (np.arange(12, 50, 3, dtype=np.int32), np.arange(5, 30, 2, dtype=np.int32)):
+        rj = np.full_like(ri, 2)
+        obs = O64.forward(c0, vt, rho, None, si, sj, w, ri, rj)[0][None].astype(np.float32)
+        g0 = O64.gradient(c0, vs, rho, None, si, sj, w[None], ri, rj, obs)
+        g3 = O64.gradient(c3, vs, rho, None, si, sj, w[None], ri, rj, obs)
+        if Ws is None:
+            nd = 3
+            Ws = O64.forward(c0, vs, rho, None, si, sj, w, ri, rj, want_snaps=True)[2][nd:nd + ny, nd:nd + nx]
+        m = np.abs(g0["raw_u"]) > 1e-6 * np.abs(g0["raw_u"]).max()
+        ratios.append((g0["raw_u"][m] / g3["raw_u"][m], m))
+    f = np.float32
+    x = (np.arange(1, nx + 1, dtype=np.float32) * f(dh))[None, :]
+    y = (np.arange(1, ny + 1, dtype=np.float32) * f(dh))[:, None]
+    xmin, xmax = f(dh), f(nx) * f(dh)
+    w3 = np.sqrt(Ws) * (np.arcsinh(((xmax - x) / y).astype(np.float64)) - np.arcsinh(((xmin - x) / y).astype(np.float64)))
+    we3 = w3 + 0.005 * w3.max()
+    cvs = float(np.mean(vs.astype(np.float64)))
+    for r, m in ratios:
+        np.testing.assert_allclose(r / cvs ** 2, we3[m], rtol=1e-9)
+    assert we3.max() / we3.min() > 10.0     # the weight varies: the check sees the geometry
+
+
 def test_pseudo_hessian_is_sum_of_squared_snapshots():
     """hessian.c: H_mu = sum over stored steps 1..NTDTINV-1 of uy^2 + (uyx/2)^2,
     H_rho = sum vyp1^2, running on across shots; hessian_out.c chain rule.
