@@ -35,6 +35,10 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3] [
        [--nt NT] [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu]
        [--rows R] [--max-batch B] [--autotune] [--exec graph|eager]
        python bench.py --config c2 --fwi N     (N plain L-BFGS iterations, wall time)
+       python bench.py --config c2 --fwi-learned N [--pairs P] [--probes Q] [--label-shots S]
+                                               (learned preconditioner: training, then N FWI
+                                                iterations plain and learned, time to plain's misfit)
+       python bench.py --unet N                (N training steps of the preconditioner's U-Net)
        python bench.py --copy-probe            (copy kernel that scales HBM counters)
 """
 import argparse
@@ -232,6 +236,24 @@ def measured_traffic(W, batch, cells_per_launch):
     return None
 
 
+def make_context(dn, W, args, nshots, prec, eprecond=1, shots=None):
+    """A context of the workload with the execution settings of the command
+    line; shots (indices) selects a subset of the acquisition."""
+    ctx = dn.Context(W["nx"], W["ny"], fdorder=W["fdorder"], maxrelerror=1, fw=W["fw"], free_surf=1, L=W["L"],
+                     fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=W["damping"], fpml=W["fc"], dtinv=W["dtinv"],
+                     snap_precision=prec, eprecond=eprecond)
+    ctx.set_exec(-1 if args.exec == "graph" else 0)
+    ctx.set_wave_shots(args.wave_shots)
+    if args.rows or args.max_batch:
+        ctx.set_batch(args.max_batch, args.rows)
+    ctx.set_group_layout(args.layout)
+    ctx.set_receivers(W["rec_i"], W["rec_j"])
+    sel = np.arange(nshots) if shots is None else np.asarray(shots)
+    sig = signals_of(dn, W, nshots)[sel]
+    ctx.set_shots(W["src_i"][sel], W["src_j"][sel], sig, W["nts"][sel] if "nts" in W else None)
+    return ctx
+
+
 def fwi_run(ctx, W, args):
     """Plain L-BFGS FWI of the workload (GRAD_METHOD 2, NLBFGS 20, EPRECOND 1,
     vs only, test shots 1, 11, 21, 31 for the line search) from the starting
@@ -261,6 +283,141 @@ def fwi_run(ctx, W, args):
                       "vs_rel_error": {"start": round(err0, 5), "end": round(err, 5)},
                       "config": {"workload": "visco SH FWI %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d"
                                              % (W["nx"], W["ny"], nshots, W["nt"], W["dtinv"]), "n_gpus": 1}}))
+
+
+LEARNED_MODES = {
+    # name: (judge_every, D fixed per stage)
+    "plain": None,
+    "learned_fixed": (0, True),
+    "learned_fixed_judge_every": (1, True),
+    "learned_recomputed": (0, False),
+}
+
+
+def fwi_learned_run(dn, W, args, nshots, prec):
+    """The learned preconditioner on the workload's FWI: training pairs from
+    seeded models of the same family (random_checkerboard) on the same grid,
+    with --label-shots of the shots and EPRECOND 0, labels from --probes
+    finite-difference Hessian probes; --train-steps Adam steps; then
+    --fwi-learned iterations of plain L-BFGS and of each learned mode from
+    the same start (fresh contexts, eprecond 1, test shots 1, 11, 21, 31),
+    and the wall time each needs to reach the plain run's final misfit."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("precond_train", os.path.join(ROOT, "denise-sh_amd",
+                                                                                 "precond_train.py"))
+    pt = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(pt)
+    log = lambda m: print("fwi-learned: " + m, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    # training pairs on the grid and acquisition of --train-config (default:
+    # the FWI workload's own), full record length
+    WT = W if args.train_config in (None, W["config"]) else workload(args.train_config)
+    nshots_t = len(WT["src_i"])
+    lsel = np.linspace(0, nshots_t - 1, min(args.label_shots, nshots_t)).round().astype(int)
+    lab = make_context(dn, WT, args, nshots_t, prec, eprecond=0, shots=lsel)
+    models = [pt.random_checkerboard(WT["nx"], WT["ny"], WT["dh"], 7000 + k) for k in range(args.pairs)]
+    X, T = pt.pairs_on_context(lab, models, nprobes=args.probes, seed0=100, log=log)
+    lab.close()
+    t_labels = time.perf_counter() - t0
+    ntr = max(1, (4 * args.pairs) // 5)
+    net = dn.UNet(seed=1)
+    t1 = time.perf_counter()
+    hist = pt.train(net, X[:ntr], T[:ntr], steps=args.train_steps, lr=2e-3, batch=4)
+    t_train = time.perf_counter() - t1
+    held = {"pairs": int(args.pairs - ntr)}
+    if args.pairs > ntr:
+        held["loss"] = round(float(net.loss_grad(X[ntr:], T[ntr:])[0]), 5)
+        held["constant_predictor"] = round(float(np.mean((np.mean(T[:ntr]) - T[ntr:]) ** 2)), 5)
+    log("labels %.0f s, training %.0f s: loss %.4f -> %.4f, held-out %s" % (t_labels, t_train, hist[0], hist[-1],
+                                                                             held))
+    runs = {}
+    for name, mode in LEARNED_MODES.items():
+        ctx = make_context(dn, W, args, nshots, prec, eprecond=1)
+        ctx.set_model(W["vs_true"], W["rho"], W["qs"])
+        obs = ctx.forward()
+        ctx.set_model(W["vs_start"], W["rho"], W["qs"])
+        ctx.set_observed(obs)
+        del obs
+        if mode is None:
+            ctx.set_learned(None)
+        else:
+            ctx.set_learned(net, mode[0], mode[1])
+        ctx.fwi_start(testshot_start=1, testshot_end=nshots, testshot_incr=10, eps_scale=0.01, stepmax=6,
+                      vs_lower=50.0, vs_upper=600.0)
+        mis, sec, taken = [], [], []
+        ts = time.perf_counter()
+        for it in range(args.fwi_learned):
+            r = ctx.fwi_iteration()
+            mis.append(float(r["L2t"][3]))
+            sec.append(time.perf_counter() - ts)
+            taken.append(int(r["precond"]))
+            log("%s iteration %d: L2t %.6g, direction %s, %.1f s" % (name, it + 1, mis[-1],
+                                                                   "learned" if taken[-1] else "plain", sec[-1]))
+        vs, _ = ctx.fwi_model()
+        ctx.close()
+        runs[name] = {"misfit_L2t": [round(m, 4) for m in mis], "seconds": [round(t, 2) for t in sec],
+                      "learned_taken": "".join(map(str, taken)),
+                      "vs_rel_error": round(float(np.linalg.norm(vs - W["vs_true"]) / np.linalg.norm(W["vs_true"])),
+                                            5)}
+    target = runs["plain"]["misfit_L2t"][-1]
+    t_plain = runs["plain"]["seconds"][-1]
+    for r in runs.values():
+        t = next((t for m, t in zip(r["misfit_L2t"], r["seconds"]) if m <= target), None)
+        r["seconds_to_plain_final_misfit"] = t
+        r["ratio_to_plain"] = None if t is None else round(t / t_plain, 4)
+    print(json.dumps({
+        "metric": "FWI wall time to the plain method's final misfit, learned / plain",
+        "value": runs["learned_fixed"]["ratio_to_plain"], "unit": "ratio", "higher_is_better": False,
+        "n_gpus": 1, "iterations": args.fwi_learned, "plain_final_misfit": target, "plain_seconds": t_plain,
+        "training": {"grid": "%dx%d" % (WT["nx"], WT["ny"]), "model_family": "random_checkerboard seeds 7000+",
+                     "pairs": args.pairs, "train_pairs": ntr, "probes_per_label": args.probes,
+                     "shots_per_label": int(len(lsel)), "adam_steps": args.train_steps,
+                     "loss_first_last": [round(hist[0], 5), round(hist[-1], 5)], "held_out": held,
+                     "label_seconds": round(t_labels, 1), "train_seconds": round(t_train, 1)},
+        "modes": {k: ("plain L-BFGS (eprecond 1)" if v is None else
+                      "learned, judge_every %d, diagonal %s" % (v[0], "fixed per stage" if v[1]
+                                                                else "recomputed every iteration"))
+                  for k, v in LEARNED_MODES.items()},
+        "runs": runs,
+        "config": {"workload": "visco SH FWI %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d"
+                               % (W["nx"], W["ny"], nshots, W["nt"], W["dtinv"]), "n_gpus": 1}}))
+
+
+# (Ci, Co, level) of the U-Net's 3x3 convolutions (denise-sh_amd/csrc/dn_unet.cpp),
+# level l at 1/2^l of the input resolution
+UNET_LAYERS = [(16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 48, 2), (48, 48, 2), (48, 64, 3),
+               (64, 64, 3), (112, 48, 2), (48, 48, 2), (80, 32, 1), (32, 32, 1), (48, 16, 0), (16, 16, 0),
+               (16, 16, 0)]
+
+
+def unet_run(dn, args):
+    """Training steps of the learned preconditioner's U-Net at the input size
+    of the 1200 x 300 workload (4 images of 304 x 1200, seeded inputs and
+    targets): steps per second, and the MFMA work of the convolutions per
+    second (forward, data gradient except for the first layer, weight
+    gradient; padded channels counted as executed)."""
+    rng = np.random.default_rng(11)
+    n, H, W = 4, 304, 1200
+    X = rng.standard_normal((n, 3, H, W)).astype(np.float32)
+    T = rng.uniform(-3.0, 0.0, (n, H, W)).astype(np.float32)
+    net = dn.UNet(seed=1)
+    for _ in range(args.warmup):
+        net.train_step(X, T, 1e-3)
+    t0 = time.perf_counter()
+    for _ in range(args.unet):
+        loss = net.train_step(X, T, 1e-3)
+    wall = time.perf_counter() - t0
+    flops = 0.0
+    for k, (ci, co, lev) in enumerate(UNET_LAYERS):
+        px = n * (H >> lev) * (W >> lev)
+        flops += 2.0 * 9 * ci * co * px * (2 if k == 0 else 3)
+    tf = flops * args.unet / wall / 1e12
+    print(json.dumps({"metric": "U-Net training steps/s", "value": round(args.unet / wall, 3), "unit": "steps/s",
+                      "higher_is_better": True, "n_gpus": 1, "steps": args.unet, "warmup": args.warmup,
+                      "ms_per_step": round(wall / args.unet * 1e3, 3), "dtype": "bf16 (fp32 accumulate)",
+                      "conv_tflop_per_step": round(flops / 1e12, 4), "conv_tflops": round(tf, 3),
+                      "mfma_peak_bf16_tflops": 2500.0, "frac_of_mfma_peak": round(tf / 2500.0, 5),
+                      "loss": loss, "config": {"workload": "U-Net train step, batch %d of %dx%d" % (n, H, W)}}))
 
 
 def copy_probe(nbytes=1 << 30, reps=4):
@@ -301,9 +458,23 @@ def main():
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     ap.add_argument("--fwi", type=int, default=0,
                     help="instead of the gradient metric: time this many L-BFGS FWI iterations of the workload")
+    ap.add_argument("--fwi-learned", type=int, default=0,
+                    help="instead of the gradient metric: train the learned preconditioner, then time this many "
+                         "FWI iterations plain and with each learned mode")
+    ap.add_argument("--pairs", type=int, default=12, help="--fwi-learned: training pairs (4/5 train, 1/5 held out)")
+    ap.add_argument("--probes", type=int, default=4, help="--fwi-learned: Hessian probes per label")
+    ap.add_argument("--label-shots", type=int, default=20, help="--fwi-learned: shots of each label's gradients")
+    ap.add_argument("--train-steps", type=int, default=300, help="--fwi-learned: Adam steps")
+    ap.add_argument("--train-config", default=None, choices=["c4", "c2", "c3"],
+                    help="--fwi-learned: workload whose grid and shots the training pairs use (default: --config)")
+    ap.add_argument("--unet", type=int, default=0,
+                    help="instead of the gradient metric: time this many U-Net training steps")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.unet:
+        unet_run(_package(), args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs (--gpus %d) are not provided" % args.gpus,
@@ -314,17 +485,10 @@ def main():
     W = workload(args.config, args.nt or None)
     nshots = len(W["src_i"]) if args.shots <= 0 else min(args.shots, len(W["src_i"]))
     prec = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snap]
-    ctx = dn.Context(W["nx"], W["ny"], fdorder=W["fdorder"], maxrelerror=1, fw=W["fw"], free_surf=1, L=W["L"],
-                     fl=W["fl"], dt=W["dt"], dh=W["dh"], damping=W["damping"], fpml=W["fc"], dtinv=W["dtinv"],
-                     snap_precision=prec)
-    ctx.set_exec(-1 if args.exec == "graph" else 0)
-    ctx.set_wave_shots(args.wave_shots)
-    if args.rows or args.max_batch:
-        ctx.set_batch(args.max_batch, args.rows)
-    ctx.set_group_layout(args.layout)
-    ctx.set_receivers(W["rec_i"], W["rec_j"])
-    sig = signals_of(dn, W, nshots)
-    ctx.set_shots(W["src_i"][:nshots], W["src_j"][:nshots], sig, W["nts"][:nshots] if "nts" in W else None)
+    if args.fwi_learned:
+        fwi_learned_run(dn, W, args, nshots, prec)
+        return
+    ctx = make_context(dn, W, args, nshots, prec)
     # observed data from the true model (untimed setup, on the device)
     print("bench: %s, observed data from the true model" % describe(W, nshots, args.snap), file=sys.stderr,
           flush=True)

@@ -101,7 +101,7 @@ class FwiRecord(C.Structure):
                 ("epst1", C.c_float * 3), ("L2t", C.c_float * 4), ("M2_vs", C.c_float), ("M2_rho", C.c_float),
                 ("eps_u", C.c_float), ("eps_rho", C.c_float), ("l2", C.c_double), ("energy", C.c_double),
                 ("step1", C.c_int), ("step3", C.c_int), ("test_forwards", C.c_int), ("next_stage", C.c_int),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("judged", C.c_int), ("drop_learned", C.c_float), ("drop_plain", C.c_float)]
 
 
 FWI_DEFAULTS = dict(grad_method=2, nlbfgs=20, eps_scale=0.01, stepmax=6, scalefac=2.0, testshot_start=1,

@@ -733,6 +733,8 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
     int step1 = 0, step3 = 0, nfwd = 0;
     float eps_scale = 0.0f;
     R.precond = 0;
+    R.judged = 0;
+    R.drop_learned = R.drop_plain = -1.0f;
     if (!learned) {
         if (line_search(c, energy, L2t, epst1, &step1, &step3, &eps_scale, &nfwd)) return -1;
     } else {
@@ -754,6 +756,7 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
         float epsl = 0.0f;
         if (line_search(c, energy, L2l, epl, &s1l, &s3l, &epsl, &nfwd)) return -1;
         const double dl = drop_of(L2l, s1l);
+        R.drop_learned = (float)dl;
         const bool judge = (f->iter == 1) || (c->judge_every > 0 && (f->iter - 1) % c->judge_every == 0);
         const bool fail = (s1l == 0);
         bool take_learned = true;
@@ -768,6 +771,8 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
             if (line_search(c, energy, L2p, epp, &s1p, &s3p, &epsp, &nfwd)) return -1;
             const double dp = drop_of(L2p, s1p);
             f->last_plain_drop = dp;
+            R.judged = 1;
+            R.drop_plain = (float)dp;
             take_learned = !fail && dl >= dp;
             if (!take_learned) {
                 std::memcpy(L2t, L2p, sizeof(L2p));

@@ -37,6 +37,49 @@ def random_model(nx, ny, dh, seed, vmin=100.0, vmax=350.0, nanom=6, amp=0.12):
     return vs_true, vs_start, rho, qs
 
 
+def random_checkerboard(nx, ny, dh, seed):
+    """The checkerboard model family with seeded parameters: vs rising
+    linearly with depth from vmin (85..115 m/s) to vmax (320..380 m/s) times
+    1 +- a checkerboard (cells 5..11 m, random horizontal phase, amplitude
+    5..12 %) as the true model, the plain gradient as the starting model;
+    rho 1700 -> 2000; Q = 20."""
+    rng = np.random.default_rng(seed)
+    j = np.arange(ny, dtype=np.float64)[:, None]
+    i = np.arange(nx, dtype=np.float64)[None, :]
+    vmin, vmax = rng.uniform(85.0, 115.0), rng.uniform(320.0, 380.0)
+    v1 = vmin + (vmax - vmin) * j / max(ny - 1, 1) + 0.0 * i
+    cell = rng.uniform(5.0, 11.0) / dh
+    cb = np.where(((np.floor(i / cell + rng.uniform(0.0, 2.0)) + np.floor(j / cell)) % 2) == 0, 1.0, -1.0)
+    vs_true = (v1 * (1.0 + rng.uniform(0.05, 0.12) * cb)).astype(np.float32)
+    rho = (1700.0 + 300.0 * j / max(ny - 1, 1) + 0.0 * i).astype(np.float32)
+    return vs_true, v1.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)
+
+
+def pairs_on_context(ctx, models, nprobes=8, eps=1e-2, seed0=100, log=None):
+    """Training pairs on a context that already holds receivers and shots
+    (EPRECOND 0 for the labels): for each (vs_true, vs_start, rho, qs) the
+    observed data from vs_true, the gradient at vs_start, the input maps and
+    the probe label.  Returns X [n][3][H8][W8], T [n][H8][W8]."""
+    n = len(models)
+    X = T = None
+    for k, (vt, v0, rho, qs) in enumerate(models):
+        ctx.set_model(vt, rho, qs)
+        obs = ctx.forward()
+        ctx.set_observed(obs)
+        ctx.set_model(v0, rho, qs)
+        ctx.gradient()
+        x = ctx.precond_features()
+        if X is None:
+            h8, w8 = x.shape[1], x.shape[2]
+            X = np.zeros((n, 3, h8, w8), np.float32)
+            T = np.zeros((n, h8, w8), np.float32)
+        X[k] = x
+        T[k] = label_from_diag(ctx.hessian_probe(nprobes=nprobes, eps=eps, seed=seed0 + k), X.shape[2], X.shape[3])
+        if log:
+            log("pair %d of %d" % (k + 1, n))
+    return X, T
+
+
 def smooth3(a):
     p = np.pad(a, 1, mode="edge")
     return sum(p[dj:dj + a.shape[0], di:di + a.shape[1]] for dj in range(3) for di in range(3)) / 9.0

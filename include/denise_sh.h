@@ -516,6 +516,12 @@ typedef struct dn_fwi_record {
     int test_forwards;    /* shot simulations spent in the line search             */
     int next_stage;       /* 1: the abort criterion ended the stage                */
     int precond;          /* direction taken: 0 eprecond (plain), 1 learned         */
+    /* learned mode: 1 when the plain direction was line-searched as well (the
+     * judge, or the learned line search found no decrease), and the relative
+     * misfit drops (L2t[1] - min(L2t[2], L2t[3])) / L2t[1] that the two line
+     * searches saw (0: no decrease found, -1: not run) */
+    int judged;
+    float drop_learned, drop_plain;
 } dn_fwi_record;
 
 /* Starts FWI on the current model, or a new stage (optimiser reset) when
