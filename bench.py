@@ -1,6 +1,6 @@
 """Benchmark of the SH adjoint-gradient hot path on one MI355X.
 
-Three workloads (viscoelastic SH, FDORDER 8 Holberg with MAXRELERROR 1, L = 2
+Four workloads (viscoelastic SH, FDORDER 8 Holberg with MAXRELERROR 1, L = 2
 relaxation mechanisms with Q = 20 and FL = 6.25 / 50 Hz, 20-point CPML on
 the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
 
@@ -14,7 +14,9 @@ the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
   three fields (115 GB in FP32), so HBM holds two shots at a time: one step
   is the gradient of one such batch (forward with snapshot storage,
   residuals, adjoint with correlation, energy preconditioning, chain rule),
-  the steps walk through the 16 batches of the 32 shots in turn.
+  the steps walk through the 16 batches of the 32 shots in turn.  Most
+  lenses lie deeper than the 0.6 s record illuminates, so the residuals
+  are small (the work per step does not depend on the data values).
 * ``c2``: NX = 1200, NY = 300, DH = 0.2 m, DT = 1e-4 s, NT = 10 000; vs
   100 -> 350 m/s with depth times a +-10 % checkerboard of 8 m cells (true)
   and the plain gradient (start); rho 1700 -> 2000; 40 shots, 25 Hz Ricker,
@@ -26,12 +28,17 @@ the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
   stage s mod 3), all mixed in the same batches; 390 receivers at 1 m
   spacing.  One step is the gradient over all 64 shots;
   --compare-sequential also times it with one shot per batch.
+* ``c5``: the learned preconditioner's held-out workload: NX = 4000,
+  NY = 800, DH = 0.2 m, DT = 1e-4 s, NT = 10 000; a model of the training
+  family (random_checkerboard of denise-sh_amd/precond_train.py, seed 9999;
+  training uses seeds 7000+); 64 shots, 25 Hz Ricker, 790 receivers at 1 m
+  spacing.  Meant for --fwi-learned with --train-config c2.
 
 Observed data are synthesised on the device from the true model before
 timing (synthetic, seeded).  The metric counts cell-updates of the forward
 and adjoint sweeps (NX*NY*NT*2 per shot).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3] [--compare-sequential]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3|c5] [--compare-sequential]
        [--nt NT] [--shots S] [--snap fp32|bf16|fp16] [--cpu-seconds T] [--no-cpu]
        [--rows R] [--max-batch B] [--autotune] [--exec graph|eager]
        python bench.py --config c2 --fwi N     (N plain L-BFGS iterations, wall time)
@@ -59,6 +66,20 @@ def _package():
     if name in sys.modules:
         return sys.modules[name]
     spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "denise-sh_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _precond_train():
+    """denise-sh_amd/precond_train.py (training pairs and the model family of
+    the learned preconditioner)."""
+    import importlib.util
+    name = "precond_train"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "denise-sh_amd", "precond_train.py"))
     mod = importlib.util.module_from_spec(spec)
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
@@ -118,6 +139,17 @@ def workload(config="c4", nt=None):
         nts = np.floor(rng.uniform(0.4, 1.2, nshots) / dt + 0.5).astype(np.int32)
         nt = int(nts.max()) if nt is None else nt
         nts = np.minimum(nts, nt)
+        damping = 350.0
+    elif config == "c5":
+        # held-out model of the learned preconditioner's training family
+        # (random_checkerboard, seeds 7000+ for training) on a 4000 x 800 grid
+        nx, ny, dh, dt = 4000, 800, 0.2, 1e-4
+        vs_true, vs_start, rho, qs = _precond_train().random_checkerboard(nx, ny, dh, 9999)
+        nshots, nrec, fc = 64, 790, 25.0
+        src_x = np.linspace(3.0, nx * dh - 3.0, nshots)
+        rec_x = 5.0 + np.arange(nrec) * 1.0
+        row = int(np.floor(0.5 / dh + 0.5))
+        nt = 10000 if nt is None else nt
         damping = 350.0
     elif config == "c4":
         nx, ny, dh, dt = 4000, 800, 0.1, 5e-5
@@ -249,7 +281,7 @@ def make_context(dn, W, args, nshots, prec, eprecond=1, shots=None):
     ctx.set_group_layout(args.layout)
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sel = np.arange(nshots) if shots is None else np.asarray(shots)
-    sig = signals_of(dn, W, nshots)[sel]
+    sig = signals_of(dn, W, len(W["src_i"]))[sel]
     ctx.set_shots(W["src_i"][sel], W["src_j"][sel], sig, W["nts"][sel] if "nts" in W else None)
     return ctx
 
@@ -302,11 +334,7 @@ def fwi_learned_run(dn, W, args, nshots, prec):
     --fwi-learned iterations of plain L-BFGS and of each learned mode from
     the same start (fresh contexts, eprecond 1, test shots 1, 11, 21, 31),
     and the wall time each needs to reach the plain run's final misfit."""
-    import importlib.util
-    spec = importlib.util.spec_from_file_location("precond_train", os.path.join(ROOT, "denise-sh_amd",
-                                                                                 "precond_train.py"))
-    pt = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(pt)
+    pt = _precond_train()
     log = lambda m: print("fwi-learned: " + m, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     # training pairs on the grid and acquisition of --train-config (default:
@@ -331,8 +359,17 @@ def fwi_learned_run(dn, W, args, nshots, prec):
     log("labels %.0f s, training %.0f s: loss %.4f -> %.4f, held-out %s" % (t_labels, t_train, hist[0], hist[-1],
                                                                              held))
     runs = {}
-    for name, mode in LEARNED_MODES.items():
-        ctx = make_context(dn, W, args, nshots, prec, eprecond=1)
+    names = [m for m in args.modes.split(",") if m]
+    for name in names:
+        if name not in LEARNED_MODES:
+            raise SystemExit("bench.py: unknown --modes entry %r (choose from %s)" % (name, ", ".join(LEARNED_MODES)))
+    if "plain" not in names or "learned_fixed" not in names:
+        raise SystemExit("bench.py: --modes must include plain and learned_fixed")
+    # with --shots below the acquisition's count, shots spread evenly over it
+    fsel = np.linspace(0, len(W["src_i"]) - 1, nshots).round().astype(int)
+    for name in names:
+        mode = LEARNED_MODES[name]
+        ctx = make_context(dn, W, args, nshots, prec, eprecond=1, shots=fsel)
         ctx.set_model(W["vs_true"], W["rho"], W["qs"])
         obs = ctx.forward()
         ctx.set_model(W["vs_start"], W["rho"], W["qs"])
@@ -377,7 +414,7 @@ def fwi_learned_run(dn, W, args, nshots, prec):
         "modes": {k: ("plain L-BFGS (eprecond 1)" if v is None else
                       "learned, judge_every %d, diagonal %s" % (v[0], "fixed per stage" if v[1]
                                                                 else "recomputed every iteration"))
-                  for k, v in LEARNED_MODES.items()},
+                  for k, v in LEARNED_MODES.items() if k in names},
         "runs": runs,
         "config": {"workload": "visco SH FWI %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d"
                                % (W["nx"], W["ny"], nshots, W["nt"], W["dtinv"]), "n_gpus": 1}}))
@@ -438,7 +475,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3"])
+    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5"])
     ap.add_argument("--compare-sequential", action="store_true",
                     help="also time the same gradient with one shot per batch (the per-shot sequential loop)")
     ap.add_argument("--nt", type=int, default=0, help="record length in steps (0: the workload's)")
@@ -465,7 +502,9 @@ def main():
     ap.add_argument("--probes", type=int, default=4, help="--fwi-learned: Hessian probes per label")
     ap.add_argument("--label-shots", type=int, default=20, help="--fwi-learned: shots of each label's gradients")
     ap.add_argument("--train-steps", type=int, default=300, help="--fwi-learned: Adam steps")
-    ap.add_argument("--train-config", default=None, choices=["c4", "c2", "c3"],
+    ap.add_argument("--modes", default=",".join(LEARNED_MODES),
+                    help="--fwi-learned: comma-separated runs (plain and learned_fixed required)")
+    ap.add_argument("--train-config", default=None, choices=["c4", "c2", "c3", "c5"],
                     help="--fwi-learned: workload whose grid and shots the training pairs use (default: --config)")
     ap.add_argument("--unet", type=int, default=0,
                     help="instead of the gradient metric: time this many U-Net training steps")

@@ -318,7 +318,7 @@ class Context:
         self._chk(lib().dn_fwi_get_model(self._h, _f(vs), _f(rho)))
         return vs, rho
 
-    def set_learned(self, net, judge_every=5, fixed=False):
+    def set_learned(self, net, judge_every=0, fixed=True):
         lib().dn_fwi_set_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         self._chk(lib().dn_fwi_set_learned(self._h, net._h if net is not None else None, judge_every))
         lib().dn_fwi_set_learned_fixed.argtypes = [C.c_void_p, C.c_int]

@@ -93,7 +93,7 @@ struct dn_ctx {
     // learned preconditioner (owned by the caller) and the judge interval
     dn_unet *unet = nullptr;
     int judge_every = 5;
-    bool learned_fixed = false;      // D from the first iteration of a stage, kept for the stage
+    bool learned_fixed = true;       // D from the first iteration of a stage, kept for the stage
     // gradient smoothing / taper workspace (dn_gradproc.hip)
     dn_gradproc_state *gradproc = nullptr;
     // data conditioning of the residual (dn_set_datacond): parameters and the

@@ -35,7 +35,6 @@ const LSpec kLayers[] = {
     {80, 32, 80, 1}, {32, 32, 32, 1}, {48, 16, 48, 1}, {16, 16, 16, 1}, {16, 16, 16, 0},
 };
 constexpr int kNL = sizeof(kLayers) / sizeof(kLayers[0]);
-constexpr int kChunk = 4096;
 
 template <typename T> hipError_t dal(T **p, size_t n) { return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)); }
 template <typename T> void dfr(T *&p)
@@ -81,12 +80,15 @@ static void free_buffers(dn_unet *u)
     u->bn = u->bH = u->bW = 0;
 }
 
-// pixels of the level holding layer l's input and output
-static long long level_pixels(int l, long long P0)
+// resolution level of layer l's input and output (1/2^level of the input)
+static int level_of(int l)
 {
     static const int lev[kNL] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
-    return P0 >> (2 * lev[l]);
+    return lev[l];
 }
+
+// pixels of the level holding layer l's input and output
+static long long level_pixels(int l, long long P0) { return P0 >> (2 * level_of(l)); }
 
 static int ensure_buffers(dn_unet *u, int n, int H, int W)
 {
@@ -115,14 +117,14 @@ static int ensure_buffers(dn_unet *u, int n, int H, int W)
     UCHK(dal(&u->xin, P0 * 3));
     UCHK(dal(&u->tgt, P0));
     UCHK(dal(&u->yout, P0));
-    size_t plen = 0;
+    size_t plen = 0, blen = 0;
     for (int l = 0; l < kNL; ++l) {
-        const long long P = level_pixels(l, P0);
-        const size_t nch = (size_t)((P + kChunk - 1) / kChunk);
+        const size_t nch = (size_t)wgrad_chunks(n, H >> level_of(l));
         plen = std::max(plen, nch * 9 * kLayers[l].co * kLayers[l].ci);
+        blen = std::max(blen, nch * kLayers[l].co);
     }
     UCHK(dal(&u->part, plen));
-    UCHK(dal(&u->bpart, (size_t)((P0 + kChunk - 1) / kChunk) * 64));
+    UCHK(dal(&u->bpart, blen));
     UCHK(dal(&u->lpart, 256));
     u->part_len = plen;
     u->bn = n;
@@ -185,7 +187,7 @@ static int wgrad(dn_unet *u, int l, int n, int H, int W, const uint16_t *x, cons
     w.Co = kLayers[l].co;
     w.x = x;
     w.dy = dy;
-    w.chunk = kChunk;
+    w.chunk_rows = wgrad_chunk_rows(n, H);
     w.part = u->part;
     w.bpart = u->bpart;
     UCHK(launch_wgrad(w, u->g + u->woff[l], u->g + u->boff[l], u->stream));

@@ -20,14 +20,15 @@ struct ConvArgs {
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t st);
 
 struct WgradArgs {
-    int n, H, W, Ci, Co;
+    int n, H, W, Ci, Co;       // Ci <= 112, Co <= 64, multiples of 16
     const uint16_t *x;         // layer input [n][H][W][Ci]
     const uint16_t *dy;        // output gradient [n][H][W][Co]
-    int chunk;                 // pixels per partial (multiple of 16)
+    int chunk_rows;            // image rows per partial: wgrad_chunk_rows(n, H)
     float *part;               // [nchunks][9][Co][Ci]
     float *bpart;              // [nchunks][Co]
 };
-int wgrad_chunks(const WgradArgs &a);
+int wgrad_chunk_rows(int n, int H);
+int wgrad_chunks(int n, int H);   // partials of a layer of n images of H rows
 // dw [9][Co][Ci] and (if db) db [Co], fp32, summed in a fixed order
 hipError_t launch_wgrad(const WgradArgs &a, float *dw, float *db, hipStream_t st);
 

@@ -1,6 +1,6 @@
 // Kernels of the learned diagonal-Hessian preconditioner: a 4-level U-Net of
-// 3x3 convolutions in BF16 with FP32 accumulation on the CDNA matrix cores
-// (v_mfma_f32_16x16x16_bf16), stored channels-last ([n][H][W][C], C a
+// 3x3 convolutions in BF16 with FP32 accumulation on the CDNA4 matrix cores
+// (v_mfma_f32_16x16x32_bf16), stored channels-last ([n][H][W][C], C a
 // multiple of 16), plus the pooling, upsampling, loss and Adam kernels of
 // its training.  Host side: dn_unet.cpp.
 #include "dn_unet.h"
@@ -9,7 +9,6 @@
 
 namespace dn {
 
-typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t u_f2bf(float x)
@@ -21,151 +20,306 @@ __device__ __forceinline__ uint16_t u_f2bf(float x)
 }
 __device__ __forceinline__ float u_bf2f(uint16_t h) { return __uint_as_float(((unsigned int)h) << 16); }
 
+// 8 BF16 values: one 16-byte MFMA operand per lane
+typedef unsigned short u8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4v mfma32(u8v a, u8v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0,
+                                                   0);
+}
+
 // ---------------------------------------------------------------------------
-// 3x3 "same" convolution as an implicit GEMM on MFMA 16x16x16 (bf16 in,
-// fp32 accumulate).  D[co][pixel] = sum_{ky,kx,ci} W[ky][kx][co][ci] *
-// X[pixel + (ky-1, kx-1)][ci].  Per MFMA: A = 16 output channels x 16 input
-// channels of one tap, B = 16 input channels x 16 pixels of one row.
-// Operand layout (16x16x16): lane l supplies A[l%16][4(l/16)+j] and
-// B[4(l/16)+j][l%16], j = 0..3, and receives D[4(l/16)+j][l%16].
-// Workgroup: 4 waves, 64 consecutive pixels of one image row and one group
-// of up to 64 output channels (blockIdx.z); each wave keeps the group's
-// (<= 4) accumulator tiles of its 16 pixels.
+// 3x3 "same" convolution as an implicit GEMM on MFMA 16x16x32 (bf16 in, fp32
+// accumulate): D[co][px] = sum over k = (tap, ci) of W[tap][co][ci] *
+// X[px + shift(tap)][ci].  K is walked in groups of 8 input channels of one
+// tap; one MFMA takes 4 groups: lane quarter kq = lane / 16 supplies group
+// 4i + kq, A = W[tap][co = lane % 16][8 channels] and B = X[pixel lane % 16
+// + shift][8 channels] (16 bytes each), and receives D[4kq + r][lane % 16],
+// r = 0..3.  Groups past the last tap are zero.
+// Workgroup: 4 waves over 256 consecutive pixels of one image row (64 per
+// wave, 4 pixel tiles) and a group of up to 64 output channels (blockIdx.z,
+// up to 4 channel tiles): 16 accumulator tiles per wave.  The weights of one
+// kernel row (3 taps x 64 channels x CI) are staged in LDS once and read by
+// all four waves; the input comes with one 16-byte load per lane and
+// operand, the three taps of a row re-reading a pixel through L1.
 // ---------------------------------------------------------------------------
+template <int CI>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 {
+    constexpr int CB = CI / 8;          // 8-channel groups per tap
+    constexpr int NG = 3 * CB;          // groups of one kernel row
+    constexpr int NKI = (NG + 3) / 4;   // MFMAs per kernel row
+    constexpr int SW = 65;              // LDS entries per group (64 channels + 1 against bank conflicts)
+    __shared__ u8v sw[NKI * 4 * SW];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nl = lane & 15, kq = lane >> 4;
     const int row = blockIdx.y;
     const int img = row / a.H, y = row - img * a.H;
-    const int xb = blockIdx.x * 64 + wave * 16;
-    const int nl = lane & 15, kq = lane >> 4;
-    const int cg = blockIdx.z * 64;                 // first output channel of the group
-    const int nmb = min(4, (a.Co - cg) >> 4);
-    f4v acc[4];
+    const int x0 = blockIdx.x * 256 + wave * 64;
+    const int cg = blockIdx.z * 64;
+    const int nmt = min(4, (a.Co - cg) >> 4);
+    f4v acc[4][4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) acc[m] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
     for (int ky = 0; ky < 3; ++ky) {
         const int yy = y + ky - 1;
         if (yy < 0 || yy >= a.H) continue;   // uniform in the workgroup
-        const uint16_t *xrow = a.x + (long long)(img * a.H + yy) * a.W * a.Ci;
-        for (int kx = 0; kx < 3; ++kx) {
-            const int xx = xb + nl + kx - 1;
-            const bool inb = (xx >= 0) && (xx < a.W);
-            const uint16_t *wp = a.w + (long long)(ky * 3 + kx) * a.Co * a.Ci;
-            for (int c0 = 0; c0 < a.Ci; c0 += 16) {
-                s4v bv = s4v{0, 0, 0, 0};
-                if (inb) bv = *(const s4v *)(xrow + (long long)xx * a.Ci + c0 + 4 * kq);
-#pragma unroll
-                for (int m = 0; m < 4; ++m)
-                    if (m < nmb) {
-                        const s4v av = *(const s4v *)(wp + (long long)(cg + m * 16 + nl) * a.Ci + c0 + 4 * kq);
-                        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[m], 0, 0, 0);
-                    }
+        __syncthreads();                     // the previous kernel row's weights are consumed
+        for (int e = threadIdx.x; e < NKI * 4 * 64; e += 256) {
+            const int g = e >> 6, co = e & 63;
+            u8v v = u8v{0, 0, 0, 0, 0, 0, 0, 0};
+            if (g < NG && cg + co < a.Co) {
+                const int kx = g / CB, cb = g - kx * CB;
+                v = *(const u8v *)(a.w + ((long long)(3 * ky + kx) * a.Co + cg + co) * CI + 8 * cb);
             }
+            sw[g * SW + co] = v;
+        }
+        __syncthreads();
+        const uint16_t *xrow = a.x + (long long)(img * a.H + yy) * a.W * CI;
+#pragma unroll
+        for (int i = 0; i < NKI; ++i) {
+            const int g = 4 * i + kq;
+            const bool live = g < NG;
+            const int kx = live ? g / CB : 0, cb = live ? g - (g / CB) * CB : 0;
+            u8v b[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int xx = x0 + 16 * t + nl + kx - 1;
+                b[t] = u8v{0, 0, 0, 0, 0, 0, 0, 0};
+                if (live && xx >= 0 && xx < a.W) b[t] = *(const u8v *)(xrow + (long long)xx * CI + 8 * cb);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (m < nmt) {
+                    const u8v av = sw[g * SW + 16 * m + nl];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) acc[m][t] = mfma32(av, b[t], acc[m][t]);
+                }
         }
     }
-    const int xo = xb + nl;
-    if (xo >= a.W) return;
-    const long long pix = (long long)(img * a.H + y) * a.W + xo;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-        if (m < nmb)
+    for (int t = 0; t < 4; ++t) {
+        const int xo = x0 + 16 * t + nl;
+        if (xo >= a.W) continue;
+        const long long pix = (long long)(img * a.H + y) * a.W + xo;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int co = cg + m * 16 + 4 * kq + j;
-                float v = acc[m][j];
-                if (a.bias) v += a.bias[co];
-                if (a.relu) v = fmaxf(v, 0.0f);
-                if (a.mask && !(u_bf2f(a.mask[pix * a.Co + co]) > 0.0f)) v = 0.0f;
-                if (a.y16) a.y16[pix * a.Co + co] = u_f2bf(v);
-                if (a.y32) a.y32[pix * a.Co + co] = v;
+        for (int m = 0; m < 4; ++m)
+            if (m < nmt) {
+                const int co0 = cg + m * 16 + 4 * kq;
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    v[r] = acc[m][t][r];
+                    if (a.bias) v[r] += a.bias[co0 + r];
+                    if (a.relu) v[r] = fmaxf(v[r], 0.0f);
+                }
+                if (a.mask) {
+                    const ushort4 mk = *(const ushort4 *)(a.mask + pix * a.Co + co0);
+                    if (!(u_bf2f(mk.x) > 0.0f)) v[0] = 0.0f;
+                    if (!(u_bf2f(mk.y) > 0.0f)) v[1] = 0.0f;
+                    if (!(u_bf2f(mk.z) > 0.0f)) v[2] = 0.0f;
+                    if (!(u_bf2f(mk.w) > 0.0f)) v[3] = 0.0f;
+                }
+                if (a.y16)
+                    *(ushort4 *)(a.y16 + pix * a.Co + co0) =
+                        make_ushort4(u_f2bf(v[0]), u_f2bf(v[1]), u_f2bf(v[2]), u_f2bf(v[3]));
+                if (a.y32) *(float4 *)(a.y32 + pix * a.Co + co0) = make_float4(v[0], v[1], v[2], v[3]);
             }
+    }
 }
 
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t st)
 {
-    if (a.Ci % 16 || a.Co % 16 || a.Ci < 16 || a.Co < 16) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((a.W + 63) / 64), (unsigned)(a.n * a.H), (unsigned)((a.Co + 63) / 64));
-    hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, st, a);
+    if (a.Co % 16 || a.Co < 16 || a.n < 1 || a.H < 1 || a.W < 1) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.W + 255) / 256), (unsigned)(a.n * a.H), (unsigned)((a.Co + 63) / 64));
+    switch (a.Ci) {
+    case 16: hipLaunchKernelGGL(k_conv3x3<16>, grid, dim3(256), 0, st, a); break;
+    case 32: hipLaunchKernelGGL(k_conv3x3<32>, grid, dim3(256), 0, st, a); break;
+    case 48: hipLaunchKernelGGL(k_conv3x3<48>, grid, dim3(256), 0, st, a); break;
+    case 64: hipLaunchKernelGGL(k_conv3x3<64>, grid, dim3(256), 0, st, a); break;
+    case 80: hipLaunchKernelGGL(k_conv3x3<80>, grid, dim3(256), 0, st, a); break;
+    case 112: hipLaunchKernelGGL(k_conv3x3<112>, grid, dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[ky][kx][co][ci] = sum_p dY[p][co] * X[p + (ky-1,kx-1)][ci]
-// as a GEMM over pixels (K).  Stage 1: one wave per (pixel chunk, tap,
-// 16x16 channel tile) writes a partial; stage 2 adds the chunks in order
-// (deterministic).  Also the bias gradient (sum over pixels of dY).
+// Weight gradient dW[tap][co][ci] = sum_p dY[p][co] * X[p + shift(tap)][ci]
+// as a GEMM over pixels (K) on MFMA 16x16x32, with the bias gradient
+// db[co] = sum_p dY[p][co].  Workgroup (4 waves): one chunk of image rows
+// (blockIdx.x) and one kernel row ky (blockIdx.y), walked in segments of 64
+// pixels of a row.  Per segment the output gradient dY[64 px][CO] and, for
+// each kx, the shifted input row X[y + ky - 1][x + kx - 1][CI] are staged in
+// LDS transposed ([channel][pixel], zero outside the image; one 16-byte load
+// of 8 channels per thread and item), so that a lane reads 8 consecutive
+// pixels of one channel as one 16-byte operand: A = dY^T (co = lane % 16),
+// B = X_shift (ci = lane % 16).  The 3 x (CO/16) x (CI/16) tiles of the
+// kernel row are dealt round-robin to the waves (at most 16 each) and
+// accumulated over the chunk; each chunk writes a partial, added over the
+// chunks in chunk order (deterministic).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_wgrad(WgradArgs a)
+constexpr int kSegPx = 64;     // pixels per segment
+constexpr int kSegPad = 72;    // LDS pixels per channel row (16-byte aligned rows, staggered banks)
+
+template <int CI, int CO>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a)
 {
-    const int lane = threadIdx.x, nl = lane & 15, kq = lane >> 4;
-    const int chunk = blockIdx.x, tap = blockIdx.y;
-    const int ky = tap / 3, kx = tap - 3 * (tap / 3);
-    const int ncb = a.Ci >> 4;
-    const int cob = blockIdx.z / ncb, cib = blockIdx.z - cob * ncb;
-    const long long P = (long long)a.n * a.H * a.W;
-    const long long p0 = (long long)chunk * a.chunk;
-    const long long p1 = min(P, p0 + a.chunk);
-    const long long shift = (long long)(ky - 1) * a.W + (kx - 1);
-    f4v acc = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-    for (long long pb = p0; pb < p1; pb += 16) {
-        s4v av, bv;
+    constexpr int NMT = CO / 16, NNT = CI / 16, NTILES = 3 * NMT * NNT;
+    static_assert(NTILES <= 64, "at most 16 tiles per wave");
+    __shared__ __attribute__((aligned(16))) uint16_t sdy[CO * kSegPad];
+    __shared__ __attribute__((aligned(16))) uint16_t sx[3 * CI * kSegPad];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nl = lane & 15, kq = lane >> 4;
+    const int ky = blockIdx.y;
+    const int rows_total = a.n * a.H;
+    const int r0 = blockIdx.x * a.chunk_rows, r1 = min(rows_total, r0 + a.chunk_rows);
+    f4v acc[16];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long p = pb + 4 * kq + j;
-            uint16_t dyv = 0, xv = 0;
-            if (p < p1) {
-                dyv = a.dy[p * a.Co + cob * 16 + nl];
-                const int xp = (int)(p % a.W);
-                const int yp = (int)((p / a.W) % a.H);
-                const int yy = yp + ky - 1, xx = xp + kx - 1;
-                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) xv = a.x[(p + shift) * a.Ci + cib * 16 + nl];
+    for (int q = 0; q < 16; ++q) acc[q] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (int r = r0; r < r1; ++r) {
+        const int img = r / a.H, y = r - img * a.H;
+        const int yy = y + ky - 1;
+        const bool yin = yy >= 0 && yy < a.H;
+        for (int x0 = 0; x0 < a.W; x0 += kSegPx) {
+            __syncthreads();   // the previous segment is consumed
+            // dY[y][x0 + j][8c .. 8c+7] -> sdy[8c + k][j]
+            for (int e = threadIdx.x; e < kSegPx * (CO / 8); e += 256) {
+                const int j = e / (CO / 8), c8 = e - j * (CO / 8);
+                const int x = x0 + j;
+                u8v v = u8v{0, 0, 0, 0, 0, 0, 0, 0};
+                if (x < a.W) v = *(const u8v *)(a.dy + ((long long)r * a.W + x) * CO + 8 * c8);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) sdy[(8 * c8 + k) * kSegPad + j] = v[k];
             }
-            av[j] = (short)dyv;
-            bv[j] = (short)xv;
+            // X[yy][x0 + jj - 1][8c .. 8c+7] -> sx[kx][8c + k][jj - kx]
+            for (int e = threadIdx.x; e < (kSegPx + 2) * (CI / 8); e += 256) {
+                const int jj = e / (CI / 8), c8 = e - jj * (CI / 8);
+                const int x = x0 + jj - 1;
+                u8v v = u8v{0, 0, 0, 0, 0, 0, 0, 0};
+                if (yin && x >= 0 && x < a.W) v = *(const u8v *)(a.x + ((long long)(img * a.H + yy) * a.W + x) * CI + 8 * c8);
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) {
+                    const int j = jj - kx;
+                    if (j >= 0 && j < kSegPx)
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) sx[(kx * CI + 8 * c8 + k) * kSegPad + j] = v[k];
+                }
+            }
+            __syncthreads();
+            if (ky == 0 && (int)threadIdx.x < CO)
+#pragma unroll 8
+                for (int j = 0; j < kSegPx; ++j) bsum += u_bf2f(sdy[threadIdx.x * kSegPad + j]);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int tile = wave + 4 * q;
+                if (tile < NTILES) {
+                    const int kx = tile / (NMT * NNT), rem = tile - kx * (NMT * NNT);
+                    const int mt = rem / NNT, nt = rem - mt * NNT;
+                    const uint16_t *pa = sdy + (mt * 16 + nl) * kSegPad + 8 * kq;
+                    const uint16_t *pb = sx + (kx * CI + nt * 16 + nl) * kSegPad + 8 * kq;
+#pragma unroll
+                    for (int s = 0; s < kSegPx; s += 32)
+                        acc[q] = mfma32(*(const u8v *)(pa + s), *(const u8v *)(pb + s), acc[q]);
+                }
+            }
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int co = cob * 16 + 4 * kq + j, ci = cib * 16 + nl;
-        a.part[(((long long)chunk * 9 + tap) * a.Co + co) * a.Ci + ci] = acc[j];
+    for (int q = 0; q < 16; ++q) {
+        const int tile = wave + 4 * q;
+        if (tile < NTILES) {
+            const int kx = tile / (NMT * NNT), rem = tile - kx * (NMT * NNT);
+            const int mt = rem / NNT, nt = rem - mt * NNT;
+            const int tap = 3 * ky + kx;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int co = mt * 16 + 4 * kq + j, ci = nt * 16 + nl;
+                a.part[(((long long)blockIdx.x * 9 + tap) * CO + co) * CI + ci] = acc[q][j];
+            }
+        }
     }
+    if (ky == 0 && (int)threadIdx.x < CO) a.bpart[(long long)blockIdx.x * CO + threadIdx.x] = bsum;
 }
 
-__global__ void k_bgrad(WgradArgs a)
-{
-    const int co = threadIdx.x;
-    if (co >= a.Co) return;
-    const long long P = (long long)a.n * a.H * a.W;
-    const long long p0 = (long long)blockIdx.x * a.chunk;
-    const long long p1 = min(P, p0 + a.chunk);
-    float s = 0.0f;
-    for (long long p = p0; p < p1; ++p) s += u_bf2f(a.dy[p * a.Co + co]);
-    a.bpart[(long long)blockIdx.x * a.Co + co] = s;
-}
-
+// out[e] = sum over c of part[c][e], c in increasing order: each thread
+// walks the chunks of one element (adjacent threads, adjacent elements)
 __global__ void k_reduce_chunks(int nchunks, long long len, const float *part, float *out)
 {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
          e += (long long)gridDim.x * blockDim.x) {
         float s = 0.0f;
-        for (int c = 0; c < nchunks; ++c) s += part[(long long)c * len + e];
+        int c = 0;
+        for (; c + 4 <= nchunks; c += 4) {
+            const float p0 = part[(long long)c * len + e], p1 = part[(long long)(c + 1) * len + e];
+            const float p2 = part[(long long)(c + 2) * len + e], p3 = part[(long long)(c + 3) * len + e];
+            s += p0;
+            s += p1;
+            s += p2;
+            s += p3;
+        }
+        for (; c < nchunks; ++c) s += part[(long long)c * len + e];
         out[e] = s;
     }
 }
 
-int wgrad_chunks(const WgradArgs &a)
+// The same sum for a short vector (the bias gradient): one workgroup per
+// element; thread t adds chunks t, t + 256, ... in order, then a fixed
+// pairwise tree over the threads
+__global__ __launch_bounds__(256) void k_reduce_chunks_short(int nchunks, int len, const float *part, float *out)
 {
-    const long long P = (long long)a.n * a.H * a.W;
-    return (int)((P + a.chunk - 1) / a.chunk);
+    __shared__ float sh[256];
+    const int e = blockIdx.x;
+    float s = 0.0f;
+    for (int c = threadIdx.x; c < nchunks; c += 256) s += part[(long long)c * len + e];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[e] = sh[0];
+}
+
+int wgrad_chunk_rows(int n, int H)
+{
+    return std::max(1, (n * H + 255) / 256);
+}
+
+int wgrad_chunks(int n, int H)
+{
+    const int cr = wgrad_chunk_rows(n, H);
+    return (n * H + cr - 1) / cr;
+}
+
+template <int CI, int CO> static void wgrad_launch(const WgradArgs &a, int nch, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_wgrad<CI, CO>), dim3(nch, 3), dim3(256), 0, st, a);
 }
 
 hipError_t launch_wgrad(const WgradArgs &a, float *dw, float *db, hipStream_t st)
 {
-    if (a.Ci % 16 || a.Co % 16 || a.Co > 64 || a.chunk % 16) return hipErrorInvalidValue;
-    const int nch = wgrad_chunks(a);
-    hipLaunchKernelGGL(k_wgrad, dim3(nch, 9, (a.Co / 16) * (a.Ci / 16)), dim3(64), 0, st, a);
+    if (a.chunk_rows != wgrad_chunk_rows(a.n, a.H) || a.n < 1 || a.H < 1 || a.W < 1) return hipErrorInvalidValue;
+    const int nch = wgrad_chunks(a.n, a.H);
+    const int key = a.Ci * 1000 + a.Co;
+    switch (key) {   // the (input, output) channel pairs of the network's layers
+    case 16016: wgrad_launch<16, 16>(a, nch, st); break;
+    case 16032: wgrad_launch<16, 32>(a, nch, st); break;
+    case 32032: wgrad_launch<32, 32>(a, nch, st); break;
+    case 32048: wgrad_launch<32, 48>(a, nch, st); break;
+    case 48048: wgrad_launch<48, 48>(a, nch, st); break;
+    case 48064: wgrad_launch<48, 64>(a, nch, st); break;
+    case 64064: wgrad_launch<64, 64>(a, nch, st); break;
+    case 112048: wgrad_launch<112, 48>(a, nch, st); break;
+    case 80032: wgrad_launch<80, 32>(a, nch, st); break;
+    case 48016: wgrad_launch<48, 16>(a, nch, st); break;
+    default: return hipErrorInvalidValue;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long long len = 9LL * a.Co * a.Ci;
@@ -173,10 +327,7 @@ hipError_t launch_wgrad(const WgradArgs &a, float *dw, float *db, hipStream_t st
                        nch, len, a.part, dw);
     e = hipGetLastError();
     if (e != hipSuccess || !db) return e;
-    hipLaunchKernelGGL(k_bgrad, dim3(nch), dim3(64), 0, st, a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(64), 0, st, nch, (long long)a.Co, a.bpart, db);
+    hipLaunchKernelGGL(k_reduce_chunks_short, dim3(a.Co), dim3(256), 0, st, nch, a.Co, a.bpart, db);
     return hipGetLastError();
 }
 

@@ -586,9 +586,9 @@ int dn_fwi_load_state(dn_ctx *ctx, const char *path, const char *jacobian, int n
  * first) and the plain L-BFGS direction, with the same history, after.
  * The network stays owned by the caller.  Not in the reference. */
 int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
-/* fixed 1: the learned diagonal D is predicted on the first iteration of a
- * stage and kept for the whole stage (a fixed initial Hessian, so the L-BFGS
- * pairs and the scaling stay consistent); 0 (default): predicted again at
+/* fixed 1 (default): the learned diagonal D is predicted on the first
+ * iteration of a stage and kept for the whole stage (a fixed initial Hessian,
+ * so the L-BFGS pairs and the scaling stay consistent); 0: predicted again at
  * every iteration from the current energy maps and model.  Not in the
  * reference. */
 int dn_fwi_set_learned_fixed(dn_ctx *ctx, int fixed);

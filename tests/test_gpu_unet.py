@@ -82,3 +82,37 @@ def test_unet_training_reduces_loss():
     final, _ = net.loss_grad(x, t)
     say("  loss %.4e -> %.4e" % (first, final))
     assert final < 0.5 * first
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 328), (2, 200, 72)], ids=["w328", "h200x2"])
+def test_unet_tiles_and_chunks_match_oracle(shape):
+    """Shapes that exercise the launch tiling of the kernels: rows wider than
+    one 256-pixel convolution workgroup and not a multiple of 64 (w328), and
+    more than 384 image rows, so that the weight gradient adds partials of
+    several rows each (h200x2)."""
+    dn = product()
+    net = dn.UNet(seed=11)
+    p = net.get_params()
+    wo, bo, n = offsets()
+    rng = np.random.default_rng(12)
+    for l, (ci, co, _) in enumerate(LAYERS):
+        p[bo[l]:bo[l] + co] = (rng.standard_normal(co) * 0.05).astype(np.float32)
+    net.set_params(p)
+    x, t = _inputs(13, *shape)
+    say("unet %dx3x%dx%d forward and loss gradient" % shape)
+    y = net.forward(x)
+    ref = UNetOracle(p, bf16=True)
+    e = rel_l2(y, ref.forward(x))
+    loss, g = net.loss_grad(x, t)
+    lr, gr = ref.loss_grad(x, t)
+    say("  forward rel L2 %.3e; loss %.6e oracle %.6e" % (e, loss, lr))
+    assert e < 1e-2
+    assert abs(loss - lr) <= 1e-2 * abs(lr)
+    worst = 0.0
+    for l, (ci, co, _) in enumerate(LAYERS):
+        sl = slice(wo[l], bo[l] + co)
+        if np.max(np.abs(gr[sl])) == 0:
+            continue
+        worst = max(worst, rel_l2(g[sl], gr[sl]))
+    say("  worst layer gradient rel L2 %.3e" % worst)
+    assert worst < 5e-2
