@@ -489,8 +489,9 @@ def main():
                     help="shots per wavefront on the plain steps (--autotune chooses)")
     ap.add_argument("--rows", type=int, default=0, help="rows per segment of the step kernel (0: automatic)")
     ap.add_argument("--max-batch", type=int, default=0, help="most shots advanced together (0: from free HBM)")
-    ap.add_argument("--layout", type=int, default=0, choices=[0, 1],
-                    help="workgroups of adjacent strips (0) or of the shots of one strip (1)")
+    ap.add_argument("--layout", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="workgroups of 8 adjacent strips of one shot (0), of 8/S strips x S shots (S = 2, 4, 8; "
+                         "1 = 8)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     ap.add_argument("--fwi", type=int, default=0,
@@ -616,7 +617,9 @@ def main():
         out["config"]["max_batch"] = args.max_batch
     out["config"]["exec"] = args.exec
     out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
-    out["config"]["group_layout"] = "tuned" if args.autotune else ("shots" if args.layout else "strips")
+    out["config"]["group_layout"] = "tuned" if args.autotune else (
+        "8 strips x 1 shot" if args.layout == 0 else "%d strips x %d shots" % (8 // (8 if args.layout == 1 else args.layout),
+                                                                          8 if args.layout == 1 else args.layout))
     if args.compare_sequential:
         # the same gradient with one shot per batch: the per-shot sequential
         # GPU loop the batched launch replaces (the reference's shot loop,

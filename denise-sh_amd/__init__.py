@@ -340,7 +340,9 @@ class Context:
         self._chk(lib().dn_set_wave_shots(self._h, n))
 
     def set_group_layout(self, mode):
-        """0: workgroups of eight adjacent strips (default); 1: of one strip of eight shots."""
+        """Workgroup layout of the step kernel: 0 eight adjacent strips of one
+        shot (default); 2, 4: 8/S adjacent strips of S shots; 1 or 8: one
+        strip of eight shots."""
         lib().dn_set_group_layout.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_set_group_layout(self._h, mode))
 

@@ -543,8 +543,10 @@ extern "C" int dn_set_wave_shots(dn_ctx *c, int n)
 extern "C" int dn_set_group_layout(dn_ctx *c, int mode)
 {
     if (!c) return -1;
-    if (mode != 0 && mode != 1) return dn_fail(c, "dn_set_group_layout: 0 (adjacent strips) or 1 (shots of a strip)");
-    c->shot_groups = (mode == 1);
+    if (mode != 0 && mode != 1 && mode != 2 && mode != 4 && mode != 8)
+        return dn_fail(c, "dn_set_group_layout: 0 (adjacent strips of one shot), 2 or 4 (shots x strips), 1 or 8 "
+                          "(one strip of eight shots)");
+    c->sg = (mode == 0) ? 1 : (mode == 1 ? 8 : mode);
     dn_drop_graphs(c);
     return 0;
 }
@@ -609,34 +611,32 @@ extern "C" int dn_autotune(dn_ctx *c, int nshots, int nsteps, int *rows_out, int
         else
             c->two_shots = saved_two;
     }
-    // workgroups of adjacent strips or of the shots of one strip (its own
-    // rows per segment, as the wave count differs)
+    // the other workgroup layouts (shots x strips per workgroup, each with
+    // its own rows per segment, as the workgroup count differs)
     {
-        const bool saved_sg = c->shot_groups;
-        c->shot_groups = !saved_sg;
-        double ms_other = 0.0;
-        int rows_other = best_rows;
-        for (int R : {best_rows / 2, best_rows, best_rows * 2}) {
-            const int Rr = std::min(R, c->g.ny);
-            if (Rr < std::min(2 * c->K, c->g.ny)) continue;
-            double ms = 0.0;
-            if (time_it(Rr, 0, &ms)) {
-                c->shot_groups = saved_sg;
-                c->rows_req = saved_rows;
-                c->graph_steps = saved_graph;
-                return -1;
-            }
-            if (ms_other == 0.0 || ms < ms_other) {
-                ms_other = ms;
-                rows_other = Rr;
+        const int saved_sg = c->sg;
+        int best_sg = saved_sg;
+        for (int sg : {1, 2, 4, 8}) {
+            if (sg == saved_sg || sg > std::max(1, n)) continue;
+            c->sg = sg;
+            for (int R : {best_rows / 2, best_rows, best_rows * 2}) {
+                const int Rr = std::min(R, c->g.ny);
+                if (Rr < std::min(2 * c->K, c->g.ny)) continue;
+                double ms = 0.0;
+                if (time_it(Rr, 0, &ms)) {
+                    c->sg = saved_sg;
+                    c->rows_req = saved_rows;
+                    c->graph_steps = saved_graph;
+                    return -1;
+                }
+                if (ms < best) {
+                    best = ms;
+                    best_rows = Rr;
+                    best_sg = sg;
+                }
             }
         }
-        if (ms_other > 0.0 && ms_other < best) {
-            best = ms_other;
-            best_rows = rows_other;
-        } else {
-            c->shot_groups = saved_sg;
-        }
+        c->sg = best_sg;
     }
     double ms_graph = 0.0;
     if (time_it(best_rows, -1, &ms_graph)) {
@@ -785,14 +785,15 @@ void dn_fill_step_common(dn_ctx *c, StepArgs &a, int n)
     a.ow = kWave + 1 - 2 * c->K;
     a.nstrips = (g.nx + a.ow - 1) / a.ow;
     a.ngroups = (a.nstrips + kStripsPerGroup - 1) / kStripsPerGroup;
-    a.shot_groups = c->shot_groups ? 1 : 0;
-    const int nq = (n + kStripsPerGroup - 1) / kStripsPerGroup;
-    a.R = a.shot_groups ? choose_rows(c, nq * kStripsPerGroup, a.nstrips)
-                        : choose_rows(c, n, a.ngroups * kStripsPerGroup);
+    a.sg = c->sg;
+    const int nq = (n + a.sg - 1) / a.sg;                              // shot groups
+    const int spg = kStripsPerGroup / a.sg;                            // strips per workgroup
+    const int ngs = (a.nstrips + spg - 1) / spg;                       // strip groups
+    a.R = (a.sg > 1) ? choose_rows(c, nq * kStripsPerGroup, ngs) : choose_rows(c, n, a.ngroups * kStripsPerGroup);
     a.nsegs = (g.ny + a.R - 1) / a.R;
     a.nblocks = n * a.ngroups * a.nsegs;
     a.nblocks8 = (a.nblocks + 7) / 8;
-    a.nblocks_sg = nq * a.nstrips * a.nsegs;
+    a.nblocks_sg = nq * ngs * a.nsegs;
     a.nblocks_sg8 = (a.nblocks_sg + 7) / 8;
     a.nblocks2 = (n >= 2 && c->two_shots) ? ((n + 1) / 2) * a.ngroups * a.nsegs : 0;
     a.nblocks2_8 = (a.nblocks2 + 7) / 8;

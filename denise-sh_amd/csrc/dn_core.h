@@ -79,7 +79,7 @@ struct dn_ctx {
     std::vector<Graph> graphs;
     int graph_steps = -1;            // -1 default chunk, 0 eager launches, > 0 requested chunk
     bool two_shots = false;          // plain steps advance two shots per wave (dn_set_wave_shots)
-    bool shot_groups = false;        // workgroups take one strip of several shots (dn_set_group_layout)
+    int sg = 1;                      // shots per workgroup of the step kernel (dn_set_group_layout)
     int *d_ntbase = nullptr;         // device step counter read by replayed chunks
     int nt_limit = 0;                // > 0: passes stop after this step (autotuner)
     long long graph_replays = 0;

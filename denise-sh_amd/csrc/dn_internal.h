@@ -71,8 +71,10 @@ struct StepArgs {
     int ngroups;                   // workgroups of kStripsPerGroup adjacent strips per segment row
     int nblocks, nblocks8;         // real workgroups and workgroups per XCD slice (one shot per wave)
     int nblocks2, nblocks2_8;      // the same with two shots per wave (0: not used)
-    int shot_groups;               // 1: a workgroup takes one strip of kStripsPerGroup shots
-    int nblocks_sg, nblocks_sg8;   // workgroups in that layout (one shot per wave)
+    int sg;                        // shots per workgroup: 1 = kStripsPerGroup adjacent strips of one
+                                   // shot; 2, 4, 8 = kStripsPerGroup / sg adjacent strips of sg
+                                   // consecutive shots (one shot per wave)
+    int nblocks_sg, nblocks_sg8;   // workgroups in a layout with sg > 1
     float DT, DH, dhi, dthalbe;
     float hc[7];
     float hcy6;                    // coefficient used for the 6th y-term of update_v (Q6: hc[5])

@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     constexpr bool S16 = ADJ && EX == 2;
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
-    const int nblk = (NSH == 2) ? a.nblocks2 : (a.shot_groups ? a.nblocks_sg : a.nblocks);
-    const int nblk8 = (NSH == 2) ? a.nblocks2_8 : (a.shot_groups ? a.nblocks_sg8 : a.nblocks8);
+    const int nblk = (NSH == 2) ? a.nblocks2 : (a.sg > 1 ? a.nblocks_sg : a.nblocks);
+    const int nblk8 = (NSH == 2) ? a.nblocks2_8 : (a.sg > 1 ? a.nblocks_sg8 : a.nblocks8);
     const int lin = (w & 7) * nblk8 + (w >> 3);
     if (lin >= nblk) return;                      // whole workgroup
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -107,22 +107,28 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // s_barrier per row).  Strip groups (default): they take adjacent strips
     // of one shot, so their loads of a row arrive at the memory as one
     // contiguous run per field; spare waves of the last group only keep the
-    // row barriers.  Shot groups (one shot per wave only):
-    // they take the same strip of consecutive shots, so their loads of the
-    // static model rows meet in the CU's L1; a wave without an active shot
-    // fetches only those rows and stores nothing.
-    const bool shot_grp = NSH == 1 && a.shot_groups;
+    // row barriers.  Shot groups (sg > 1, one shot per wave only): wave wv
+    // takes shot wv % sg of sg consecutive shots and strip wv / sg of
+    // kStripsPerGroup / sg adjacent strips, so the sg waves of a strip load
+    // the same static model rows in the same row iteration and those loads
+    // meet in the CU's L1; a wave without an active shot fetches only those
+    // rows and stores nothing.
+    const bool shot_grp = NSH == 1 && a.sg > 1;
     int p, strip, seg, q0 = 0;
     bool live;
     if (shot_grp) {
-        const int nq = (a.nshots + kStripsPerGroup - 1) / kStripsPerGroup;
+        const int sg = a.sg, spg = kStripsPerGroup / sg;
+        const int nq = (a.nshots + sg - 1) / sg;
+        const int ngs = (a.nstrips + spg - 1) / spg;
         const int q = lin % nq;
         const int tile = lin / nq;
-        strip = tile % a.nstrips;
-        seg = tile / a.nstrips;
-        q0 = q * kStripsPerGroup;
-        p = q0 + wv;
-        live = true;
+        const int grp = tile % ngs;
+        seg = tile / ngs;
+        q0 = q * sg;
+        p = q0 + wv % sg;
+        const int st = grp * spg + wv / sg;
+        live = st < a.nstrips;
+        strip = live ? st : a.nstrips - 1;
     } else {
         const int npair = (a.nshots + NSH - 1) / NSH;
         p = lin % npair;
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     }
     if (shot_grp) {
         // the workgroup goes on while any of its shots does (barriers)
-        for (int k = 0; k < kStripsPerGroup && !any; ++k) {
+        for (int k = 0; k < a.sg && !any; ++k) {
             const int t = q0 + k;
             if (t < a.nshots) {
                 const int S = a.shot_map[t];
@@ -684,7 +690,7 @@ static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hi
 {
     // plain steps of batches with two or more shots take two shots per wave
     const bool two = !extra && a.nshots >= 2 && a.nblocks2 > 0;
-    const int n8 = two ? a.nblocks2_8 : (a.shot_groups ? a.nblocks_sg8 : a.nblocks8);
+    const int n8 = two ? a.nblocks2_8 : (a.sg > 1 ? a.nblocks_sg8 : a.nblocks8);
     dim3 grid((unsigned)(n8 * 8)), block(kWave * kStripsPerGroup);
     if (adjoint) {
         if (extra && a.snap_kind != 0)

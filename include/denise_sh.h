@@ -277,10 +277,12 @@ int dn_set_exec(dn_ctx *ctx, int graph_steps);
  * wave).  Results are identical; dn_autotune picks the faster. */
 int dn_set_wave_shots(dn_ctx *ctx, int shots_per_wave);
 /* Workgroup layout of the time-step kernel (one shot per wave): 0 (default)
- * eight adjacent strips of one shot; 1 the same strip of eight consecutive
- * shots of the batch, whose loads of the static model fields meet in the
- * compute unit's cache.  Results are identical; dn_autotune picks the
- * faster.  No counterpart in the reference (a launch-shape choice). */
+ * eight adjacent strips of one shot; 2 or 4: four or two adjacent strips of
+ * 2 or 4 consecutive shots; 1 or 8: one strip of eight consecutive shots.
+ * With several shots per workgroup, their waves load the same static model
+ * rows in the same row iteration, and those loads meet in the compute
+ * unit's cache.  Results are identical; dn_autotune picks the fastest.  No
+ * counterpart in the reference (a launch-shape choice). */
 int dn_set_group_layout(dn_ctx *ctx, int mode);
 
 /* Tile-shape autotuner: times nsteps forward steps of the first nshots

@@ -56,7 +56,7 @@ def test_calls_reject_bad_input_and_the_context_stays_usable():
         (lambda: ctx.set_shots(np.array([20], np.int32), np.array([3], np.int32), ricker(100, DT, 60.0)[None],
                                np.array([101], np.int32)), "outside 1..100"),
         (lambda: ctx.set_wave_shots(3), "1 or 2"),
-        (lambda: ctx.set_group_layout(2), "adjacent strips"),
+        (lambda: ctx.set_group_layout(3), "adjacent strips"),
     ]
     for call, text in bad:
         with pytest.raises(dn.DeniseError) as e:

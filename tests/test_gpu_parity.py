@@ -364,11 +364,13 @@ def test_two_shots_per_wave_equals_one(L):
 
 @pytest.mark.parametrize("snap", [0, 1])
 def test_shot_group_layout_equals_strip_groups(snap):
-    """dn_set_group_layout(1) (workgroups of one strip of four shots) gives
-    the seismograms and gradients of the default layout bit for bit: six
-    shots (the second group has two spare waves), uneven record lengths (a
-    group's shots finish at different steps), with one and two shots per
-    wave on the plain steps, FP32 and BF16 snapshots."""
+    """dn_set_group_layout 2, 4 and 8 (workgroups of 4 strips x 2 shots, 2
+    strips x 4 shots, 1 strip x 8 shots) give the seismograms and gradients
+    of the default layout (8 strips of one shot) bit for bit: six shots (the
+    last shot group has spare waves for 4 and 8) on 3 strips (the last strip
+    group has spare waves for 2 and 4), uneven record lengths (a group's
+    shots finish at different steps), with one and two shots per wave on the
+    plain steps, FP32 and BF16 snapshots."""
     nx, ny, nt = 130, 52, 300
     nts = [300, 240, 280, 300, 200, 260]
     ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=6, dtinv=3, nts=nts, snap_precision=snap)
@@ -377,7 +379,7 @@ def test_shot_group_layout_equals_strip_groups(snap):
         obs[s, :, n:] = 0.0
     ctx.set_observed(obs)
     out = {}
-    for layout in (0, 1):
+    for layout in (0, 2, 4, 8):
         for nsh in (1, 2):
             ctx.set_group_layout(layout)
             ctx.set_wave_shots(nsh)
