@@ -344,7 +344,18 @@ def fwi_learned_run(dn, W, args, nshots, prec):
     lsel = np.linspace(0, nshots_t - 1, min(args.label_shots, nshots_t)).round().astype(int)
     lab = make_context(dn, WT, args, nshots_t, prec, eprecond=0, shots=lsel)
     models = [pt.random_checkerboard(WT["nx"], WT["ny"], WT["dh"], 7000 + k) for k in range(args.pairs)]
-    X, T = pt.pairs_on_context(lab, models, nprobes=args.probes, seed0=100, log=log)
+    geometry = None
+    if args.train_geometry == "mixed":
+        # each pair its own acquisition: a random subset of 8..S shots of the
+        # workload's positions and a record of 0.4..1 of its length
+        sig_all = signals_of(dn, WT, nshots_t)
+        grng = np.random.default_rng(4711)
+        def geometry(k):
+            m = int(grng.integers(8, min(args.label_shots, nshots_t) + 1))
+            sel = np.sort(grng.choice(nshots_t, size=m, replace=False))
+            nt_k = int(grng.uniform(0.4, 1.0) * WT["nt"])
+            lab.set_shots(WT["src_i"][sel], WT["src_j"][sel], sig_all[sel], np.full(m, nt_k, np.int32))
+    X, T = pt.pairs_on_context(lab, models, nprobes=args.probes, seed0=100, log=log, geometry=geometry)
     lab.close()
     t_labels = time.perf_counter() - t0
     ntr = max(1, (4 * args.pairs) // 5)
@@ -407,6 +418,7 @@ def fwi_learned_run(dn, W, args, nshots, prec):
         "value": runs["learned_fixed"]["ratio_to_plain"], "unit": "ratio", "higher_is_better": False,
         "n_gpus": 1, "iterations": args.fwi_learned, "plain_final_misfit": target, "plain_seconds": t_plain,
         "training": {"grid": "%dx%d" % (WT["nx"], WT["ny"]), "model_family": "random_checkerboard seeds 7000+",
+                     "geometry": args.train_geometry,
                      "pairs": args.pairs, "train_pairs": ntr, "probes_per_label": args.probes,
                      "shots_per_label": int(len(lsel)), "adam_steps": args.train_steps,
                      "loss_first_last": [round(hist[0], 5), round(hist[-1], 5)], "held_out": held,
@@ -503,6 +515,9 @@ def main():
     ap.add_argument("--probes", type=int, default=4, help="--fwi-learned: Hessian probes per label")
     ap.add_argument("--label-shots", type=int, default=20, help="--fwi-learned: shots of each label's gradients")
     ap.add_argument("--train-steps", type=int, default=300, help="--fwi-learned: Adam steps")
+    ap.add_argument("--train-geometry", default="fixed", choices=["fixed", "mixed"],
+                    help="--fwi-learned: every pair with the --label-shots acquisition (fixed) or with a random "
+                         "subset of 8..label-shots shots and a record of 0.4..1 of the length (mixed)")
     ap.add_argument("--modes", default=",".join(LEARNED_MODES),
                     help="--fwi-learned: comma-separated runs (plain and learned_fixed required)")
     ap.add_argument("--train-config", default=None, choices=["c4", "c2", "c3", "c5"],

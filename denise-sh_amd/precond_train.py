@@ -55,14 +55,18 @@ def random_checkerboard(nx, ny, dh, seed):
     return vs_true, v1.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)
 
 
-def pairs_on_context(ctx, models, nprobes=8, eps=1e-2, seed0=100, log=None):
+def pairs_on_context(ctx, models, nprobes=8, eps=1e-2, seed0=100, log=None, geometry=None):
     """Training pairs on a context that already holds receivers and shots
     (EPRECOND 0 for the labels): for each (vs_true, vs_start, rho, qs) the
     observed data from vs_true, the gradient at vs_start, the input maps and
-    the probe label.  Returns X [n][3][H8][W8], T [n][H8][W8]."""
+    the probe label.  geometry(k), if given, sets the shots of pair k on the
+    context first (acquisitions that vary over the training set).  Returns
+    X [n][3][H8][W8], T [n][H8][W8]."""
     n = len(models)
     X = T = None
     for k, (vt, v0, rho, qs) in enumerate(models):
+        if geometry is not None:
+            geometry(k)
         ctx.set_model(vt, rho, qs)
         obs = ctx.forward()
         ctx.set_observed(obs)
