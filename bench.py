@@ -6,7 +6,7 @@ the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
 
 * ``c4`` (default, the headline): NX = 4000, NY = 800, DH = 0.1 m,
   DT = 5e-5 s, NT = 12 000; vs in four layers 80 / 150 / 250 / 400 m/s with
-  twelve seeded Gaussian lenses (+-15 %, sigma 3 m) in the true model, the
+  twelve seeded Gaussian lenses (+-15 %, sigma 3 m, 2-40 m deep) in the true model, the
   plain layers as the model the gradient is taken at; rho 1800 -> 2000;
   32 shots evenly spread from 10 m to 390 m along the row j = 5 (0.5 m
   deep), 60 Hz Ricker, 394
@@ -14,9 +14,7 @@ the sides and bottom, free surface on top, snapshots every DTINV = 4 steps):
   three fields (115 GB in FP32), so HBM holds two shots at a time: one step
   is the gradient of one such batch (forward with snapshot storage,
   residuals, adjoint with correlation, energy preconditioning, chain rule),
-  the steps walk through the 16 batches of the 32 shots in turn.  Most
-  lenses lie deeper than the 0.6 s record illuminates, so the residuals
-  are small (the work per step does not depend on the data values).
+  the steps walk through the 16 batches of the 32 shots in turn.
 * ``c2``: NX = 1200, NY = 300, DH = 0.2 m, DT = 1e-4 s, NT = 10 000; vs
   100 -> 350 m/s with depth times a +-10 % checkerboard of 8 m cells (true)
   and the plain gradient (start); rho 1700 -> 2000; 40 shots, 25 Hz Ricker,
@@ -88,8 +86,9 @@ def _precond_train():
 
 def layered_lenses(nx, ny, dh, seed):
     """vs layers 80 / 150 / 250 / 400 m/s with depth and twelve seeded
-    Gaussian lenses (+-15 %, sigma 3 m) in the true model; the plain layers
-    as the starting model; rho 1800 -> 2000; Q 20."""
+    Gaussian lenses (+-15 %, sigma 3 m, centres between 2 % and 50 % of the
+    depth) in the true model; the plain layers as the starting model;
+    rho 1800 -> 2000; Q 20."""
     j = np.arange(ny)[:, None]
     i = np.arange(nx)[None, :]
     layer = np.minimum(4 * j // ny, 3)
@@ -97,7 +96,7 @@ def layered_lenses(nx, ny, dh, seed):
     rng = np.random.default_rng(seed)
     pert = np.zeros((ny, nx))
     for _ in range(12):
-        cx, cy = rng.uniform(0, nx * dh), rng.uniform(0.2 * ny * dh, ny * dh)
+        cx, cy = rng.uniform(0, nx * dh), rng.uniform(0.02 * ny * dh, 0.5 * ny * dh)
         pert += rng.choice([-1.0, 1.0]) * 0.15 * np.exp(-(((i * dh - cx) ** 2 + (j * dh - cy) ** 2) / (2 * 3.0 ** 2)))
     rho = (1800.0 + 200.0 * j / (ny - 1) + 0.0 * i).astype(np.float32)
     return (vs0 * (1 + pert)).astype(np.float32), vs0.astype(np.float32), rho, np.full((ny, nx), 20.0, np.float32)

@@ -256,19 +256,19 @@ def test_reduced_precision_snapshots_4000x800():
 
 def test_4000x800_window_against_fp64_oracle():
     """The reduced-precision workload's grid spacing, time step and model on
-    a cropped window (600 x 300 cells: x 319-379 m, the top 30 m, over the
-    shallowest lens) with a 6000-step record, one shot, every fourth step
-    stored, FP32 / BF16 / FP16 snapshot storage.  Against the FP32 oracle:
-    FP32 storage within the gradient bar.  Against the FP64 oracle (the
-    observed data modelled in FP64 as well): two residuals.
-    (a) The workload's own: layered start model, the lens the only
-        difference; the residual is ~3e-6 of the data, the size of FP32
-        rounding in a 6000-step simulation, so the FP32 gradient (the
-        reference's arithmetic) differs from the FP64 one by tens of
-        percent -- printed, not bounded.
-    (b) The start model 5 % slower everywhere: a residual of the data's
-        size; the FP32 gradient is within 1e-3 of the FP64 one, BF16 and
-        FP16 storage within 1e-2."""
+    a cropped window (600 x 300 cells: x 319-379 m, the top 30 m, over four
+    lenses, the shallowest 2.3 m deep) with a 6000-step record, one shot,
+    every fourth step stored, FP32 / BF16 / FP16 snapshot storage.  Against
+    the FP32 oracle: FP32 storage within the gradient bar.  Against the FP64
+    oracle (the observed data modelled in FP64 as well): two residuals.
+    (a) The workload's own: layered start model, the lenses the only
+        difference.  The FP32 oracle's gradient is 6e-7 from the FP64
+        one; the product's FP32 path within 1e-5, BF16 and FP16 storage
+        within 1e-2.
+    (b) The start model 5 % slower than the true one everywhere: a residual
+        spread over the whole record, where the FP32 oracle's gradient is
+        1.1e-3 from the FP64 one (cancellation in the correlation sum);
+        the product's FP32 path within 2e-3, BF16 and FP16 within 1e-2."""
     sys.path.insert(0, ROOT)
     import bench
     from pyoracle import make_cfg
@@ -308,7 +308,6 @@ def test_4000x800_window_against_fp64_oracle():
             er = _report_grad("%s grad_rho" % tag, grho, r32["grad_rho"], r64["grad_rho"])
             if prec == 0:
                 assert ev <= GRAD_TOL and er <= GRAD_TOL
-            if name.startswith("b"):
-                bound = 1e-3 if prec == 0 else 1e-2
-                assert rel_l2(gvs, r64["grad_vs"]) <= bound and rel_l2(grho, r64["grad_rho"]) <= bound
+            bound = (1e-5 if name.startswith("a") else 2e-3) if prec == 0 else 1e-2
+            assert rel_l2(gvs, r64["grad_vs"]) <= bound and rel_l2(grho, r64["grad_rho"]) <= bound
             ctx.close()
