@@ -46,7 +46,7 @@ class Params(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("maxrelerror", C.c_int), ("fw", C.c_int),
                 ("free_surf", C.c_int), ("L", C.c_int), ("fl", C.c_float * 4), ("dt", C.c_float), ("dh", C.c_float),
                 ("damping", C.c_float), ("fpml", C.c_float), ("npower", C.c_float), ("k_max_pml", C.c_float),
-                ("dtinv", C.c_int), ("snap_precision", C.c_int), ("eprecond", C.c_int)]
+                ("dtinv", C.c_int), ("snap_precision", C.c_int), ("eprecond", C.c_int), ("boundary", C.c_int)]
 
 
 class Taper(C.Structure):
@@ -182,13 +182,14 @@ class Context:
     """Handle API wrapper: one dn_ctx on the current HIP device."""
 
     def __init__(self, nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, fl=(), dt=1e-4, dh=0.25,
-                 damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, dtinv=1, snap_precision=0, eprecond=1):
+                 damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, dtinv=1, snap_precision=0, eprecond=1,
+                 boundary=0):
         p = Params()
         p.nx, p.ny, p.fdorder, p.maxrelerror, p.fw, p.free_surf, p.L = nx, ny, fdorder, maxrelerror, fw, free_surf, L
         for k, v in enumerate(fl):
             p.fl[k] = v
         p.dt, p.dh, p.damping, p.fpml, p.npower, p.k_max_pml = dt, dh, damping, fpml, npower, k_max_pml
-        p.dtinv, p.snap_precision, p.eprecond = dtinv, snap_precision, eprecond
+        p.dtinv, p.snap_precision, p.eprecond, p.boundary = dtinv, snap_precision, eprecond, boundary
         self.p = p
         self.nx, self.ny = nx, ny
         self._h = lib().dn_create(C.byref(p))

@@ -154,7 +154,7 @@ bool ensure(DropIn &d)
     const dn_globals &G = d.G;
     if (!ensure_stream(d)) return false;
     Geo &g = d.g;
-    g = make_geo(G.NX, G.NY, G.FW, G.FREE_SURF, G.L);
+    g = make_geo(G.NX, G.NY, G.FW, G.FREE_SURF, G.L, G.BOUNDARY);
     if (d.mode) return true;   // resident arrays are allocated as they are seen
     if (d.cap_plane == (size_t)g.plane && d.cap_L == G.L && d.cap_fw == G.FW && d.cap_nx == G.NX &&
         d.cap_ny == G.NY)
@@ -397,12 +397,15 @@ extern "C" int dn_set_globals(const dn_globals *G)
 {
     if (!G) return dn_fail(nullptr, "dn_set_globals: NULL");
     if (G->FDORDER < 2 || G->FDORDER > 12 || (G->FDORDER % 2)) return dn_fail(nullptr, "wrong FDORDER %d", G->FDORDER);
-    if (G->BOUNDARY != 0) return dn_fail(nullptr, "BOUNDARY=1 (periodic wavefield) is not provided");
+    if (G->BOUNDARY != 0 && G->BOUNDARY != 1) return dn_fail(nullptr, "BOUNDARY must be 0 or 1");
+    if (G->BOUNDARY == 1 && G->NX < G->FDORDER)
+        return dn_fail(nullptr, "BOUNDARY=1 needs NX >= FDORDER (got NX=%d)", G->NX);
     if (G->GRAD_FORM != 2) return dn_fail(nullptr, "only GRAD_FORM=2 is provided");
     if (G->L < 0 || G->L > kMaxL) return dn_fail(nullptr, "L=%d outside 0..%d", G->L, kMaxL);
     DropIn &d = state();
     if (d.mode && d.have_globals &&
-        (G->NX != d.G.NX || G->NY != d.G.NY || G->L != d.G.L || G->FW != d.G.FW || G->FDORDER != d.G.FDORDER))
+        (G->NX != d.G.NX || G->NY != d.G.NY || G->L != d.G.L || G->FW != d.G.FW || G->FDORDER != d.G.FDORDER ||
+         G->BOUNDARY != d.G.BOUNDARY))
         return dn_fail(nullptr, "dn_set_globals: grid changed while drop-in arrays are resident; "
                                 "call dn_dropin_resident(0) first");
     d.G = *G;
@@ -423,7 +426,7 @@ extern "C" int dn_dropin_resident(int mode)
     if (mode) {
         if (!d.have_globals) return dn_fail(nullptr, "dn_dropin_resident: call dn_set_globals() first");
         if (!ensure_stream(d)) return -1;
-        d.g = make_geo(d.G.NX, d.G.NY, d.G.FW, d.G.FREE_SURF, d.G.L);
+        d.g = make_geo(d.G.NX, d.G.NY, d.G.FW, d.G.FREE_SURF, d.G.L, d.G.BOUNDARY);
         if (d.mode == 2 && mode == 1 && !sync_to_host(d)) return -1;   // deliver deferred seismograms
     }
     d.mode = mode;

@@ -30,11 +30,13 @@ struct Geo {
     int rows;           // padded rows
     long long plane;    // pitch * rows
     int fw, free_surf, L;
+    int periodic;       // BOUNDARY 1: wavefields wrap across the left and right edges, no x-CPML
 };
 
-inline Geo make_geo(int nx, int ny, int fw, int free_surf, int L)
+inline Geo make_geo(int nx, int ny, int fw, int free_surf, int L, int periodic = 0)
 {
     Geo g{};
+    g.periodic = periodic ? 1 : 0;
     g.nx = nx;
     g.ny = ny;
     g.xoff = kPadX;

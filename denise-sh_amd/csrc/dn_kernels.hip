@@ -178,8 +178,13 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     const int lane = threadIdx.x % kWave;
     const int c0 = 1 + strip * a.ow;          // first owned column
     const int c = c0 - (K - 1) + lane;        // column of this lane
-    const bool in_x = (c >= 1) && (c <= g.nx);
-    const bool own_x = live && (lane >= K - 1) && (lane < K - 1 + a.ow) && in_x;
+    // BOUNDARY 1: a lane left of column 1 or right of NX computes the
+    // column it wraps to (the halo the reference fills by exchange_v /
+    // exchange_s); every address of the lane uses cw.  NX >= 64, so one
+    // wrap lands inside the grid.
+    const int cw = !g.periodic ? c : (c < 1 ? c + g.nx : (c > g.nx ? c - g.nx : c));
+    const bool in_x = g.periodic || ((c >= 1) && (c <= g.nx));
+    const bool own_x = live && (lane >= K - 1) && (lane < K - 1 + a.ow) && (c >= 1) && (c <= g.nx);
     const int j0 = 1 + seg * a.R;
     const int j1 = min(j0 + a.R - 1, g.ny);
     const int nxny = g.nx * g.ny;                 // per-lane offsets are 32-bit
@@ -245,7 +250,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // unconditional and cells outside the interior read as 0 (wavefields)
     // or are not used (model fields).  Lanes 0..K-1 also fetch the K sxy
     // columns left of the wave, lanes K..2K-2 the K-1 columns right of it.
-    const int hcol = (lane < K) ? (c0 - 2 * K + 1 + lane) : ((lane < 2 * K - 1) ? (c0 + 65 - 2 * K + lane) : c);
+    const int hcol0 = (lane < K) ? (c0 - 2 * K + 1 + lane) : ((lane < 2 * K - 1) ? (c0 + 65 - 2 * K + lane) : c);
+    const int hcol = !g.periodic ? hcol0 : (hcol0 < 1 ? hcol0 + g.nx : (hcol0 > g.nx ? hcol0 - g.nx : hcol0));
     const int hslot = (lane < K) ? lane : (kWave + lane);
     const long long shot_off = (long long)s * nxny;
     float *__restrict__ grb = (EX && a.grad_rho) ? a.grad_rho + shot_off : nullptr;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // absorbing regions do not overlap, so a lane (row) lies in at most one.
     // x: the lane's profile index (1..2FW, 0 outside) and its six profile
     // values, constant over the rows; y: the profile index of a row
-    const int xk = (fw > 0 && in_x) ? ((c <= fw) ? c : ((c >= g.nx - fw + 1) ? c - g.nx + fw2 : 0)) : 0;
+    const int xk = (fw > 0 && in_x && !g.periodic) ? ((c <= fw) ? c : ((c >= g.nx - fw + 1) ? c - g.nx + fw2 : 0)) : 0;
     const bool xin = xk > 0;
     const float vKx = pmlb[0 * pn + xk], vax = pmlb[1 * pn + xk], vbx = pmlb[2 * pn + xk];
     const float sKx = pmlb[3 * pn + xk], sax = pmlb[4 * pn + xk], sbx = pmlb[5 * pn + xk];
@@ -284,7 +290,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
 #pragma unroll
     for (int u = 0; u < NSH; ++u) {
 #pragma unroll
-        for (int t = 0; t < 2 * K - 1; ++t) zw[u][t] = act[u] ? ldf(szi, sof[u] + gi32(g, jstart - K + t, c) * 4u) : 0.0f;
+        for (int t = 0; t < 2 * K - 1; ++t) zw[u][t] = act[u] ? ldf(szi, sof[u] + gi32(g, jstart - K + t, cw) * 4u) : 0.0f;
 #pragma unroll
         for (int t = 0; t < 2 * K; ++t) vw[u][t] = 0.0f;
 #pragma unroll
@@ -328,8 +334,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         return (a.snap_kind == 1 ? bf162f(h) : h2f(h)) * scale;
     };
     auto fetch = [&](int jv, RowM &m, RowS (&o)[NSH]) {
-        const unsigned gv = gi32(g, jv, c) * 4u;
-        const unsigned gs = gi32(g, jv - K, c) * 4u;
+        const unsigned gv = gi32(g, jv, cw) * 4u;
+        const unsigned gs = gi32(g, jv - K, cw) * 4u;
         // the stress-row inputs are fetched only where the iteration updates
         // the stress (row jv - K inside the segment): the first 2K-1
         // iterations of a segment advance the velocity alone (wave-uniform)
@@ -369,7 +375,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             m.sby = pmlb[(b0 + 2) * pn + yks];
         }
         const bool vrow = jv >= 1 && jv <= g.ny, srow = jsr >= 1 && jsr <= g.ny;
-        const unsigned gz = gi32(g, jv + K - 1, c) * 4u, gh = gi32(g, jv, hcol) * 4u;
+        const unsigned gz = gi32(g, jv + K - 1, cw) * 4u, gh = gi32(g, jv, hcol) * 4u;
 #pragma unroll
         for (int u = 0; u < NSH; ++u) {
             if (!act[u]) continue;
@@ -378,9 +384,9 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             o[u].halo = ldf(sxi, sof[u] + gh);
             o[u].vy = ldf(vyi, sof[u] + gv);
             if (xin && vrow) o[u].px = ldf(pxi, spx[u] + (unsigned)((jv - 1) * fw2 + (xk - 1)) * 4u);
-            if (ykv > 0 && in_x) o[u].py = ldf(pyi, spy[u] + (unsigned)((ykv - 1) * g.nx + (c - 1)) * 4u);
+            if (ykv > 0 && in_x) o[u].py = ldf(pyi, spy[u] + (unsigned)((ykv - 1) * g.nx + (cw - 1)) * 4u);
             if (xin && srow && sneed) o[u].qx = ldf(pvxb, spx[u] + (unsigned)((jsr - 1) * fw2 + (xk - 1)) * 4u);
-            if (yks > 0 && in_x) o[u].qy = ldf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u);
+            if (yks > 0 && in_x) o[u].qy = ldf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (cw - 1)) * 4u);
             if (VISCO && sneed) {
 #pragma unroll
                 for (int l = 0; l + 1 < L; l += 2) {
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             // snapshot storage (FP32 or 16-bit) is fixed per variant.
             RowS &e = o[0];
             const bool vin = (jv >= j0 && jv <= j1) && own_x;
-            const unsigned cv = vin ? (unsigned)((jv - 1) * g.nx + (c - 1)) : 0u;
+            const unsigned cv = vin ? (unsigned)((jv - 1) * g.nx + (cw - 1)) : 0u;
             if ((ADJ && corr) || (!ADJ && hsum)) {
                 e.gr = grb[cv];
                 if (ADJ && !S16) e.f0 = snb[cv];
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
             const int jsn = jv - K;
             const bool sin_ = (jsn >= j0 && jsn <= j1) && own_x;
-            const unsigned cs = sin_ ? (unsigned)((jsn - 1) * g.nx + (c - 1)) : 0u;
+            const unsigned cs = sin_ ? (unsigned)((jsn - 1) * g.nx + (cw - 1)) : 0u;
             if ((ADJ && corr) || (!ADJ && fstore)) e.wa = wab[cs];
             if ((ADJ && corr) || (!ADJ && hsum)) e.gu = gub[cs];
             if (ADJ && corr && !S16) {
@@ -437,7 +443,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         if (!ADJ)
             for (int l = sb[u]; l < se[u]; ++l) {
                 const int si = a.src_i[l], sj = a.src_j[l];
-                if (sj >= jstart && sj <= jend && si >= c0 - (K - 1) && si <= c0 - K + kWave) has_src[u] = true;
+                if (sj >= jstart && sj <= jend && (g.periodic || (si >= c0 - (K - 1) && si <= c0 - K + kWave)))
+                    has_src[u] = true;
             }
     }
     // per-snapshot max |x| of the values this wave stages for the 16-bit
@@ -479,15 +486,15 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 const int ykv = ykof(jv);
                 if (ykv > 0) {   // y-CPML, top (no free surface) or bottom
                     const float p = cm.vby * cr[u].py + cm.vay * syy_y;
-                    if (own_v) stf(pyo, spy[u] + (unsigned)((ykv - 1) * g.nx + (c - 1)) * 4u, p);
+                    if (own_v) stf(pyo, spy[u] + (unsigned)((ykv - 1) * g.nx + (cw - 1)) * 4u, p);
                     syy_y = syy_y / cm.vKy + p;
                 }
             }
-            const unsigned gi = gi32(g, jv, c);
+            const unsigned gi = gi32(g, jv, cw);
             const float vold = cr[u].vy;
             const float bb = cm.b;
             const float tsum = sxy_x + syy_y;
-            const unsigned ci = (unsigned)((jv - 1) * g.nx + (c - 1));
+            const unsigned ci = (unsigned)((jv - 1) * g.nx + (cw - 1));
             if (EX && !ADJ && hsum && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1 (hess_utty)
                 grb[ci] = cr[u].gr + (acc * acc);
@@ -510,12 +517,12 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             if (!ADJ) {
                 if (has_src[u])
                     for (int l = sb[u]; l < se[u]; ++l)
-                        if (a.src_j[l] == jv && a.src_i[l] == c)
+                        if (a.src_j[l] == jv && a.src_i[l] == cw)
                             vn = vn + a.src_sig[(long long)l * a.nt_max + (nt - 1)];
             } else {
                 const int off = cm.rrow;
                 if (off >= 0) {
-                    const int r = a.rec_tab[off + c];
+                    const int r = a.rec_tab[off + cw];
                     if (r >= 0) vn = vn + a.resid[((long long)sl[u] * a.ntr + r) * a.nt_max + (nt - 1)];
                 }
             }
@@ -529,7 +536,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 if (!ADJ && a.seis != nullptr) {
                     const int off = cm.rrow;
                     if (off >= 0) {
-                        const int r = a.rec_tab[off + c];
+                        const int r = a.rec_tab[off + cw];
                         if (r >= 0) a.seis[((long long)sl[u] * a.ntr + r) * a.nt_max + (nt - 1)] = vn;
                     }
                 }
@@ -573,11 +580,11 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                     const int yks = ykof(js);
                     if (yks > 0) {   // y-CPML of the stress (profile chosen in fetch, Q2)
                         const float p = cm.sby * cr[u].qy + cm.say * vyy;
-                        stf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (c - 1)) * 4u, p);
+                        stf(pvyb, spy[u] + (unsigned)((yks - 1) * g.nx + (cw - 1)) * 4u, p);
                         vyy = vyy / cm.sKy + p;
                     }
                 }
-                const unsigned gs = gi32(g, js, c);
+                const unsigned gs = gi32(g, js, cw);
                 const float sxo = sq[u][0];
                 const float syo = zw[u][0];
                 float sxn, syn;
@@ -615,7 +622,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 stf(sxo_, sof[u] + gs * 4u, sxn);
                 stf(szo, sof[u] + gs * 4u, syn);
                 if (EX) {
-                    const unsigned ci = (unsigned)((js - 1) * g.nx + (c - 1));
+                    const unsigned ci = (unsigned)((js - 1) * g.nx + (cw - 1));
                     const float vc = vw[u][K - 1];
                     if (!ADJ && fstore) {
                         if (a.store_mode == 1) {
@@ -1158,8 +1165,15 @@ hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_
 
 // ---------------------------------------------------------------------------
 // Split kernels behind the reference-named drop-ins: one thread per interior
-// cell, in-place update exactly as the reference loop bodies.
+// cell, in-place update exactly as the reference loop bodies.  With BOUNDARY 1
+// the x neighbours beyond the edges are read from the columns they wrap to
+// (what exchange_v / exchange_s copy into the reference's halo).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int xwrap(const Geo &g, int i)
+{
+    return !g.periodic ? i : (i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i));
+}
+
 __global__ void k_split_v(SplitArgs a)
 {
     const Geo &g = a.g;
@@ -1167,21 +1181,21 @@ __global__ void k_split_v(SplitArgs a)
     const int j = blockIdx.y + 1;
     if (i > g.nx || j > g.ny) return;
     const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
-    float sxy_x = a.hc[1] * (a.sxy[gidx(g, j, i)] - a.sxy[gidx(g, j, i - 1)]);
+    float sxy_x = a.hc[1] * (a.sxy[gidx(g, j, i)] - a.sxy[gidx(g, j, xwrap(g, i - 1))]);
     float syy_y = a.hc[1] * (a.syz[gidx(g, j, i)] - a.syz[gidx(g, j - 1, i)]);
     for (int m = 2; m <= K; ++m) {
-        sxy_x = sxy_x + a.hc[m] * (a.sxy[gidx(g, j, i + m - 1)] - a.sxy[gidx(g, j, i - m)]);
+        sxy_x = sxy_x + a.hc[m] * (a.sxy[gidx(g, j, xwrap(g, i + m - 1))] - a.sxy[gidx(g, j, xwrap(g, i - m))]);
         const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
         syy_y = syy_y + hy * (a.syz[gidx(g, j + m - 1, i)] - a.syz[gidx(g, j - m, i)]);
     }
     if (fw > 0) {
         float *psx = a.psi_b, *psy = a.psi_a;
-        if (i <= fw) {
+        if (!g.periodic && i <= fw) {
             float &p = psx[(long long)(j - 1) * fw2 + (i - 1)];
             p = a.pml.bx[i] * p + a.pml.ax[i] * sxy_x;
             sxy_x = sxy_x / a.pml.Kx[i] + p;
         }
-        if (i >= g.nx - fw + 1) {
+        if (!g.periodic && i >= g.nx - fw + 1) {
             const int h1 = i - g.nx + fw2;
             float &p = psx[(long long)(j - 1) * fw2 + (h1 - 1)];
             p = a.pml.bx[h1] * p + a.pml.ax[h1] * sxy_x;
@@ -1218,22 +1232,22 @@ __global__ void k_split_s(SplitArgs a)
     const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
     const float dhi = VISCO ? (float)(1.0 / (double)a.DH) : a.DT / a.DH;
     const float dthalbe = (float)((double)a.DT / 2.0);
-    float vyx = a.hc[1] * (a.vy[gidx(g, j, i + 1)] - a.vy[gidx(g, j, i)]);
+    float vyx = a.hc[1] * (a.vy[gidx(g, j, xwrap(g, i + 1))] - a.vy[gidx(g, j, i)]);
     float vyy = a.hc[1] * (a.vy[gidx(g, j + 1, i)] - a.vy[gidx(g, j, i)]);
     for (int m = 2; m <= K; ++m) {
-        vyx = vyx + a.hc[m] * (a.vy[gidx(g, j, i + m)] - a.vy[gidx(g, j, i - m + 1)]);
+        vyx = vyx + a.hc[m] * (a.vy[gidx(g, j, xwrap(g, i + m))] - a.vy[gidx(g, j, xwrap(g, i - m + 1))]);
         vyy = vyy + a.hc[m] * (a.vy[gidx(g, j + m, i)] - a.vy[gidx(g, j - m + 1, i)]);
     }
     vyx = vyx * dhi;
     vyy = vyy * dhi;
     if (fw > 0) {
         float *pvy = a.psi_a, *pvx = a.psi_b;
-        if (i <= fw) {
+        if (!g.periodic && i <= fw) {
             float &p = pvx[(long long)(j - 1) * fw2 + (i - 1)];
             p = a.pml.bxh[i] * p + a.pml.axh[i] * vyx;
             vyx = vyx / a.pml.Kxh[i] + p;
         }
-        if (i >= g.nx - fw + 1) {
+        if (!g.periodic && i >= g.nx - fw + 1) {
             const int h1 = i - g.nx + fw2;
             float &p = pvx[(long long)(j - 1) * fw2 + (h1 - 1)];
             p = a.pml.bxh[h1] * p + a.pml.axh[h1] * vyx;

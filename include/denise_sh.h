@@ -41,7 +41,7 @@ typedef struct dn_globals {
     int FDORDER;         /* 2..12                                          */
     int FW;              /* CPML width                                     */
     int FREE_SURF;       /* 1: no top CPML (implicit free surface)         */
-    int BOUNDARY;        /* must be 0 (periodic wavefield not provided)    */
+    int BOUNDARY;        /* 0 or 1 (periodic left and right edges, NX >= 64) */
     int L;               /* relaxation mechanisms                          */
     int GRAD_FORM;       /* must be 2                                      */
     int QUELLTYPB;       /* adjoint source type (2: force in y)            */
@@ -183,6 +183,12 @@ typedef struct dn_params {
     int eprecond;        /* EPRECOND: 1 energy preconditioning sqrt(Ws Wr), 3 sqrt(Ws)
                             times the Plessix-Mulder receiver approximation
                             (eprecond1.c:15-50), 0 none */
+    int boundary;        /* BOUNDARY: 0 absorbing (CPML) left and right; 1 periodic:
+                            vy and sxy wrap across the left and right edges
+                            after each half step (exchange_v.c:99-161,
+                            exchange_s.c:95-157 on one domain) and the x-CPML
+                            is off (the !BOUNDARY guards of update_v_PML.c:63,71
+                            and update_s_*_PML.c); needs NX >= 64 */
 } dn_params;
 
 /* Context on the current HIP device; NULL on failure (dn_last_error(NULL)).
@@ -285,11 +291,14 @@ int dn_set_wave_shots(dn_ctx *ctx, int shots_per_wave);
  * counterpart in the reference (a launch-shape choice). */
 int dn_set_group_layout(dn_ctx *ctx, int mode);
 
-/* Tile-shape autotuner: times nsteps forward steps of the first nshots
- * shots for each candidate rows-per-segment, then one or two shots per
- * wave, the other workgroup layout (at half, equal and double the rows)
- * and graph replay, keeps the fastest setting and returns rows and graph
- * use. */
+/* Tile-shape autotuner: on the batch the gradient runs (as many of the
+ * first nshots shots as share a batch with their snapshots stored), times
+ * nsteps forward steps for each candidate rows-per-segment (0: the cost
+ * model's choice, then fixed heights), then one or two shots per wave, the
+ * workgroup layouts (at the cost model's, half, equal and double the rows)
+ * and graph replay (kept unless eager launches are more than 2 % faster),
+ * each the faster of two timed passes; keeps the fastest setting and
+ * returns rows (0: the cost model's) and graph use. */
 int dn_autotune(dn_ctx *ctx, int nshots, int nsteps, int *rows_per_segment, int *graph_steps);
 
 /* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis

@@ -22,7 +22,8 @@ class Cfg(C.Structure):
                 ("free_surf", C.c_int), ("L", C.c_int), ("nt", C.c_int), ("dtinv", C.c_int),
                 ("dt", C.c_double), ("dh", C.c_double), ("damping", C.c_double), ("fpml", C.c_double),
                 ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int),
-                ("c_vs", C.c_double), ("c_rho", C.c_double), ("skip_res", C.c_void_p), ("skip_en", C.c_void_p)]
+                ("c_vs", C.c_double), ("c_rho", C.c_double), ("skip_res", C.c_void_p), ("skip_en", C.c_void_p),
+                ("periodic", C.c_int)]
 
 
 def build():
@@ -30,9 +31,10 @@ def build():
 
 
 def make_cfg(nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, nt=100, dtinv=1, dt=1e-4, dh=0.25,
-             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=(), eprecond=1):
+             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=(), eprecond=1, boundary=0):
     c = Cfg()
     c.eprecond = eprecond
+    c.periodic = 1 if boundary else 0
     c.nx, c.ny, c.fdorder, c.maxrelerror, c.fw, c.free_surf, c.L, c.nt, c.dtinv = (
         nx, ny, fdorder, maxrelerror, fw, free_surf, L, nt, dtinv)
     # scalar parameters are float32 values in the reference (read as float)
@@ -186,6 +188,10 @@ class Oracle:
         return hv, hr
 
     # ---- single sweeps (padded arrays, shapes from padded_shape) ----
+    def wrap_x(self, cfg, a):
+        """BOUNDARY 1 halo exchange of one padded field (exchange_v / exchange_s on one domain)."""
+        self.fn("orc_wrap_x")(C.byref(cfg), self._r(a))
+
     def update_v(self, cfg, sw, vy, vyp1, syz, sxy, rho, hc, pml, psi_syy_y, psi_sxy_x):
         self.fn("orc_update_v")(C.byref(cfg), sw, self._r(vy), self._r(vyp1), self._r(syz), self._r(sxy),
                                 self._r(rho), self._r(hc), self._r(pml["K_x"]), self._r(pml["a_x"]),

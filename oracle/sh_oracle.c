@@ -73,6 +73,11 @@ typedef struct {
                             0: the means of the model passed in            */
     const unsigned char *skip_res, *skip_en;   /* [nshots][ntr] data
                             conditioning masks of orc_residual_cond, or NULL */
+    int periodic;        /* BOUNDARY 1: no CPML at the left and right edges
+                            (update_v_PML.c:63,71; update_s_*_PML.c: the same
+                            guards) and the wavefields wrapped across them
+                            after each half step (exchange_v.c:99-161 for vy,
+                            exchange_s.c:95-157 for sxy, FDORDER/2 columns) */
 } orc_cfg;
 
 #define ND(c) ((c)->fdorder / 2 + 1)
@@ -610,13 +615,14 @@ void F(orc_update_v)(const orc_cfg *c, int sw, real *vy, real *vyp1, const real 
         }
         if (FW > 0) {
             for (i = 1; i <= NX; i++) {
-                if (!(i <= FW || i >= NX - FW + 1 || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
+                const int xl = !c->periodic && i <= FW, xr = !c->periodic && i >= NX - FW + 1;
+                if (!(xl || xr || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
                 real sxy_x = sx[i], syy_y = sy[i];
-                if (i <= FW) {
+                if (xl) {
                     PSI_X(psi_sxy_x, c, j, i) = b_x[i] * PSI_X(psi_sxy_x, c, j, i) + a_x[i] * sxy_x;
                     sxy_x = sxy_x / K_x[i] + PSI_X(psi_sxy_x, c, j, i);
                 }
-                if (i >= NX - FW + 1) {
+                if (xr) {
                     h1 = i - NX + 2 * FW;
                     PSI_X(psi_sxy_x, c, j, h1) = b_x[h1] * PSI_X(psi_sxy_x, c, j, h1) + a_x[h1] * sxy_x;
                     sxy_x = sxy_x / K_x[h1] + PSI_X(psi_sxy_x, c, j, h1);
@@ -696,13 +702,14 @@ static inline void s_cpml_row(const orc_cfg *c, int j, real *gx, real *gy, const
     const int NX = c->nx, NY = c->ny, FW = c->fw;
     int i, h1;
     for (i = 1; i <= NX; i++) {
-        if (!(i <= FW || i >= NX - FW + 1 || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
+        const int xl = !c->periodic && i <= FW, xr = !c->periodic && i >= NX - FW + 1;
+        if (!(xl || xr || ((!c->free_surf) && (j <= FW)) || j >= NY - FW + 1)) continue;
         real vyx = gx[i], vyy = gy[i];
-        if (i <= FW) {
+        if (xl) {
             PSI_X(psi_vyx, c, j, i) = b_x_half[i] * PSI_X(psi_vyx, c, j, i) + a_x_half[i] * vyx;
             vyx = vyx / K_x_half[i] + PSI_X(psi_vyx, c, j, i);
         }
-        if (i >= NX - FW + 1) {
+        if (xr) {
             h1 = i - NX + 2 * FW;
             PSI_X(psi_vyx, c, j, h1) = b_x_half[h1] * PSI_X(psi_vyx, c, j, h1) + a_x_half[h1] * vyx;
             vyx = vyx / K_x_half[h1] + PSI_X(psi_vyx, c, j, h1);
@@ -968,12 +975,27 @@ static void sim_zero(orc_sim *s)
     memset(s->psi_vyx, 0, (size_t)2 * FW * c->ny * sizeof(real));
 }
 
+/* exchange_v.c:99-161 / exchange_s.c:95-157 with BOUNDARY 1 on one
+ * domain: columns 1..FDORDER/2 are copied to NX+1..NX+FDORDER/2 and
+ * NX-FDORDER/2+1..NX to 1-FDORDER/2..0 (fdo = FDORDER/2 + 1, l < fdo). */
+void F(orc_wrap_x)(const orc_cfg *c, real *a)
+{
+    const int NX = c->nx, NY = c->ny, k = c->fdorder / 2;
+    int j, l;
+    for (j = 1; j <= NY; j++)
+        for (l = 1; l <= k; l++) {
+            AT(a, c, j, NX + l) = AT(a, c, j, l);
+            AT(a, c, j, 1 - l) = AT(a, c, j, NX - l + 1);
+        }
+}
+
 static void sim_step(orc_sim *s, int sw, int ninj, const int *ii, const int *jj, const real *val)
 {
     const orc_cfg *c = s->c;
     F(orc_update_v)(c, sw, s->vy, s->vyp1, s->syz, s->sxy, s->rho, s->hc, PML_KX(s), PML_AX(s), PML_BX(s),
                     PML_KY(s), PML_AY(s), PML_BY(s), s->psi_syy_y, s->psi_sxy_x);
     F(orc_inject)(c, s->vy, ninj, ii, jj, val);
+    if (c->periodic) F(orc_wrap_x)(c, s->vy);   /* exchange_v after update_v_PML (denise.c:975) */
     if (c->L > 0)
         F(orc_update_s_visc)(c, s->vy, s->uy, s->uyx, s->syz, s->sxy, s->r, s->q, s->fipjp, s->f, s->dip, s->d,
                              s->bip, s->bjm, s->cip, s->cjm, s->hc, PML_KXH(s), PML_AXH(s), PML_BXH(s), PML_KY(s),
@@ -981,6 +1003,7 @@ static void sim_step(orc_sim *s, int sw, int ninj, const int *ii, const int *jj,
     else
         F(orc_update_s_elastic)(c, s->vy, s->uy, s->uyx, s->syz, s->sxy, s->uip, s->ujp, s->hc, PML_KXH(s),
                                 PML_AXH(s), PML_BXH(s), PML_KYH(s), PML_AYH(s), PML_BYH(s), s->psi_vyy, s->psi_vyx);
+    if (c->periodic) F(orc_wrap_x)(c, s->sxy);  /* exchange_s (denise.c:1009) */
 }
 
 /* number of stored steps, denise.c:287 */

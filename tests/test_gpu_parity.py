@@ -208,7 +208,7 @@ def test_autotuner_keeps_results():
     ref = ctx.forward()
     rows, graph = ctx.autotune(4, 30)
     say("  tuned rows %d graph %d" % (rows, graph))
-    assert 1 <= rows <= 60 and graph in (-1, 0)
+    assert 0 <= rows <= 60 and graph in (-1, 0)   # rows 0: the cost model's choice
     assert np.array_equal(ctx.forward(), ref)
 
 
