@@ -624,7 +624,10 @@ def main():
         "misfit_l2": l2,
     }
     if tuned is not None:
-        out["config"]["rows_per_segment"], out["config"]["graph_steps"] = tuned
+        shape = ctx.launch_shape()
+        out["config"]["rows_per_segment"], out["config"]["graph_steps"] = shape[0], shape[1]
+        out["config"]["tuned"] = {"rows_per_segment": shape[0], "graph_steps": shape[1], "wave_shots": shape[2],
+                                  "group_shots": shape[3], "note": "rows 0: the cost model's choice per batch"}
     elif args.rows:
         out["config"]["rows_per_segment"] = args.rows
     if args.max_batch:
@@ -637,8 +640,12 @@ def main():
     if args.compare_sequential:
         # the same gradient with one shot per batch: the per-shot sequential
         # GPU loop the batched launch replaces (the reference's shot loop,
-        # denise.c:875)
+        # denise.c:875), with the default launch shape for one shot (eight
+        # strips per workgroup, one shot per wave; a layout tuned for the
+        # batch could leave waves idle here)
         ctx.set_batch(1, args.rows)
+        ctx.set_group_layout(0)
+        ctx.set_wave_shots(1)
         t1 = time.perf_counter()
         ce_s = 0
         for k in range(max(1, args.steps)):

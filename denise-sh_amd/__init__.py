@@ -32,7 +32,7 @@ EXPORTS = [
     "dn_calc_res_noalloc",
     "dn_set_time_filt", "zero_fdveps", "zero_fdveps_visc", "dn_dropin_resident",
     "dn_dropin_sync", "dn_fwi_save_state", "dn_fwi_load_state",
-    "dn_fwi_set_learned_fixed",
+    "dn_fwi_set_learned_fixed", "dn_get_launch_shape",
 ]
 
 
@@ -346,6 +346,13 @@ class Context:
         strip of eight shots."""
         lib().dn_set_group_layout.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_set_group_layout(self._h, mode))
+
+    def launch_shape(self):
+        """(rows per segment (0: cost model), graph steps, shots per wave, shots per workgroup)."""
+        v = [C.c_int() for _ in range(4)]
+        lib().dn_get_launch_shape.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
+        self._chk(lib().dn_get_launch_shape(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, nshots, nsteps=40):
         rows, graph = C.c_int(), C.c_int()

@@ -550,6 +550,16 @@ extern "C" int dn_set_wave_shots(dn_ctx *c, int n)
     return 0;
 }
 
+extern "C" int dn_get_launch_shape(const dn_ctx *c, int *rows, int *graph, int *wave_shots, int *group_shots)
+{
+    if (!c) return dn_fail(nullptr, "dn_get_launch_shape: NULL context");
+    if (rows) *rows = c->rows_req;
+    if (graph) *graph = c->graph_steps;
+    if (wave_shots) *wave_shots = c->two_shots ? 2 : 1;
+    if (group_shots) *group_shots = c->sg;
+    return 0;
+}
+
 extern "C" int dn_set_group_layout(dn_ctx *c, int mode)
 {
     if (!c) return -1;

@@ -300,6 +300,13 @@ int dn_set_group_layout(dn_ctx *ctx, int mode);
  * each the faster of two timed passes; keeps the fastest setting and
  * returns rows (0: the cost model's) and graph use. */
 int dn_autotune(dn_ctx *ctx, int nshots, int nsteps, int *rows_per_segment, int *graph_steps);
+/* The launch shape in use (set by dn_set_batch / dn_set_exec /
+ * dn_set_wave_shots / dn_set_group_layout or chosen by dn_autotune): rows per
+ * segment (0: the cost model's choice per batch), graph steps, shots per wave
+ * and shots per workgroup (1: eight adjacent strips of one shot).  No
+ * counterpart in the reference. */
+int dn_get_launch_shape(const dn_ctx *ctx, int *rows_per_segment, int *graph_steps, int *wave_shots,
+                        int *group_shots);
 
 /* Forward modelling of shots [shot0, shot0+nshots): seismograms to seis
  * (host [nshots][ntr][nt_max]) when non-NULL. */
