@@ -41,7 +41,7 @@ typedef struct dn_globals {
     int FDORDER;         /* 2..12                                          */
     int FW;              /* CPML width                                     */
     int FREE_SURF;       /* 1: no top CPML (implicit free surface)         */
-    int BOUNDARY;        /* 0 or 1 (periodic left and right edges, NX >= 64) */
+    int BOUNDARY;        /* 0, or 1: periodic left and right edges (NX >= FDORDER) */
     int L;               /* relaxation mechanisms                          */
     int GRAD_FORM;       /* must be 2                                      */
     int QUELLTYPB;       /* adjoint source type (2: force in y)            */
