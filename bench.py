@@ -587,12 +587,12 @@ def main():
     # dominant kernel: the fused step kernel k_step (all but a fraction of a
     # percent of the GPU time), over the launches of both passes: the
     # forward pass moves 84 + 20/DTINV bytes per cell-update, the adjoint
-    # 84 + 36/DTINV (30/DTINV with 16-bit snapshots)
+    # 84 + 36/DTINV (14/DTINV and 30/DTINV with 16-bit snapshots)
     nla = max(la[0] + la[1], 1)
     per_launch_ms = (ms[0] + ms[1]) / nla
     units_per_launch = (ce[0] + ce[1]) / nla              # cell-updates per launch
     base = 12.0 * (3 + 2 * W["L"])
-    b_fwd = base + 20.0 / W["dtinv"]
+    b_fwd = base + (20.0 - (6.0 if prec != 0 else 0.0)) / W["dtinv"]
     b_adj = base + (36.0 - (6.0 if prec != 0 else 0.0)) / W["dtinv"]
     bytes_per_cell = (ce[0] * b_fwd + ce[1] * b_adj) / max(ce[0] + ce[1], 1)
     achieved = units_per_launch * bytes_per_cell / (per_launch_ms * 1e-3) / 1e9

@@ -698,7 +698,8 @@ static size_t per_shot_bytes(const dn_ctx *c, bool gradient)
     if (gradient) {
         const size_t ns = (size_t)dn_nsnap(c);
         b += (c->p.snap_precision == 0 ? 4 : 2) * 3 * nxny * ns;
-        if (c->p.snap_precision != 0) b += 3 * nxny * 4 + ns * 12;
+        if (c->p.snap_precision != 0) b += ns * 12 + 36;
+        if (c->p.snap_precision == 2) b += 3 * nxny * 4;   // FP16: FP32 staging plane
     }
     return b;
 }
@@ -753,8 +754,11 @@ int dn_ensure_batch(dn_ctx *c, int want, bool gradient)
         } else {
             HIPCHK(c, dalloc(&c->d_snap16, S * ns * 3 * nxny));
             HIPCHK(c, dalloc(&c->d_snap_scale, S * ns * 3));
-            HIPCHK(c, dalloc(&c->d_stage, S * 3 * nxny));
-            HIPCHK(c, dalloc(&c->d_stage_max, S * 3));
+            // BF16 is encoded by the storing step itself (a ring of three
+            // maxima per field); FP16 needs the snapshot's own maximum
+            // first, so it goes through an FP32 staging plane
+            if (c->p.snap_precision == 2) HIPCHK(c, dalloc(&c->d_stage, S * 3 * nxny));
+            HIPCHK(c, dalloc(&c->d_stage_max, S * 9));
         }
     }
     c->batch_cap = cap;
@@ -990,7 +994,8 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     if (!adjoint) {
         a.seis = record ? c->d_seis : nullptr;
         if (record) HIPCHK(c, launch_fill(c->d_seis, (size_t)n * c->ntr * c->nt_max, 0.0f, c->stream));
-        a.store_mode = store ? (kind == 0 ? 1 : 2) : 0;
+        a.store_mode = store ? (kind == 0 ? 1 : (kind == 1 ? 3 : 2)) : 0;
+        if (store && kind == 1) HIPCHK(c, launch_fill_u32(c->d_stage_max, (size_t)n * 9, 0u, c->stream));
         a.wacc = c->d_Ws;
         if (store) HIPCHK(c, launch_fill(c->d_Ws, (size_t)n * nxny, 0.0f, c->stream));
         if (c->hess_mode) {
@@ -1016,7 +1021,7 @@ int dn_run_pass(dn_ctx *c, const std::vector<int> &shots, bool adjoint, bool sto
     a.snap_shot_stride = (long long)nsnap * 3 * (long long)nxny;
     a.stage = c->d_stage;
     a.stage_max = c->d_stage_max;
-    const bool pack_mode = !adjoint && store && kind != 0;
+    const bool pack_mode = !adjoint && store && kind == 2;
     std::vector<char> corr_res(c->p.dtinv, 0);
     for (int sh : shots) corr_res[c->shot_nt[sh] % c->p.dtinv] = 1;
     const bool fwd_extra = store || c->hess_mode;

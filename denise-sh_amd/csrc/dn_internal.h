@@ -121,13 +121,16 @@ struct StepArgs {
     // gradient storage
     int dtinv;
     int store_mode;                         // forward: 0 no snapshots, 1 fp32 direct, 2 fp32 staging
+                                            // (fp16), 3 bf16 direct
     float *snap;                            // fp32 [s][nsnap][3][ny][nx] (store_mode 1 / adjoint fp32)
     long long snap_shot_stride;             // elements per shot
-    const uint16_t *snap16;                 // adjoint read of bf16/fp16 snapshots [s][nsnap][3][ny][nx]
-    const float *snap_scale;                // [s][nsnap][3] decode scale
+    uint16_t *snap16;                       // bf16/fp16 snapshots [s][nsnap][3][ny][nx]: written by
+                                            // store_mode 3 and k_snap_pack, read by the adjoint
+    float *snap_scale;                      // [s][nsnap][3] decode scale
     int snap_kind;                          // 0 fp32, 1 bf16, 2 fp16
     float *stage;                           // store_mode 2: [s][3][ny][nx]
-    unsigned int *stage_max;                // store_mode 2: [s][3] max |x| as float bits
+    unsigned int *stage_max;                // max |x| as float bits; store_mode 2: [s][3];
+                                            // store_mode 3: [s][3 snapshots, ring][3]
     float *wacc;                            // Ws (forward) or Wr (adjoint) [s][ny][nx]
     float *grad_u, *grad_rho;               // adjoint [s][ny][nx]; forward with hess: H_mu, H_rho sums
     int hess;                               // forward: accumulate the pseudo-Hessian (hess_save.c, hessian.c)
@@ -188,7 +191,7 @@ hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, cons
                              const float *vs_pad, const float *rho_pad, float DT, float *grad_vs,
                              float *grad_rho, hipStream_t st);
 
-// bf16/fp16 snapshot packing (store_mode 2)
+// fp16 snapshot packing (store_mode 2)
 // (snapshot slot = (nt_base ? *nt_base : 0) + nt - 1) / dtinv)
 hipError_t launch_snap_pack(int nshots, int nx, int ny, const float *stage, const unsigned int *stage_max,
                             uint16_t *snap16, float *snap_scale, long long shot_stride16, int nsnap,

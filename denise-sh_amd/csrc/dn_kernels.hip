@@ -322,6 +322,34 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     float *__restrict__ snw = (EX && a.snap) ? a.snap + sbase : nullptr;
     const uint16_t *__restrict__ snb16 = (EX && a.snap16) ? a.snap16 + sbase : nullptr;
     float *__restrict__ stg = (EX && a.stage) ? a.stage + (long long)s * 3 * nxny : nullptr;
+    uint16_t *__restrict__ sn16w = (EX && !ADJ && a.snap16) ? a.snap16 + sbase : nullptr;
+    // BF16 snapshots stored directly (store_mode 3): snapshot k of a field is
+    // encoded as bf16(x * 2^-e) and decoded by 2^e, with e the exponent of
+    // the largest |x| of snapshot k - 1 (0 for the first).  Scaling by a
+    // power of two leaves the rounding of normal values unchanged, so the
+    // scale only normalises the stored magnitudes.  The maxima go through a
+    // ring of three per shot and field: read k - 1, accumulate k, clear k + 1
+    // (read by the launch of snapshot k - 2, written by that of k + 1).
+    float enc[3] = {1.0f, 1.0f, 1.0f};
+    if (EX && !ADJ && fstore && a.store_mode == 3) {
+        unsigned int *ring = a.stage_max + (long long)s * 9;
+        const int kp = (slot + 2) % 3, kc = (slot + 1) % 3;
+        const bool first = live && strip == 0 && seg == 0 && lane == 0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const float m = __uint_as_float(ring[kp * 3 + f]);
+            int e = 0;
+            if (m > 0.0f && isfinite(m)) {
+                (void)frexpf(m, &e);
+                e = min(max(e, -100), 100);
+            }
+            enc[f] = ldexpf(1.0f, -e);
+            if (first) {
+                ring[kc * 3 + f] = 0u;
+                a.snap_scale[((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3 + f] = ldexpf(1.0f, e);
+            }
+        }
+    }
     float dsc[3] = {1.0f, 1.0f, 1.0f};
     if (S16 && corr) {
         const float *scp = a.snap_scale + ((long long)s * (a.snap_shot_stride / (3 * nxny)) + slot) * 3;
@@ -503,6 +531,9 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
                 if (a.store_mode == 1) {
                     snw[ci] = acc;
+                } else if (a.store_mode == 3) {
+                    sn16w[ci] = f2bf16(acc * enc[0]);
+                    smx0 = fmaxf(smx0, fabsf(acc));
                 } else {
                     stg[ci] = acc;
                     smx0 = fmaxf(smx0, fabsf(acc));
@@ -629,6 +660,12 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                             float *b0 = snw + ci;
                             b0[nxny] = vyy;        // uy  (forward_prop_y)
                             b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
+                        } else if (a.store_mode == 3) {
+                            uint16_t *b0 = sn16w + ci;
+                            b0[nxny] = f2bf16(vyy * enc[1]);
+                            b0[2 * nxny] = f2bf16(vyx * enc[2]);
+                            smx1 = fmaxf(smx1, fabsf(vyy));
+                            smx2 = fmaxf(smx2, fabsf(vyx));
                         } else {
                             float *b0 = stg + ci;
                             b0[nxny] = vyy;
@@ -678,16 +715,18 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         row(jv, cm, cr);
     }
 
-    // per-snapshot max |x| for the reduced-precision codec (store_mode 2)
-    if (EX && !ADJ && fstore && a.store_mode == 2 && live) {
-        // the maxima over the cells this wave staged (owned cells only)
+    // per-snapshot max |x| for the reduced-precision codecs (store_mode 2:
+    // the scale of this snapshot; 3: the scale of the next one)
+    if (EX && !ADJ && fstore && a.store_mode >= 2 && live) {
+        // the maxima over the cells this wave stored (owned cells only)
         const float m0 = wave_max(smx0);
         const float m1 = wave_max(smx1);
         const float m2 = wave_max(smx2);
+        unsigned int *mx = a.store_mode == 2 ? a.stage_max + s * 3 : a.stage_max + s * 9 + (slot % 3) * 3;
         if (lane == 0) {
-            atomicMax(&a.stage_max[s * 3 + 0], __float_as_uint(m0));
-            atomicMax(&a.stage_max[s * 3 + 1], __float_as_uint(m1));
-            atomicMax(&a.stage_max[s * 3 + 2], __float_as_uint(m2));
+            atomicMax(&mx[0], __float_as_uint(m0));
+            atomicMax(&mx[1], __float_as_uint(m1));
+            atomicMax(&mx[2], __float_as_uint(m2));
         }
     }
 }
@@ -1085,8 +1124,9 @@ hipError_t launch_chain_rule(const Geo &g, int nx, int ny, const float *gu, cons
 }
 
 // ---------------------------------------------------------------------------
-// Reduced-precision snapshot codec: x16 = round(x * T/max), decode scale
-// max/T per snapshot and field (T = 1 for bf16, 2^15 for fp16).
+// Staged snapshot codec (FP16 snapshots, store_mode 2): x16 = round(x *
+// T/max), decode scale max/T per snapshot and field (T = 2^15 for fp16, 1
+// for bf16; BF16 snapshots are normally encoded by the storing step).
 // ---------------------------------------------------------------------------
 __global__ void k_snap_pack(int nxny, const float *stage, const unsigned int *stage_max, uint16_t *snap16,
                             float *snap_scale, long long shot_stride16, int nsnap, const int *nt_base, int nt,

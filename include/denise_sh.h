@@ -179,7 +179,9 @@ typedef struct dn_params {
     float damping, fpml; /* DAMPING, FPML                                  */
     float npower, k_max_pml;
     int dtinv;           /* DTINV: snapshot every dtinv-th step            */
-    int snap_precision;  /* 0: fp32, 1: bf16, 2: fp16 (per-snapshot scale) */
+    int snap_precision;  /* 0: fp32; 1: bf16, scale 2^-e per snapshot and field with e
+                            the exponent of the previous snapshot's max |x|;
+                            2: fp16, scale 2^15/max |x| of the snapshot itself */
     int eprecond;        /* EPRECOND: 1 energy preconditioning sqrt(Ws Wr), 3 sqrt(Ws)
                             times the Plessix-Mulder receiver approximation
                             (eprecond1.c:15-50), 0 none */

@@ -23,7 +23,7 @@ class Cfg(C.Structure):
                 ("dt", C.c_double), ("dh", C.c_double), ("damping", C.c_double), ("fpml", C.c_double),
                 ("npower", C.c_double), ("k_max_pml", C.c_double), ("fl", C.c_double * 8), ("eprecond", C.c_int),
                 ("c_vs", C.c_double), ("c_rho", C.c_double), ("skip_res", C.c_void_p), ("skip_en", C.c_void_p),
-                ("periodic", C.c_int)]
+                ("periodic", C.c_int), ("snap_bf16", C.c_int)]
 
 
 def build():
@@ -31,10 +31,11 @@ def build():
 
 
 def make_cfg(nx, ny, fdorder=8, maxrelerror=1, fw=20, free_surf=1, L=0, nt=100, dtinv=1, dt=1e-4, dh=0.25,
-             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=(), eprecond=1, boundary=0):
+             damping=300.0, fpml=40.0, npower=2.0, k_max_pml=1.0, fl=(), eprecond=1, boundary=0, snap_bf16=0):
     c = Cfg()
     c.eprecond = eprecond
     c.periodic = 1 if boundary else 0
+    c.snap_bf16 = 1 if snap_bf16 else 0
     c.nx, c.ny, c.fdorder, c.maxrelerror, c.fw, c.free_surf, c.L, c.nt, c.dtinv = (
         nx, ny, fdorder, maxrelerror, fw, free_surf, L, nt, dtinv)
     # scalar parameters are float32 values in the reference (read as float)
@@ -188,6 +189,13 @@ class Oracle:
         return hv, hr
 
     # ---- single sweeps (padded arrays, shapes from padded_shape) ----
+    def snap_bf16(self, snaps):
+        """BF16 snapshot storage of a [nsnap][3][ny][nx] series, in place (orc_snap_bf16)."""
+        nsnap = snaps.shape[0]
+        nxny = int(np.prod(snaps.shape[2:]))
+        assert snaps.dtype == self.dt and snaps.flags.c_contiguous
+        self.fn("orc_snap_bf16")(nsnap, C.c_size_t(nxny), self._r(snaps))
+
     def wrap_x(self, cfg, a):
         """BOUNDARY 1 halo exchange of one padded field (exchange_v / exchange_s on one domain)."""
         self.fn("orc_wrap_x")(C.byref(cfg), self._r(a))

@@ -27,6 +27,7 @@
  * "parity unpinned".
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
@@ -78,6 +79,9 @@ typedef struct {
                             guards) and the wavefields wrapped across them
                             after each half step (exchange_v.c:99-161 for vy,
                             exchange_s.c:95-157 for sxy, FDORDER/2 columns) */
+    int snap_bf16;       /* 1: the gradient's forward snapshots are held in
+                            BF16 with a power-of-two scale per snapshot
+                            (orc_snap_bf16) before the adjoint reads them  */
 } orc_cfg;
 
 #define ND(c) ((c)->fdorder / 2 + 1)
@@ -1006,6 +1010,49 @@ static void sim_step(orc_sim *s, int sw, int ninj, const int *ii, const int *jj,
     if (c->periodic) F(orc_wrap_x)(c, s->sxy);  /* exchange_s (denise.c:1009) */
 }
 
+/* ------------------------------------------------------------------------
+ * BF16 snapshot storage (the product's reduced-precision option; the
+ * reference stores FP32 only).  Snapshot k of a field is replaced by
+ * bf16(x * 2^-e) * 2^e, with e the frexp exponent of max |x| over snapshot
+ * k - 1 of the same field (0 for k = 0 or a zero maximum; clamped to
+ * [-100, 100]); bf16 rounds to nearest even on the float bits.  snaps
+ * [nsnap][3][nxny] as written by orc_forward.
+ * ---------------------------------------------------------------------- */
+static float bf16_round(float v)
+{
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&v, &u, 4);
+    return v;
+}
+
+void F(orc_snap_bf16)(int nsnap, size_t nxny, real *snaps)
+{
+    float prev[3] = {0.0f, 0.0f, 0.0f};
+    int k, f;
+    size_t t;
+    for (k = 0; k < nsnap; k++)
+        for (f = 0; f < 3; f++) {
+            real *x = snaps + ((size_t)k * 3 + (size_t)f) * nxny;
+            float m = 0.0f, enc, dec;
+            int e = 0;
+            for (t = 0; t < nxny; t++) {
+                const float a = fabsf((float)x[t]);
+                if (a > m) m = a;
+            }
+            if (prev[f] > 0.0f && isfinite(prev[f])) {
+                (void)frexpf(prev[f], &e);
+                e = e < -100 ? -100 : (e > 100 ? 100 : e);
+            }
+            enc = ldexpf(1.0f, -e);
+            dec = ldexpf(1.0f, e);
+            for (t = 0; t < nxny; t++) x[t] = (real)(bf16_round((float)x[t] * enc) * dec);
+            prev[f] = m;
+        }
+}
+
 /* number of stored steps, denise.c:287 */
 int F(orc_ntdtinv)(int nt, int dtinv) { return (int)ceil((float)nt / (float)dtinv); }
 
@@ -1228,6 +1275,7 @@ static int gradient_impl(const orc_cfg *c, const float *vs, const float *rho, co
         rc = F(orc_forward)(c, vs, rho, qs, 1, &src_i[ishot], &src_j[ishot], signals + (size_t)ishot * NT, ntr,
                             rec_i, rec_j, seis, snaps, Ws);
         if (rc) break;
+        if (c->snap_bf16) F(orc_snap_bf16)(NTDTINV, (size_t)NX * NY, snaps);
         for (t = 0; t < (size_t)ntr * NT; t++)
             obsr[t] = obs ? (real)obs[(size_t)ishot * ntr * NT + t] : obs_real[(size_t)ishot * ntr * NT + t];
         F(orc_residual_cond)(ntr, NT, c->dt, seis, obsr, diff, l2, energy,

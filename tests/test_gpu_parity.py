@@ -239,6 +239,45 @@ def test_reduced_precision_snapshots(kind, tol, nx, ny):
     assert 0 < ev <= tol and er <= tol
 
 
+BF16_CASES = [
+    # name, nx, ny, fdorder, L, free_surf, fw, nt, dtinv, nshots, graph
+    ("visc_fd8_dtinv3_graph", 100, 44, 8, 2, 1, 10, 800, 3, 3, -1),
+    ("el_fd4_nofs_dtinv2_eager", 97, 43, 4, 0, 0, 10, 500, 2, 2, 0),
+    ("visc_fd6_dtinv4_graph", 90, 40, 6, 1, 0, 8, 600, 4, 2, -1),
+]
+
+
+@pytest.mark.parametrize("case", BF16_CASES, ids=[c[0] for c in BF16_CASES])
+def test_bf16_snapshots_match_oracle_bit_for_bit(case):
+    """BF16 snapshots are encoded by the storing step with a power-of-two
+    scale per snapshot and field, taken from the previous snapshot's
+    maximum.  The oracle holding its snapshots the same way
+    (orc_snap_bf16) gives the same gradient bit for bit, over graph replay
+    and eager launches."""
+    name, nx, ny, fdorder, L, fs, fw, nt, dtinv, nshots, graph = case
+    say("case bf16 snapshots", name)
+    ctx, cfg, d = setup_case(nx, ny, fdorder, L, fs, fw, nt, 60.0, nshots=nshots, dtinv=dtinv, snap_precision=1)
+    ctx.set_exec(graph)
+    obs = _obs_for(ctx, cfg, d, nx, ny, nshots)
+    ctx.set_observed(obs)
+    l2, en = ctx.gradient()
+    gvs, grho, raw_u, raw_rho = ctx.get_gradient()
+    cfg.snap_bf16 = 1
+    r = oracle("f32").gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"],
+                               d["rec_j"], obs)
+    cfg.snap_bf16 = 0
+    r32 = oracle("f32").gradient(cfg, d["vs"], d["rho"], d["qs"], d["src_i"], d["src_j"], d["sig"], d["rec_i"],
+                                 d["rec_j"], obs)
+    say("  grad_vs rel L2 vs oracle with bf16 snapshots %.3e (max abs diff %.3e); vs fp32 storage %.3e" %
+        (rel_l2(gvs, r["grad_vs"]), np.max(np.abs(gvs - r["grad_vs"])), rel_l2(gvs, r32["grad_vs"])))
+    assert np.max(np.abs(r["grad_vs"])) > 0
+    assert np.array_equal(gvs, r["grad_vs"]) and np.array_equal(grho, r["grad_rho"])
+    assert np.array_equal(raw_u, r["raw_u"]) and np.array_equal(raw_rho, r["raw_rho"])
+    # FP64 sums in another order
+    assert abs(l2 - r["l2"]) <= 1e-9 * abs(r["l2"]) and abs(en - r["energy"]) <= 1e-9 * abs(r["energy"])
+    assert rel_l2(gvs, r32["grad_vs"]) > 0
+
+
 def test_pseudo_hessian():
     """HESSIAN = 1 path: band-limited spike (QUELLART 6 low-passed at
     FC_HESSIAN), sums of squared forward fields, chain rule.  The reference's
