@@ -94,6 +94,8 @@ struct dn_ctx {
     dn_unet *unet = nullptr;
     int judge_every = 5;
     bool learned_fixed = true;       // D from the first iteration of a stage, kept for the stage
+    bool learned_revert = true;      // once the plain direction replaces the learned one, plain
+                                     // L-BFGS for the rest of the stage
     // gradient smoothing / taper workspace (dn_gradproc.hip)
     dn_gradproc_state *gradproc = nullptr;
     // data conditioning of the residual (dn_set_datacond): parameters and the

@@ -170,6 +170,7 @@ struct dn_fwi_state {
     float *feat = nullptr, *pred = nullptr;
     double last_plain_drop = -1.0;
     bool D_valid = false;      // D holds this stage's learned diagonal (fixed mode)
+    bool learned_off = false;  // the plain direction replaced the learned one in this stage
 };
 
 static void fwi_free(dn_fwi_state *f)
@@ -321,6 +322,7 @@ extern "C" int dn_fwi_start(dn_ctx *c, const dn_fwi_opts *o)
     f->iter = 1;
     f->lbfgs_ptr = 1;
     f->D_valid = false;
+    f->learned_off = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -594,6 +596,14 @@ extern "C" int dn_fwi_set_learned_fixed(dn_ctx *c, int fixed)
     return 0;
 }
 
+extern "C" int dn_fwi_set_learned_revert(dn_ctx *c, int per_stage)
+{
+    if (!c) return -1;
+    c->learned_revert = per_stage != 0;
+    if (c->fwi) c->fwi->learned_off = false;
+    return 0;
+}
+
 extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
 {
     if (!c) return -1;
@@ -710,9 +720,10 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
     }
     if (c->precond_hook && c->precond_hook(c, f->gu, f->grho)) return -1;
     // judge_every < 0: the learned direction only on the first -judge_every
-    // iterations of a stage, plain L-BFGS (with the same history) after
+    // iterations of a stage, plain L-BFGS (with the same history) after; the
+    // same after the plain direction replaced the learned one (revert mode)
     const bool learned = (c->unet != nullptr) && (f->o.grad_method == 2) &&
-                         !(c->judge_every < 0 && f->iter > -c->judge_every);
+                         !(c->judge_every < 0 && f->iter > -c->judge_every) && !f->learned_off;
     if (learned) {
         if (!f->r2) {
             HIPCHK(c, hipMalloc((void **)&f->r2, 2 * n * sizeof(float)));
@@ -774,6 +785,7 @@ extern "C" int dn_fwi_iteration(dn_ctx *c, dn_fwi_record *rec)
             R.judged = 1;
             R.drop_plain = (float)dp;
             take_learned = !fail && dl >= dp;
+            if (!take_learned && c->learned_revert) f->learned_off = true;
             if (!take_learned) {
                 std::memcpy(L2t, L2p, sizeof(L2p));
                 std::memcpy(epst1, epp, sizeof(epp));

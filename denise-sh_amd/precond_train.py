@@ -158,11 +158,11 @@ def train(net, X, T, steps=300, lr=2e-3, batch=8, seed=0):
     return hist
 
 
-def run_fwi(sv, vs_true, vs_start, rho, qs, iters, net=None, judge_every=3, **opts):
+def run_fwi(sv, vs_true, vs_start, rho, qs, iters, net=None, judge_every=3, revert=True, **opts):
     """FWI of vs (rho kept) on one survey; returns per-iteration records with
     the elapsed seconds, and the final model."""
     sv.setup(vs_true, vs_start, rho, qs)
-    sv.ctx.set_learned(net, judge_every)
+    sv.ctx.set_learned(net, judge_every, revert=revert)
     o = dict(testshot_start=1, testshot_end=sv.nshots, testshot_incr=2, vs_lower=50.0, vs_upper=800.0,
              rho_lower=500.0, rho_upper=4000.0, inv_rho_iter=1000000, eps_scale=0.01, stepmax=4)
     o.update(opts)

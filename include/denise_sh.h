@@ -612,6 +612,12 @@ int dn_fwi_set_learned(dn_ctx *ctx, dn_unet *net, int judge_every);
  * every iteration from the current energy maps and model.  Not in the
  * reference. */
 int dn_fwi_set_learned_fixed(dn_ctx *ctx, int fixed);
+/* per_stage 1 (default): once the plain direction replaces the learned one
+ * (the judge prefers it, or the learned line search finds no decrease), the
+ * rest of the stage runs plain L-BFGS with the same history, so plain and
+ * learned steps never alternate; 0: the learned direction is formed again at
+ * the next iteration.  Not in the reference. */
+int dn_fwi_set_learned_revert(dn_ctx *ctx, int per_stage);
 /* The network's input maps from the last gradient and the current model,
  * host [3][H8][W8] with H8, W8 = ny, nx rounded up to multiples of 8 (edge
  * rows/columns repeated): log10 of sum Ws and sum Wr relative to their

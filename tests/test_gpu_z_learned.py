@@ -39,22 +39,32 @@ def drop_of(rec):
     return float(np.float32(np.float64(np.float32(L[0] - min(L[1], L[2]))) / np.float64(L[0])))
 
 
-def check_judge_rules(recs, judge_every):
+def check_judge_rules(recs, judge_every, revert=True):
     """The rules of the learned mode on each record: the learned direction's
-    line search always runs; the plain one runs on the first iteration of the
+    line search runs; the plain one runs on the first iteration of the
     stage, every judge_every-th iteration and whenever the learned search
     found no decrease; when both ran, the learned direction is taken only
-    with a decrease at least as large as the plain one's."""
+    with a decrease at least as large as the plain one's.  With revert, once
+    the plain direction has been taken the rest of the stage is plain
+    L-BFGS: no learned search and no judge."""
+    off = False
     for r in recs:
         it = r["iter"]
-        due = it == 1 or (judge_every > 0 and (it - 1) % judge_every == 0)
-        assert r["drop_learned"] >= 0.0
-        if due or r["drop_learned"] == 0.0:
-            assert r["judged"] == 1, it
-            assert r["drop_plain"] >= 0.0
-            assert r["precond"] == int(r["drop_learned"] > 0.0 and r["drop_learned"] >= r["drop_plain"]), it
+        if off:
+            assert r["judged"] == 0 and r["precond"] == 0, it
+            assert r["drop_learned"] == -1.0 and r["drop_plain"] == -1.0, it
         else:
-            assert r["judged"] == 0 and r["precond"] == 1 and r["drop_plain"] == -1.0, it
+            due = it == 1 or (judge_every > 0 and (it - 1) % judge_every == 0)
+            assert r["drop_learned"] >= 0.0
+            if due or r["drop_learned"] == 0.0:
+                assert r["judged"] == 1, it
+                assert r["drop_plain"] >= 0.0
+                assert r["precond"] == int(r["drop_learned"] > 0.0 and r["drop_learned"] >= r["drop_plain"]), it
+                off = revert and r["precond"] == 0
+            else:
+                assert r["judged"] == 0 and r["precond"] == 1 and r["drop_plain"] == -1.0, it
+        if r["next_stage"]:
+            off = False
 
 
 def check_against_plain(learned, plain):
@@ -138,14 +148,17 @@ def test_network_from_1200x300_judged_on_held_out_4000x800():
     assert hist[-1] < hist[0]
     vt, v0, rho, qs = pt.random_checkerboard(4000, 800, 0.25, 4242)
     out = {}
-    for mode in ("plain", "learned"):
+    for mode in ("plain", "learned", "learned_no_revert"):
         sv = pt.Survey(dn, 4000, 800, **geom)
-        recs, vs = pt.run_fwi(sv, vt, v0, rho, qs, 3, net=net if mode == "learned" else None, judge_every=1)
+        recs, vs = pt.run_fwi(sv, vt, v0, rho, qs, 3, net=None if mode == "plain" else net, judge_every=1,
+                              revert=mode == "learned")
         sv.ctx.close()
         out[mode] = recs
-        say("4000x800 %-7s misfit %s  precond %s  drops %s  %.2f s" % (
+        say("4000x800 %-17s misfit %s  precond %s  drops %s  %.2f s" % (
             mode, " ".join("%.5g" % r["L2t"][3] for r in recs), "".join(str(r["precond"]) for r in recs),
             " ".join("%.4g/%.4g" % (r["drop_learned"], r["drop_plain"]) for r in recs), recs[-1]["seconds"]))
         assert all(np.isfinite(r["L2t"][3]) for r in recs)
-    check_judge_rules(out["learned"], 1)
-    check_against_plain(out["learned"], out["plain"])
+    check_judge_rules(out["learned"], 1, revert=True)
+    check_judge_rules(out["learned_no_revert"], 1, revert=False)
+    for m in ("learned", "learned_no_revert"):
+        check_against_plain(out[m], out["plain"])
