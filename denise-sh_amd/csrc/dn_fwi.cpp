@@ -937,13 +937,16 @@ extern "C" int dn_fwi_get_direction(dn_ctx *c, float *gvs, float *grho)
 // can take the previous model and gradient from those reference-named files.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr char kStateMagic[8] = {'D', 'N', 'F', 'W', 'I', 'S', '0', '1'};
+constexpr char kStateMagic[8] = {'D', 'N', 'F', 'W', 'I', 'S', '0', '2'};
 
 struct StateHead {
     char magic[8];
     int32_t nx, ny, grad_method, nlbfgs, iter, iter_true, stage, min_iter_help, lbfgs_ptr, nhist;
     float C_vs, C_rho;
     double last_plain_drop;
+    int32_t learned;   // learned preconditioner: bit 0 the stage's diagonal D follows the
+                       // arrays, bit 1 the stage reverted to plain L-BFGS
+    int32_t reserved;
 };
 
 // the device arrays of the state, in file order
@@ -981,6 +984,8 @@ extern "C" int dn_fwi_save_state(dn_ctx *c, const char *path)
     h.C_vs = c->C_vs;
     h.C_rho = c->C_rho;
     h.last_plain_drop = f->last_plain_drop;
+    const bool save_D = f->D_valid && f->D != nullptr;
+    h.learned = (save_D ? 1 : 0) | (f->learned_off ? 2 : 0);
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return dn_fail(c, "Could not open %s", path);
     bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1 &&
@@ -994,6 +999,14 @@ extern "C" int dn_fwi_save_state(dn_ctx *c, const char *path)
             return dn_fail(c, "dn_fwi_save_state: device copy failed");
         }
         ok = std::fwrite(buf.data(), sizeof(float), a.second, fp) == a.second;
+    }
+    if (ok && save_D) {
+        buf.resize(f->nxny);
+        if (hipMemcpy(buf.data(), f->D, f->nxny * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+            std::fclose(fp);
+            return dn_fail(c, "dn_fwi_save_state: device copy failed");
+        }
+        ok = std::fwrite(buf.data(), sizeof(float), f->nxny, fp) == f->nxny;
     }
     ok = (std::fclose(fp) == 0) && ok;
     return ok ? 0 : dn_fail(c, "Could not write %s", path);
@@ -1024,6 +1037,15 @@ extern "C" int dn_fwi_load_state(dn_ctx *c, const char *path, const char *jacobi
         ok = std::fread(buf.data(), sizeof(float), a.second, fp) == a.second &&
              hipMemcpy(a.first, buf.data(), a.second * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
     }
+    if (ok && (h.learned & 1)) {
+        if (!f->D && hipMalloc((void **)&f->D, f->nxny * sizeof(float)) != hipSuccess) {
+            std::fclose(fp);
+            return dn_fail(c, "dn_fwi_load_state: device allocation failed");
+        }
+        buf.resize(f->nxny);
+        ok = std::fread(buf.data(), sizeof(float), f->nxny, fp) == f->nxny &&
+             hipMemcpy(f->D, buf.data(), f->nxny * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    }
     std::fclose(fp);
     if (!ok) return dn_fail(c, "%s: truncated FWI state file", path);
     if (jacobian) {
@@ -1051,6 +1073,8 @@ extern "C" int dn_fwi_load_state(dn_ctx *c, const char *path, const char *jacobi
     f->min_iter_help = h.min_iter_help;
     f->lbfgs_ptr = h.lbfgs_ptr;
     f->last_plain_drop = h.last_plain_drop;
+    f->D_valid = (h.learned & 1) != 0;
+    f->learned_off = (h.learned & 2) != 0;
     c->C_vs = h.C_vs;
     c->C_rho = h.C_rho;
     c->C_fixed = true;

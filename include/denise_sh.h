@@ -582,7 +582,9 @@ int dn_fwi_write_files(dn_ctx *ctx, const dn_fwi_record *rec, const dn_fwi_files
  * keeps the L-BFGS ring and the PCG vectors in memory and so cannot resume):
  * dn_fwi_save_state writes iteration, stage and abort counters, the misfit
  * history, C_vs / C_rho, current and starting model, previous model and
- * normalised gradient, the PCG vectors and the whole L-BFGS ring.
+ * normalised gradient, the PCG vectors and the whole L-BFGS ring, and for
+ * the learned preconditioner the stage's fixed diagonal D and whether the
+ * stage reverted to plain L-BFGS (the network itself is the caller's).
  * dn_fwi_load_state restores it into a context after dn_fwi_start with the
  * same GRAD_METHOD and NLBFGS (model, shots and data set as for the saved
  * run); the next dn_fwi_iteration continues the saved run.  With jacobian

@@ -276,3 +276,46 @@ def test_fwi_resumes_bit_identically_from_state_files(tmp_path, grad_method):
             c.fwi_load_state(state, jacobian=jac, name_fix=0)
         assert ".old.00" in str(e.value)
         c.close()
+
+
+@pytest.mark.parametrize("revert", [True, False], ids=["revert", "no_revert"])
+def test_learned_fwi_resumes_bit_identically(tmp_path, revert):
+    """The learned preconditioner's state goes into the state file too: the
+    stage's fixed diagonal D and whether the stage reverted to plain L-BFGS.
+    Four learned iterations in one run equal two, dn_fwi_save_state, and two
+    more in a new context after dn_fwi_load_state, bit for bit (an untrained
+    network of a fixed seed: the test is about the state, not the gain)."""
+    dn = product()
+    O = oracle("f32")
+    net = dn.UNet(seed=5)
+    opts = dict(testshot_start=1, testshot_end=3, testshot_incr=2, vs_lower=50.0, vs_upper=800.0,
+                rho_lower=500.0, rho_upper=4000.0, eps_scale=0.01, stepmax=4, inv_rho_iter=0, grad_method=2)
+    say("learned resume, revert", revert)
+
+    def start():
+        P = _small_problem(dn, O)
+        ctx = P["ctx"]
+        ctx.set_learned(net, 0, fixed=True, revert=revert)
+        ctx.fwi_start(**opts)
+        return ctx
+
+    a = start()
+    full = [a.fwi_iteration() for _ in range(4)]
+    model_full = a.fwi_model()
+    a.close()
+    say("  directions %s, judged %s" % ("".join(str(r["precond"]) for r in full),
+                                       "".join(str(r["judged"]) for r in full)))
+    assert full[0]["judged"] == 1
+    state = str(tmp_path / "fwi_state.bin")
+    b = start()
+    for k in range(2):
+        assert b.fwi_iteration() == full[k]
+    b.fwi_save_state(state)
+    b.close()
+    c = start()
+    c.fwi_load_state(state)
+    for k in range(2, 4):
+        assert c.fwi_iteration() == full[k], k + 1
+    vs, rho = c.fwi_model()
+    assert np.array_equal(vs, model_full[0]) and np.array_equal(rho, model_full[1])
+    c.close()
