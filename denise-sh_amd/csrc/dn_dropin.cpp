@@ -488,11 +488,18 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
             ok = ok && a.psi_a && a.psi_b;
         }
         if (!ok || !a.vy || !a.vyp1 || !a.sxy || !a.syz || !a.rho) return;
+        // a few points ride in the sweep's launch, more take their own
+        const bool fused = pts.size() <= (size_t)kSplitPoints;
+        a.npts = fused ? (int)pts.size() : 0;
+        for (int k = 0; k < a.npts; ++k) {
+            a.pt_o[k] = pts[k].o;
+            a.pt_v[k] = pts[k].v;
+        }
         if (launch_split_v(a, d.stream) != hipSuccess) {
             fail("update_v_PML kernel failed");
             return;
         }
-        inject(d, a.vy, pts);
+        if (!fused) inject(d, a.vy, pts);
         return;
     }
     if (!put_pml(d, a.pml, vecs)) {

@@ -204,6 +204,7 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t st);
 hipError_t launch_fill_u32(unsigned int *p, size_t n, unsigned int v, hipStream_t st);
 
 // split (one field update per launch) kernels used by the reference-named drop-ins
+constexpr int kSplitPoints = 8;   // point additions carried by one k_split_v launch
 struct SplitArgs {
     Geo g;
     float DT, DH;
@@ -217,6 +218,11 @@ struct SplitArgs {
     float *r, *q;
     float bip[kMaxL], bjm[kMaxL], cip[kMaxL], cjm[kMaxL];
     float *psi_a, *psi_b;                   // v: psi_syy_y [2fw][nx], psi_sxy_x [ny][2fw]; s: psi_vyy, psi_vyx
+    // update_v: point additions after the sweep (update_v_PML.c:563-583),
+    // made by the thread of the cell, in list order
+    int npts;
+    long long pt_o[kSplitPoints];           // padded cell index
+    float pt_v[kSplitPoints];
 };
 hipError_t launch_split_v(const SplitArgs &a, hipStream_t st);
 hipError_t launch_split_s(const SplitArgs &a, bool visco, hipStream_t st);

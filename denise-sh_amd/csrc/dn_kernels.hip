@@ -1214,15 +1214,22 @@ __device__ __forceinline__ int xwrap(const Geo &g, int i)
     return !g.periodic ? i : (i < 1 ? i + g.nx : (i > g.nx ? i - g.nx : i));
 }
 
-__global__ void k_split_v(SplitArgs a)
+// K = FDORDER / 2 is a template parameter so the stencil loops unroll; the
+// arithmetic is the same in the same order for every K.  A workgroup takes
+// 64 columns of kSplitRows rows.
+constexpr int kSplitRows = 4;
+
+template <int K>
+__global__ __launch_bounds__(64 * kSplitRows) void k_split_v(SplitArgs a)
 {
     const Geo &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1;
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * kSplitRows + (int)(threadIdx.x / 64) + 1;
     if (i > g.nx || j > g.ny) return;
-    const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
+    const int fw = g.fw, fw2 = 2 * g.fw;
     float sxy_x = a.hc[1] * (a.sxy[gidx(g, j, i)] - a.sxy[gidx(g, j, xwrap(g, i - 1))]);
     float syy_y = a.hc[1] * (a.syz[gidx(g, j, i)] - a.syz[gidx(g, j - 1, i)]);
+#pragma unroll
     for (int m = 2; m <= K; ++m) {
         sxy_x = sxy_x + a.hc[m] * (a.sxy[gidx(g, j, xwrap(g, i + m - 1))] - a.sxy[gidx(g, j, xwrap(g, i - m))]);
         const float hy = (K == 6 && m == 6) ? a.hcy6 : a.hc[m];
@@ -1259,21 +1266,25 @@ __global__ void k_split_v(SplitArgs a)
     if ((double)rv < 1e-20) b = 0.0f;
     if (a.sw == 0) a.vyp1[o] = (b * (sxy_x + syy_y)) / a.DH;
     if (a.sw == 1) a.vyp1[o] = a.vy[o];
-    a.vy[o] = a.vy[o] + ((b * a.DT) * (sxy_x + syy_y)) / a.DH;
+    float vn = a.vy[o] + ((b * a.DT) * (sxy_x + syy_y)) / a.DH;
+    for (int k = 0; k < a.npts; ++k)
+        if (a.pt_o[k] == o) vn = vn + a.pt_v[k];
+    a.vy[o] = vn;
 }
 
-template <bool VISCO>
-__global__ void k_split_s(SplitArgs a)
+template <bool VISCO, int K>
+__global__ __launch_bounds__(64 * kSplitRows) void k_split_s(SplitArgs a)
 {
     const Geo &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y + 1;
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x % 64) + 1;
+    const int j = blockIdx.y * kSplitRows + (int)(threadIdx.x / 64) + 1;
     if (i > g.nx || j > g.ny) return;
-    const int K = a.K, fw = g.fw, fw2 = 2 * g.fw;
+    const int fw = g.fw, fw2 = 2 * g.fw;
     const float dhi = VISCO ? (float)(1.0 / (double)a.DH) : a.DT / a.DH;
     const float dthalbe = (float)((double)a.DT / 2.0);
     float vyx = a.hc[1] * (a.vy[gidx(g, j, xwrap(g, i + 1))] - a.vy[gidx(g, j, i)]);
     float vyy = a.hc[1] * (a.vy[gidx(g, j + 1, i)] - a.vy[gidx(g, j, i)]);
+#pragma unroll
     for (int m = 2; m <= K; ++m) {
         vyx = vyx + a.hc[m] * (a.vy[gidx(g, j, xwrap(g, i + m))] - a.vy[gidx(g, j, xwrap(g, i - m + 1))]);
         vyy = vyy + a.hc[m] * (a.vy[gidx(g, j + m, i)] - a.vy[gidx(g, j - m + 1, i)]);
@@ -1339,18 +1350,33 @@ __global__ void k_split_s(SplitArgs a)
     }
 }
 
-hipError_t launch_split_v(const SplitArgs &a, hipStream_t st)
+template <int K>
+static void launch_split_k(const SplitArgs &a, int which, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_split_v, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
-    return hipGetLastError();
-}
-hipError_t launch_split_s(const SplitArgs &a, bool visco, hipStream_t st)
-{
-    if (visco)
-        hipLaunchKernelGGL(k_split_s<true>, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
+    const dim3 grid((a.g.nx + 63) / 64, (a.g.ny + kSplitRows - 1) / kSplitRows), block(64 * kSplitRows);
+    if (which == 0)
+        hipLaunchKernelGGL((k_split_v<K>), grid, block, 0, st, a);
+    else if (which == 1)
+        hipLaunchKernelGGL((k_split_s<false, K>), grid, block, 0, st, a);
     else
-        hipLaunchKernelGGL(k_split_s<false>, dim3((a.g.nx + 63) / 64, a.g.ny), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_split_s<true, K>), grid, block, 0, st, a);
+}
+
+static hipError_t launch_split(const SplitArgs &a, int which, hipStream_t st)
+{
+    switch (a.K) {
+    case 1: launch_split_k<1>(a, which, st); break;
+    case 2: launch_split_k<2>(a, which, st); break;
+    case 3: launch_split_k<3>(a, which, st); break;
+    case 4: launch_split_k<4>(a, which, st); break;
+    case 5: launch_split_k<5>(a, which, st); break;
+    case 6: launch_split_k<6>(a, which, st); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
+
+hipError_t launch_split_v(const SplitArgs &a, hipStream_t st) { return launch_split(a, 0, st); }
+hipError_t launch_split_s(const SplitArgs &a, bool visco, hipStream_t st) { return launch_split(a, visco ? 2 : 1, st); }
 
 }  // namespace dn
