@@ -318,12 +318,14 @@ def fwi_run(ctx, W, args):
 
 LEARNED_MODES = {
     # name: (judge_every, D fixed per stage, plain for the rest of the stage
-    # once the plain direction replaced the learned one)
+    # once the plain direction replaced the learned one, power of D)
     "plain": None,
-    "learned_fixed": (0, True, True),
-    "learned_fixed_judge_every": (1, True, True),
-    "learned_recomputed": (0, False, True),
-    "learned_fixed_no_revert": (0, True, False),
+    "learned_fixed": (0, True, True, 1.0),
+    "learned_fixed_judge_every": (1, True, True, 1.0),
+    "learned_recomputed": (0, False, True, 1.0),
+    "learned_fixed_no_revert": (0, True, False, 1.0),
+    "learned_fixed_power05": (0, True, True, 0.5),
+    "learned_fixed_power15": (0, True, True, 1.5),
 }
 
 
@@ -390,7 +392,7 @@ def fwi_learned_run(dn, W, args, nshots, prec):
         if mode is None:
             ctx.set_learned(None)
         else:
-            ctx.set_learned(net, mode[0], mode[1], mode[2])
+            ctx.set_learned(net, mode[0], mode[1], mode[2], mode[3])
         ctx.fwi_start(testshot_start=1, testshot_end=nshots, testshot_incr=10, eps_scale=0.01, stepmax=6,
                       vs_lower=50.0, vs_upper=600.0)
         mis, sec, taken = [], [], []
@@ -425,10 +427,10 @@ def fwi_learned_run(dn, W, args, nshots, prec):
                      "loss_first_last": [round(hist[0], 5), round(hist[-1], 5)], "held_out": held,
                      "label_seconds": round(t_labels, 1), "train_seconds": round(t_train, 1)},
         "modes": {k: ("plain L-BFGS (eprecond 1)" if v is None else
-                      "learned, judge_every %d, diagonal %s, %s" % (
+                      "learned, judge_every %d, diagonal %s, %s, power %g" % (
                           v[0], "fixed per stage" if v[1] else "recomputed every iteration",
                           "plain for the rest of the stage once the plain direction is taken" if v[2]
-                          else "learned again after a plain iteration"))
+                          else "learned again after a plain iteration", v[3]))
                   for k, v in LEARNED_MODES.items() if k in names},
         "runs": runs,
         "config": {"workload": "visco SH FWI %dx%d FDORDER8 L2, %d shots x %d steps, DTINV %d"

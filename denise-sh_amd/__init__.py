@@ -32,7 +32,7 @@ EXPORTS = [
     "dn_calc_res_noalloc",
     "dn_set_time_filt", "zero_fdveps", "zero_fdveps_visc", "dn_dropin_resident",
     "dn_dropin_sync", "dn_fwi_save_state", "dn_fwi_load_state",
-    "dn_fwi_set_learned_fixed", "dn_get_launch_shape", "dn_fwi_set_learned_revert",
+    "dn_fwi_set_learned_fixed", "dn_get_launch_shape", "dn_fwi_set_learned_revert", "dn_fwi_set_learned_power",
 ]
 
 
@@ -319,13 +319,15 @@ class Context:
         self._chk(lib().dn_fwi_get_model(self._h, _f(vs), _f(rho)))
         return vs, rho
 
-    def set_learned(self, net, judge_every=0, fixed=True, revert=True):
+    def set_learned(self, net, judge_every=0, fixed=True, revert=True, power=1.0):
         lib().dn_fwi_set_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         self._chk(lib().dn_fwi_set_learned(self._h, net._h if net is not None else None, judge_every))
         lib().dn_fwi_set_learned_fixed.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_fwi_set_learned_fixed(self._h, 1 if fixed else 0))
         lib().dn_fwi_set_learned_revert.argtypes = [C.c_void_p, C.c_int]
         self._chk(lib().dn_fwi_set_learned_revert(self._h, 1 if revert else 0))
+        lib().dn_fwi_set_learned_power.argtypes = [C.c_void_p, C.c_float]
+        self._chk(lib().dn_fwi_set_learned_power(self._h, power))
         self._net = net      # keep the network alive while the context uses it
 
     def precond_features(self):

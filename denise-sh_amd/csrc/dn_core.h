@@ -96,6 +96,7 @@ struct dn_ctx {
     bool learned_fixed = true;       // D from the first iteration of a stage, kept for the stage
     bool learned_revert = true;      // once the plain direction replaces the learned one, plain
                                      // L-BFGS for the rest of the stage
+    double learned_power = 1.0;      // D = (We / H_pred)^power
     // gradient smoothing / taper workspace (dn_gradproc.hip)
     dn_gradproc_state *gradproc = nullptr;
     // data conditioning of the residual (dn_set_datacond): parameters and the

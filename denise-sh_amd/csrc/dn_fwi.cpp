@@ -604,6 +604,15 @@ extern "C" int dn_fwi_set_learned_revert(dn_ctx *c, int per_stage)
     return 0;
 }
 
+extern "C" int dn_fwi_set_learned_power(dn_ctx *c, float power)
+{
+    if (!c) return -1;
+    if (!(power > 0.0f && power <= 2.0f)) return dn_fail(c, "dn_fwi_set_learned_power: power must be in (0, 2]");
+    c->learned_power = power;
+    if (c->fwi) c->fwi->D_valid = false;
+    return 0;
+}
+
 extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
 {
     if (!c) return -1;
@@ -613,9 +622,10 @@ extern "C" int dn_fwi_set_learned(dn_ctx *c, dn_unet *net, int judge_every)
     return 0;
 }
 
-// D = We / H_pred (We = sqrt(sum Ws * sum Wr) + 0.005 max, the energy
-// weight of eprecond1.c; H_pred = 10^net), normalised to mean 1 and kept in
-// [1e-2, 1e2]: the learned diagonal relative to the plain preconditioning
+// D = (We / H_pred)^power (We = sqrt(sum Ws * sum Wr) + 0.005 max, the
+// energy weight of eprecond1.c; H_pred = 10^net; power 1 by default),
+// normalised to mean 1 and kept in [1e-2, 1e2]: the learned diagonal
+// relative to the plain preconditioning
 static int learned_diagonal(dn_ctx *c)
 {
     dn_fwi_state *f = c->fwi;
@@ -639,7 +649,7 @@ static int learned_diagonal(dn_ctx *c)
             const size_t t = (size_t)j * g.nx + i;
             const double we = std::sqrt((double)ws[t] * (double)wr[t]) + 0.005 * wmax;
             const double h = std::pow(10.0, std::min(30.0, std::max(-30.0, (double)pred[(size_t)j * W8 + i])));
-            D[t] = (float)(we / h);
+            D[t] = (float)(c->learned_power == 1.0 ? we / h : std::pow(we / h, c->learned_power));
             sum += D[t];
         }
     const double mean = sum / (double)nxny;

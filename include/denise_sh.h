@@ -620,6 +620,10 @@ int dn_fwi_set_learned_fixed(dn_ctx *ctx, int fixed);
  * learned steps never alternate; 0: the learned direction is formed again at
  * the next iteration.  Not in the reference. */
 int dn_fwi_set_learned_revert(dn_ctx *ctx, int per_stage);
+/* The learned diagonal as D = (We / H_pred)^power, power in (0, 2] (default
+ * 1): between the plain energy scaling (power -> 0) and the predicted Hessian
+ * diagonal (1).  Not in the reference. */
+int dn_fwi_set_learned_power(dn_ctx *ctx, float power);
 /* The network's input maps from the last gradient and the current model,
  * host [3][H8][W8] with H8, W8 = ny, nx rounded up to multiples of 8 (edge
  * rows/columns repeated): log10 of sum Ws and sum Wr relative to their
