@@ -278,6 +278,7 @@ def make_context(dn, W, args, nshots, prec, eprecond=1, shots=None):
     if args.rows or args.max_batch:
         ctx.set_batch(args.max_batch, args.rows)
     ctx.set_group_layout(args.layout)
+    ctx.set_batch_order(args.batch_order == "length")
     ctx.set_receivers(W["rec_i"], W["rec_j"])
     sel = np.arange(nshots) if shots is None else np.asarray(shots)
     sig = signals_of(dn, W, len(W["src_i"]))[sel]
@@ -509,6 +510,9 @@ def main():
     ap.add_argument("--layout", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="workgroups of 8 adjacent strips of one shot (0), of 8/S strips x S shots (S = 2, 4, 8; "
                          "1 = 8)")
+    ap.add_argument("--batch-order", default="length", choices=["length", "shots"],
+                    help="uneven record lengths over several batches: batches by decreasing length (default) or "
+                         "in shot order (the same bits)")
     ap.add_argument("--exec", default="graph", choices=["graph", "eager"],
                     help="time loop as replayed hipGraph chunks (default) or one host launch per step")
     ap.add_argument("--fwi", type=int, default=0,
@@ -639,6 +643,7 @@ def main():
     if args.max_batch:
         out["config"]["max_batch"] = args.max_batch
     out["config"]["exec"] = args.exec
+    out["config"]["batch_order"] = args.batch_order
     out["config"]["wave_shots"] = "tuned" if args.autotune else args.wave_shots
     out["config"]["group_layout"] = "tuned" if args.autotune else (
         "8 strips x 1 shot" if args.layout == 0 else "%d strips x %d shots" % (8 // (8 if args.layout == 1 else args.layout),

@@ -32,7 +32,7 @@ EXPORTS = [
     "dn_calc_res_noalloc",
     "dn_set_time_filt", "zero_fdveps", "zero_fdveps_visc", "dn_dropin_resident",
     "dn_dropin_sync", "dn_fwi_save_state", "dn_fwi_load_state",
-    "dn_fwi_set_learned_fixed", "dn_get_launch_shape", "dn_fwi_set_learned_revert", "dn_fwi_set_learned_power",
+    "dn_fwi_set_learned_fixed", "dn_get_launch_shape", "dn_fwi_set_learned_revert", "dn_fwi_set_learned_power", "dn_set_batch_order",
 ]
 
 
@@ -135,6 +135,8 @@ def lib():
         L.dn_set_shots.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, ip, ip, fp]
         L.dn_set_observed.argtypes = [C.c_void_p, fp]
         L.dn_set_batch.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.dn_set_batch_order.argtypes = [C.c_void_p, C.c_int]
+        L.dn_holbergcoeff.argtypes = [C.c_int, C.c_int, fp]
         L.dn_batch_shots.argtypes = [C.c_void_p, C.c_int, ip]
         L.dn_forward.argtypes = [C.c_void_p, C.c_int, C.c_int, fp]
         L.dn_gradient.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -365,6 +367,9 @@ class Context:
 
     def set_batch(self, max_shots=0, rows_per_segment=0):
         self._chk(lib().dn_set_batch(self._h, max_shots, rows_per_segment))
+
+    def set_batch_order(self, by_record_length=True):
+        self._chk(lib().dn_set_batch_order(self._h, 1 if by_record_length else 0))
 
     def batch_shots(self, gradient=True):
         """Shots advanced together in one batch (allocates the batch workspace)."""

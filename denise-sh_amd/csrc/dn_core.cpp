@@ -215,6 +215,11 @@ static void free_batch(dn_ctx *c)
     c->cur_map.clear();   // the shot map must be uploaded again into the new workspace
     c->batch_cap = 0;
     c->batch_nt_max = 0;
+    // allocated after the workspace, from what it leaves
+    dfree_t(c->d_terms);
+    c->terms_cap = 0;
+    dfree_t(c->d_rank);
+    c->rank_cap = 0;
 }
 
 extern "C" void dn_destroy(dn_ctx *c)
@@ -521,6 +526,13 @@ extern "C" int dn_set_batch(dn_ctx *c, int max_shots, int rows_per_segment)
     c->batch_req = std::max(0, max_shots);
     c->rows_req = std::max(0, rows_per_segment);
     free_batch(c);
+    return 0;
+}
+
+extern "C" int dn_set_batch_order(dn_ctx *c, int by_record_length)
+{
+    if (!c) return -1;
+    c->order_by_nt = by_record_length != 0;
     return 0;
 }
 
@@ -1206,19 +1218,76 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
         HIPCHK(c, launch_fill(c->d_gu, nxny, 0.0f, c->stream));
         HIPCHK(c, launch_fill(c->d_grho, nxny, 0.0f, c->stream));
     }
-    for (size_t b0 = 0; b0 < shots.size();) {
-        if (dn_ensure_batch(c, (int)(shots.size() - b0), true)) return -1;
-        const size_t n = std::min((size_t)c->batch_cap, shots.size() - b0);
-        std::vector<int> part(shots.begin() + b0, shots.begin() + b0 + n);
+    if (dn_ensure_batch(c, (int)shots.size(), true)) return -1;
+    // Uneven record lengths over several batches: the batches take the shots
+    // by decreasing length, so that shots of similar length advance together
+    // and few launches carry finished shots.  The sums over shots keep the
+    // shots' own order: each batch stores its per-shot terms at the shots'
+    // ranks and one kernel adds them up in rank order at the end, with the
+    // arithmetic of the in-order accumulation (the same bits).
+    std::vector<int> order = shots, rank_of;
+    bool by_nt = false;
+    if (c->order_by_nt && (int)shots.size() > c->batch_cap) {
+        int lo = c->shot_nt[shots[0]], hi = lo;
+        for (int s : shots) {
+            lo = std::min(lo, c->shot_nt[s]);
+            hi = std::max(hi, c->shot_nt[s]);
+        }
+        if (lo != hi) {
+            const size_t need = shots.size() * 4 * nxny;
+            if (c->terms_cap < need) {
+                dfree_t(c->d_terms);
+                c->terms_cap = 0;
+                if (dalloc(&c->d_terms, need) == hipSuccess)
+                    c->terms_cap = need;
+                else
+                    (void)hipGetLastError();   // no room: the batches keep the shots' order
+            }
+            if (c->rank_cap < c->batch_cap) {
+                dfree_t(c->d_rank);
+                c->rank_cap = 0;
+                if (dalloc(&c->d_rank, (size_t)c->batch_cap) == hipSuccess)
+                    c->rank_cap = c->batch_cap;
+                else
+                    (void)hipGetLastError();
+            }
+            by_nt = c->terms_cap >= need && c->rank_cap >= c->batch_cap;
+        }
+    }
+    if (by_nt) {
+        std::vector<int> idx(shots.size());
+        for (size_t k = 0; k < idx.size(); ++k) idx[k] = (int)k;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return c->shot_nt[shots[a]] > c->shot_nt[shots[b]]; });
+        rank_of = idx;
+        for (size_t k = 0; k < idx.size(); ++k) order[k] = shots[idx[k]];
+    }
+    std::vector<std::vector<int>> ranks;   // host sources of the rank uploads, alive until the sync below
+    for (size_t b0 = 0; b0 < order.size();) {
+        if (dn_ensure_batch(c, (int)(order.size() - b0), true)) return -1;
+        const size_t n = std::min((size_t)c->batch_cap, order.size() - b0);
+        std::vector<int> part(order.begin() + b0, order.begin() + b0 + n);
         if (dn_run_pass(c, part, false, true, true)) return -1;
         if (dn_batch_residual(c, part)) return -1;
         if (dn_run_pass(c, part, true, false, false)) return -1;
-        HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr, c->d_gu_shot,
-                                             c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs, c->C_rho, c->d_maxbits,
-                                             c->p.eprecond, c->d_ws_sum, c->d_wr_sum, b0 == 0 && !accumulate,
-                                             c->stream));
+        if (by_nt) {
+            ranks.emplace_back(rank_of.begin() + b0, rank_of.begin() + b0 + n);
+            HIPCHK(c, hipMemcpyAsync(c->d_rank, ranks.back().data(), n * sizeof(int), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, launch_eprecond_terms((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr, c->d_gu_shot,
+                                            c->d_grho_shot, c->C_vs, c->C_rho, c->d_maxbits, c->p.eprecond,
+                                            c->d_rank, c->d_terms, c->stream));
+        } else {
+            HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr,
+                                                 c->d_gu_shot, c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs,
+                                                 c->C_rho, c->d_maxbits, c->p.eprecond, c->d_ws_sum, c->d_wr_sum,
+                                                 b0 == 0 && !accumulate, c->stream));
+        }
         b0 += n;
     }
+    if (by_nt)
+        HIPCHK(c, launch_eprecond_sum((int)shots.size(), c->g.nx, c->g.ny, c->d_terms, c->d_gu, c->d_grho,
+                                      c->d_ws_sum, c->d_wr_sum, !accumulate, c->stream));
     HIPCHK(c, launch_chain_rule(c->g, c->g.nx, c->g.ny, c->d_gu, c->d_grho, c->d_vs, c->d_rho, c->p.dt,
                                 c->d_gvs_out, c->d_grho_out, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -50,6 +50,13 @@ struct dn_ctx {
     float *d_sig = nullptr, *d_obs = nullptr;
     // batch workspace
     int batch_req = 0, rows_req = 0, batch_cap = 0, batch_nt_max = 0, batch_fit = 0;
+    // gradients over several batches of uneven record lengths: batches by
+    // decreasing length, per-shot terms [rank][4][nx*ny] summed in shot order
+    bool order_by_nt = true;
+    float *d_terms = nullptr;
+    size_t terms_cap = 0;
+    int *d_rank = nullptr;
+    int rank_cap = 0;
     bool batch_has_snap = false;
     float *d_vy[2] = {nullptr, nullptr}, *d_sxy[2] = {nullptr, nullptr}, *d_syz[2] = {nullptr, nullptr};
     float *d_psx[2] = {nullptr, nullptr}, *d_psy[2] = {nullptr, nullptr};

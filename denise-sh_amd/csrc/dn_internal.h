@@ -179,6 +179,15 @@ hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, float DH, cons
                                       const float *gu_shot, const float *grho_shot, float *gu, float *grho,
                                       float C_vs, float C_rho, unsigned int *maxbits, int precond, float *ws_sum,
                                       float *wr_sum, int first, hipStream_t st);
+// the same in two parts, for batches that do not follow the shots' order:
+// the per-shot terms (Ws, Wr and the scaled gradients) of a batch stored at
+// the shots' ranks, terms [rank][4][nx*ny], then their sum in rank order,
+// with the arithmetic of launch_eprecond_accumulate
+hipError_t launch_eprecond_terms(int nshots, int nx, int ny, float DH, const float *Ws, const float *Wr,
+                                 const float *gu_shot, const float *grho_shot, float C_vs, float C_rho,
+                                 unsigned int *maxbits, int precond, const int *rank, float *terms, hipStream_t st);
+hipError_t launch_eprecond_sum(int nterms, int nx, int ny, const float *terms, float *gu, float *grho,
+                               float *ws_sum, float *wr_sum, int first, hipStream_t st);
 
 // finite-difference Hessian probes: Rademacher +-1 of (seed, probe, cell)
 hipError_t launch_probe_perturb(const Geo &g, const float *vs0_pad, float *vs_pad, float eps, unsigned int seed,

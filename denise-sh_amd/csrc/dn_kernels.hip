@@ -1047,6 +1047,78 @@ hipError_t launch_eprecond_accumulate(int nshots, int nx, int ny, float DH, cons
     return hipGetLastError();
 }
 
+__global__ void k_eprecond_terms(int nxny, int nx, float DH, const float *Ws, const float *Wr, const float *gu_shot,
+                                 const float *grho_shot, float C_vs, float C_rho, const unsigned int *maxbits,
+                                 int precond, const int *rank, float *terms)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (t >= nxny) return;
+    const float EPSILON_WE = 0.005f;
+    const long long o = (long long)s * nxny + t;
+    float su, sr;
+    if (precond) {
+        float we = energy_weight(precond, Ws[o], Wr[o], t, nx, DH);
+        we = we + (EPSILON_WE * __uint_as_float(maxbits[s]));
+        su = gu_shot[o] / (we * C_vs * C_vs);
+        sr = grho_shot[o] / (we * C_rho * C_rho);
+    } else {
+        su = gu_shot[o];
+        sr = grho_shot[o];
+    }
+    float *T = terms + (long long)rank[s] * 4 * nxny;
+    T[t] = Ws[o];
+    T[nxny + t] = Wr[o];
+    T[2 * (long long)nxny + t] = su;
+    T[3 * (long long)nxny + t] = sr;
+}
+
+__global__ void k_eprecond_sum(int nterms, int nxny, const float *terms, float *gu, float *grho, float *ws_sum,
+                               float *wr_sum, int first)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxny) return;
+    float au = gu[t], ar = grho[t];
+    float es = first ? 0.0f : ws_sum[t], er = first ? 0.0f : wr_sum[t];
+    for (int r = 0; r < nterms; ++r) {
+        const float *T = terms + (long long)r * 4 * nxny;
+        es = es + T[t];
+        er = er + T[nxny + t];
+        au = au + T[2 * (long long)nxny + t];
+        ar = ar + T[3 * (long long)nxny + t];
+    }
+    gu[t] = au;
+    grho[t] = ar;
+    ws_sum[t] = es;
+    wr_sum[t] = er;
+}
+
+hipError_t launch_eprecond_terms(int nshots, int nx, int ny, float DH, const float *Ws, const float *Wr,
+                                 const float *gu_shot, const float *grho_shot, float C_vs, float C_rho,
+                                 unsigned int *maxbits, int precond, const int *rank, float *terms, hipStream_t st)
+{
+    const int nxny = nx * ny;
+    if (precond) {
+        hipError_t e = launch_fill_u32(maxbits, (size_t)nshots, 0u, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_we_max, dim3(64, nshots), dim3(256), 0, st, nxny, nx, DH, precond, Ws, Wr, maxbits);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_eprecond_terms, dim3((nxny + 255) / 256, nshots), dim3(256), 0, st, nxny, nx, DH, Ws, Wr,
+                       gu_shot, grho_shot, C_vs, C_rho, maxbits, precond, rank, terms);
+    return hipGetLastError();
+}
+
+hipError_t launch_eprecond_sum(int nterms, int nx, int ny, const float *terms, float *gu, float *grho,
+                               float *ws_sum, float *wr_sum, int first, hipStream_t st)
+{
+    const int nxny = nx * ny;
+    hipLaunchKernelGGL(k_eprecond_sum, dim3((nxny + 255) / 256), dim3(256), 0, st, nterms, nxny, terms, gu, grho,
+                       ws_sum, wr_sum, first);
+    return hipGetLastError();
+}
+
 // Rademacher sign of (seed, probe, cell): one round of a 32-bit integer
 // hash (splitmix-style finaliser) of the combined key, top bit.
 __device__ inline float rademacher(unsigned int seed, int probe, int cell)

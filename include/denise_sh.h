@@ -248,6 +248,13 @@ int dn_set_time_filt(dn_ctx *ctx, int time_filt, float fc_start, float fc, int o
 /* Maximum number of shots advanced together (0 = automatic from free HBM)
  * and rows per workgroup segment of the time-step kernel (0 = automatic). */
 int dn_set_batch(dn_ctx *ctx, int max_shots_per_batch, int rows_per_segment);
+/* by_record_length 1 (default): a gradient whose shots have uneven record
+ * lengths and take several batches forms the batches by decreasing length
+ * (shots of similar length advance together, fewer launches carry finished
+ * shots); the sums over shots keep the shots' order, so the result is the
+ * same bits as with 0 (batches in shot order).  Not in the reference, whose
+ * shot loop is sequential (denise.c:875). */
+int dn_set_batch_order(dn_ctx *ctx, int by_record_length);
 
 /* Shots one batch advances together (gradient != 0: with snapshot storage),
  * sized from free HBM and the dn_set_batch cap; the batch workspace is

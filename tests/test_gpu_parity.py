@@ -439,6 +439,38 @@ def test_shot_group_layout_equals_strip_groups(snap):
 
 
 
+def test_batches_by_record_length_give_the_shot_order_sums():
+    """Seven shots of uneven record lengths in batches of three: batches
+    formed by decreasing length (the default) give the gradient, the misfit
+    and the energy maps of batches in shot order, of one shot at a time and
+    of one batch of all seven, bit for bit (the per-shot terms are summed in
+    shot order)."""
+    say("case batches by record length")
+    nx, ny, nt = 110, 46, 300
+    nts = [300, 180, 260, 120, 300, 220, 150]
+    ctx, cfg, d = setup_case(nx, ny, 8, 2, 1, 10, nt, 60.0, nshots=7, dtinv=3, nts=nts)
+    obs = _obs_for(ctx, cfg, d, nx, ny, 7)
+    for s, n in enumerate(nts):
+        obs[s, :, n:] = 0.0
+    ctx.set_observed(obs)
+    out = {}
+    for name, batch, by_len in (("length, 3", 3, True), ("shot order, 3", 3, False), ("one at a time", 1, True),
+                                ("all seven", 7, True)):
+        ctx.set_batch(batch)
+        ctx.set_batch_order(by_len)
+        l2, en = ctx.gradient()
+        out[name] = (ctx.get_gradient(), ctx.energy_maps(), l2, en)
+    ctx.set_batch(0)
+    ctx.set_batch_order(True)
+    a = out["length, 3"]
+    assert np.abs(a[0][0]).max() > 0
+    for name, b in out.items():
+        same = all(np.array_equal(x, y) for x, y in zip(a[0] + list(a[1]), b[0] + list(b[1])))
+        say("  %s: gradients and energy maps equal %s, misfit %.9e" % (name, same, b[2]))
+        assert same and a[2] == b[2] and a[3] == b[3], name
+    ctx.close()
+
+
 def test_gradient_means_are_those_of_the_first_gradient():
     """The normalisation means C_vs, C_rho come from the model of the first
     gradient and stay fixed (denise.c:746-819 computes them in the first

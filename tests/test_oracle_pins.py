@@ -510,6 +510,17 @@ def test_library_exports_every_declared_symbol():
     assert set(dn.EXPORTS) == names
 
 
+def test_binding_declares_the_argument_types_of_every_call():
+    """Every C-ABI function the Python binding calls has its argument types
+    declared somewhere in the binding: without them ctypes passes a context
+    handle as a C int and cuts the pointer to 32 bits."""
+    src = open(os.path.join(ROOT, "denise-sh_amd", "__init__.py")).read()
+    called = set(re.findall(r"(?:lib\(\)|\bL)\.(dn_\w+)\(", src))
+    declared = set(re.findall(r"\.(dn_\w+)\.argtypes\s*=", src))
+    assert called, "no calls parsed"
+    assert not sorted(called - declared), sorted(called - declared)
+
+
 def test_seismo_ssg_and_psource_host_semantics():
     dn = product()
     nx, ny, nt = 8, 6, 10
