@@ -86,16 +86,18 @@ template <int K, int L, bool ADJ, int EX, int NSH>
 __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs a)
 {
     // EX: 0 for the plain steps (most steps), which carry none of the
-    // snapshot code and its pointers; 1 stores snapshots / sums the
-    // pseudo-Hessian (forward) or correlates with FP32 snapshots (adjoint);
-    // 2 correlates with 16-bit snapshots, decoded where used (adjoint).
+    // snapshot code and its pointers; 1 stores FP32 snapshots or stages
+    // FP16 ones / sums the pseudo-Hessian (forward) or correlates with FP32
+    // snapshots (adjoint); 2 stores BF16 snapshots (forward) or correlates
+    // with 16-bit snapshots, decoded where used (adjoint).
     // NSH: shots per wave.  With two, a wave advances the same strip of two
     // shots of the batch: the static model fields and the receiver table are
     // loaded once for both, and the two shots' independent arithmetic hides
     // each other's latency.
     static_assert(NSH == 1 || (NSH == 2 && !EX), "two shots per wave only on steps without EX work");
-    static_assert(EX >= 0 && EX <= 2 && (EX < 2 || ADJ), "EX variant");
+    static_assert(EX >= 0 && EX <= 2, "EX variant");
     constexpr bool S16 = ADJ && EX == 2;
+    constexpr bool B16W = !ADJ && EX == 2;   // BF16 snapshots written by the storing step
     constexpr bool VISCO = L > 0;
     const int w = blockIdx.x;
     const int nblk = (NSH == 2) ? a.nblocks2 : (a.sg > 1 ? a.nblocks_sg : a.nblocks);
@@ -322,8 +324,8 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     float *__restrict__ snw = (EX && a.snap) ? a.snap + sbase : nullptr;
     const uint16_t *__restrict__ snb16 = (EX && a.snap16) ? a.snap16 + sbase : nullptr;
     float *__restrict__ stg = (EX && a.stage) ? a.stage + (long long)s * 3 * nxny : nullptr;
-    uint16_t *__restrict__ sn16w = (EX && !ADJ && a.snap16) ? a.snap16 + sbase : nullptr;
-    // BF16 snapshots stored directly (store_mode 3): snapshot k of a field is
+    uint16_t *__restrict__ sn16w = (B16W && a.snap16) ? a.snap16 + sbase : nullptr;
+    // BF16 snapshots stored directly (EX 2 forward): snapshot k of a field is
     // encoded as bf16(x * 2^-e) and decoded by 2^e, with e the exponent of
     // the largest |x| of snapshot k - 1 (0 for the first).  Scaling by a
     // power of two leaves the rounding of normal values unchanged, so the
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
     // ring of three per shot and field: read k - 1, accumulate k, clear k + 1
     // (read by the launch of snapshot k - 2, written by that of k + 1).
     float enc[3] = {1.0f, 1.0f, 1.0f};
-    if (EX && !ADJ && fstore && a.store_mode == 3) {
+    if (B16W && fstore) {
         unsigned int *ring = a.stage_max + (long long)s * 9;
         const int kp = (slot + 2) % 3, kc = (slot + 1) % 3;
         const bool first = live && strip == 0 && seg == 0 && lane == 0;
@@ -529,11 +531,11 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
             }
             if (EX && !ADJ && fstore && own_v) {
                 const float acc = (bb * tsum) / a.DH;  // vyp1, update_v_PML.c:184
-                if (a.store_mode == 1) {
-                    snw[ci] = acc;
-                } else if (a.store_mode == 3) {
+                if (B16W) {
                     sn16w[ci] = f2bf16(acc * enc[0]);
                     smx0 = fmaxf(smx0, fabsf(acc));
+                } else if (a.store_mode == 1) {
+                    snw[ci] = acc;
                 } else {
                     stg[ci] = acc;
                     smx0 = fmaxf(smx0, fabsf(acc));
@@ -656,16 +658,16 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
                     const unsigned ci = (unsigned)((js - 1) * g.nx + (cw - 1));
                     const float vc = vw[u][K - 1];
                     if (!ADJ && fstore) {
-                        if (a.store_mode == 1) {
+                        if (B16W) {
+                            uint16_t *b0 = sn16w + ci;
+                            b0[nxny] = f2bf16(vyy * enc[1]);   // uy
+                            b0[2 * nxny] = f2bf16(vyx * enc[2]);   // uyx
+                            smx1 = fmaxf(smx1, fabsf(vyy));
+                            smx2 = fmaxf(smx2, fabsf(vyx));
+                        } else if (a.store_mode == 1) {
                             float *b0 = snw + ci;
                             b0[nxny] = vyy;        // uy  (forward_prop_y)
                             b0[2 * nxny] = vyx;    // uyx (forward_prop_u)
-                        } else if (a.store_mode == 3) {
-                            uint16_t *b0 = sn16w + ci;
-                            b0[nxny] = f2bf16(vyy * enc[1]);
-                            b0[2 * nxny] = f2bf16(vyx * enc[2]);
-                            smx1 = fmaxf(smx1, fabsf(vyy));
-                            smx2 = fmaxf(smx2, fabsf(vyx));
                         } else {
                             float *b0 = stg + ci;
                             b0[nxny] = vyy;
@@ -715,14 +717,14 @@ __global__ __launch_bounds__(kWave * kStripsPerGroup) void k_step(const StepArgs
         row(jv, cm, cr);
     }
 
-    // per-snapshot max |x| for the reduced-precision codecs (store_mode 2:
-    // the scale of this snapshot; 3: the scale of the next one)
-    if (EX && !ADJ && fstore && a.store_mode >= 2 && live) {
+    // per-snapshot max |x| for the reduced-precision codecs (FP16 staging:
+    // the scale of this snapshot; BF16: the scale of the next one)
+    if (EX && !ADJ && fstore && (B16W || a.store_mode == 2) && live) {
         // the maxima over the cells this wave stored (owned cells only)
         const float m0 = wave_max(smx0);
         const float m1 = wave_max(smx1);
         const float m2 = wave_max(smx2);
-        unsigned int *mx = a.store_mode == 2 ? a.stage_max + s * 3 : a.stage_max + s * 9 + (slot % 3) * 3;
+        unsigned int *mx = B16W ? a.stage_max + s * 9 + (slot % 3) * 3 : a.stage_max + s * 3;
         if (lane == 0) {
             atomicMax(&mx[0], __float_as_uint(m0));
             atomicMax(&mx[1], __float_as_uint(m1));
@@ -748,7 +750,9 @@ static hipError_t launch_step_kl(const StepArgs &a, bool adjoint, bool extra, hi
         else
             hipLaunchKernelGGL((k_step<K, L, true, 0, 1>), grid, block, 0, st, a);
     } else {
-        if (extra)
+        if (extra && a.store_mode == 3)
+            hipLaunchKernelGGL((k_step<K, L, false, 2, 1>), grid, block, 0, st, a);
+        else if (extra)
             hipLaunchKernelGGL((k_step<K, L, false, 1, 1>), grid, block, 0, st, a);
         else if (two)
             hipLaunchKernelGGL((k_step<K, L, false, 0, 2>), grid, block, 0, st, a);
