@@ -323,6 +323,7 @@ LEARNED_MODES = {
     "plain": None,
     "learned_fixed": (0, True, True, 1.0),
     "learned_fixed_judge_every": (1, True, True, 1.0),
+    "learned_fixed_judge_5": (5, True, True, 1.0),
     "learned_recomputed": (0, False, True, 1.0),
     "learned_fixed_no_revert": (0, True, False, 1.0),
     "learned_fixed_power05": (0, True, True, 0.5),
