@@ -1243,15 +1243,15 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
                 else
                     (void)hipGetLastError();   // no room: the batches keep the shots' order
             }
-            if (c->rank_cap < c->batch_cap) {
+            if (c->rank_cap < (int)shots.size()) {
                 dfree_t(c->d_rank);
                 c->rank_cap = 0;
-                if (dalloc(&c->d_rank, (size_t)c->batch_cap) == hipSuccess)
-                    c->rank_cap = c->batch_cap;
+                if (dalloc(&c->d_rank, shots.size()) == hipSuccess)
+                    c->rank_cap = (int)shots.size();
                 else
                     (void)hipGetLastError();
             }
-            by_nt = c->terms_cap >= need && c->rank_cap >= c->batch_cap;
+            by_nt = c->terms_cap >= need && c->rank_cap >= (int)shots.size();
         }
     }
     if (by_nt) {
@@ -1261,8 +1261,11 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
                          [&](int a, int b) { return c->shot_nt[shots[a]] > c->shot_nt[shots[b]]; });
         rank_of = idx;
         for (size_t k = 0; k < idx.size(); ++k) order[k] = shots[idx[k]];
+        // the ranks of the whole order at once; batch b0.. reads d_rank + b0
+        HIPCHK(c, hipMemcpyAsync(c->d_rank, rank_of.data(), rank_of.size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    std::vector<std::vector<int>> ranks;   // host sources of the rank uploads, alive until the sync below
     for (size_t b0 = 0; b0 < order.size();) {
         if (dn_ensure_batch(c, (int)(order.size() - b0), true)) return -1;
         const size_t n = std::min((size_t)c->batch_cap, order.size() - b0);
@@ -1271,12 +1274,9 @@ int dn_gradient_shots(dn_ctx *c, const std::vector<int> &shots, bool accumulate)
         if (dn_batch_residual(c, part)) return -1;
         if (dn_run_pass(c, part, true, false, false)) return -1;
         if (by_nt) {
-            ranks.emplace_back(rank_of.begin() + b0, rank_of.begin() + b0 + n);
-            HIPCHK(c, hipMemcpyAsync(c->d_rank, ranks.back().data(), n * sizeof(int), hipMemcpyHostToDevice,
-                                     c->stream));
             HIPCHK(c, launch_eprecond_terms((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr, c->d_gu_shot,
                                             c->d_grho_shot, c->C_vs, c->C_rho, c->d_maxbits, c->p.eprecond,
-                                            c->d_rank, c->d_terms, c->stream));
+                                            c->d_rank + b0, c->d_terms, c->stream));
         } else {
             HIPCHK(c, launch_eprecond_accumulate((int)n, c->g.nx, c->g.ny, c->p.dh, c->d_Ws, c->d_Wr,
                                                  c->d_gu_shot, c->d_grho_shot, c->d_gu, c->d_grho, c->C_vs,

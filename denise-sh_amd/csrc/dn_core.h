@@ -55,7 +55,7 @@ struct dn_ctx {
     bool order_by_nt = true;
     float *d_terms = nullptr;
     size_t terms_cap = 0;
-    int *d_rank = nullptr;
+    int *d_rank = nullptr;            // [shots of the gradient] rank of each position of the order
     int rank_cap = 0;
     bool batch_has_snap = false;
     float *d_vy[2] = {nullptr, nullptr}, *d_sxy[2] = {nullptr, nullptr}, *d_syz[2] = {nullptr, nullptr};
