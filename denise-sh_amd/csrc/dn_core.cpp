@@ -343,6 +343,14 @@ int dn_update_means(dn_ctx *c)
     return 0;
 }
 
+extern "C" int dn_get_means(const dn_ctx *c, float *c_vs, float *c_rho)
+{
+    if (!c) return -1;
+    if (c_vs) *c_vs = c->C_vs;
+    if (c_rho) *c_rho = c->C_rho;
+    return c->C_fixed ? 0 : 1;
+}
+
 extern "C" int dn_get_model(dn_ctx *c, float *vs, float *rho)
 {
     if (!c) return -1;

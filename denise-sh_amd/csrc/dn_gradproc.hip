@@ -1,6 +1,8 @@
 // Gradient post-processing on the device: the spatially variable Gaussian
-// smoothing of smooth_grad.c (SPATFILTER 4, gauss_filt_var.c:10-215) and
-// the gradient tapers of taper_grad.c:11-584 (sws 1, 2, 3 and a user map).
+// smoothing of smooth_grad.c (SPATFILTER 4, gauss_filt_var.c:10-215), its
+// fixed Gaussian (SPATFILTER 3, gauss_filt.c:56-160: one weight table for
+// every cell, normalised before the sum) and the gradient tapers of
+// taper_grad.c:11-584 (sws 1, 2, 3 and a user map).
 //
 // Gaussian smoothing.  Each cell's half widths (hx, hz) follow its local
 // wavelength vs / FC_END; its kernel weights exp(-ii^2/sx - jj^2/sz) depend
@@ -183,6 +185,59 @@ bool make_gauss_plan(int nx, int ny, float DH, float WD_DAMP, float WD_DAMP1, fl
             h = std::max(h, std::max(P.pairs[p].hx, P.pairs[p].hz));
         }
     return true;
+}
+
+// gauss_filt.c:56-160 (SPATFILTER 3): one Gaussian for every cell.
+// FILT_SIZE_GRAD = round(WD_DAMP (C_vs / FC_END) / DH), made odd; sigma =
+// hfs / 2 in integer arithmetic; the weights, summed in order from zero (the
+// reference leaves its sum uninitialised: this is its result when the sum
+// starts at zero), are divided by the sum before the convolution, so the
+// pair's divisor is 1.  Returns 1 for FILT_SIZE_GRAD 0 (nothing to do), 0
+// for a plan, -1 with err set for sizes the reference cannot filter (a
+// negative size skips its normalisation; hfs < 2 gives sigma 0 and 0/0).
+int make_gauss_fixed_plan(int nx, int ny, float DH, float WD_DAMP, float FC_END, float C_vs, GaussPlan &P,
+                          std::string &err)
+{
+    const float lam = C_vs / FC_END;
+    const float r = (WD_DAMP * lam) / DH;
+    if (!(std::fabs(r) < 1e7f)) {
+        err = "smooth_grad: filter size out of range (check C_vs, WD_DAMP, FC_END)";
+        return -1;
+    }
+    int fs = (int)std::round(r);
+    if (fs == 0) return 1;
+    if (fs < 0) {
+        err = "smooth_grad: SPATFILTER 3 needs WD_DAMP > 0 (gauss_filt.c normalises only positive sizes)";
+        return -1;
+    }
+    if (!(fs % 2)) fs += 1;
+    const int hfs = fs / 2;
+    if (hfs < 2) {
+        err = "smooth_grad: SPATFILTER 3 needs FILT_SIZE_GRAD >= 5 (below, sigma = hfs / 2 is 0)";
+        return -1;
+    }
+    if (hfs > kMaxHalf) {
+        err = "smooth_grad: Gaussian half width above 1024 cells";
+        return -1;
+    }
+    const float sigma = (float)(hfs / 2);
+    const float s = (float)(2.0 * sigma * sigma);
+    P.cell.assign((size_t)nx * ny, 0);
+    P.pairs.assign(1, GPair{hfs, hfs, 0, 1.0f});
+    P.w.clear();
+    float sum = 0.0f;
+    for (int ii = -hfs; ii <= hfs; ++ii)
+        for (int jj = -hfs; jj <= hfs; ++jj) {
+            const float a = (float)(ii * ii) / s, b = (float)(jj * jj) / s;
+            const float k = (float)::exp((double)(-a - b));
+            P.w.push_back(k);
+            sum += k;
+        }
+    for (float &w : P.w) w /= sum;
+    P.tiles_x = (nx + kTileX - 1) / kTileX;
+    P.tiles_y = (ny + kTileY - 1) / kTileY;
+    P.tile_half.assign((size_t)P.tiles_x * P.tiles_y, hfs);
+    return 0;
 }
 
 // device copies of a plan plus a scratch plane; grown on demand
@@ -453,16 +508,25 @@ int dn_smooth_grad_device(dn_ctx *c, int spatfilter, float wd_damp, float wd_dam
                           float *g_rho)
 {
     if (spatfilter == 0) return 0;
-    if (spatfilter != 4)
-        return dn_fail(c, "smooth_grad: SPATFILTER %d is not available (0 none, 4 gauss_filt_var)", spatfilter);
+    if (spatfilter != 3 && spatfilter != 4)
+        return dn_fail(c, "smooth_grad: SPATFILTER %d is not available (0 none, 3 gauss_filt, 4 gauss_filt_var)",
+                       spatfilter);
     if (!(fc_end > 0.0f)) return dn_fail(c, "smooth_grad: FC_END must be > 0");
     const Geo &g = c->g;
-    std::vector<float> vel;
-    if (vs_host(c, vel)) return -1;
     GaussPlan P;
     std::string err;
-    if (!make_gauss_plan(g.nx, g.ny, c->p.dh, wd_damp, wd_damp1, fc_end, vel.data(), P, err))
-        return dn_fail(c, "%s", err.c_str());
+    if (spatfilter == 3) {
+        // C_vs: the vs mean the gradient normalisation uses (denise.c:746-819)
+        if (!(c->C_vs > 0.0f)) return dn_fail(c, "smooth_grad: SPATFILTER 3 needs C_vs (run a gradient first)");
+        const int rc = make_gauss_fixed_plan(g.nx, g.ny, c->p.dh, wd_damp, fc_end, c->C_vs, P, err);
+        if (rc < 0) return dn_fail(c, "%s", err.c_str());
+        if (rc > 0) return 0;
+    } else {
+        std::vector<float> vel;
+        if (vs_host(c, vel)) return -1;
+        if (!make_gauss_plan(g.nx, g.ny, c->p.dh, wd_damp, wd_damp1, fc_end, vel.data(), P, err))
+            return dn_fail(c, "%s", err.c_str());
+    }
     dn_gradproc_state *s = gp(c);
     const size_t n = (size_t)g.nx * g.ny;
     HIPCHK(c, upload_plan(P, s->gd, n, c->stream));
@@ -540,6 +604,27 @@ extern "C" int dn_gauss_filt_var(int nx, int ny, float dh, float wd_damp, float 
     std::string err;
     if (!make_gauss_plan(nx, ny, dh, wd_damp, wd_damp1, fc_end, vel, P, err))
         return dn_fail(nullptr, "%s", err.c_str());
+    const size_t n = (size_t)nx * ny;
+    SACHK(grow(&S.g, S.cap_g, n));
+    SACHK(upload_plan(P, S.gd, n, S.st));
+    SACHK(hipMemcpyAsync(S.g, grad, n * sizeof(float), hipMemcpyHostToDevice, S.st));
+    SACHK(run_gauss(nx, ny, P, S.gd, S.g, S.st));
+    SACHK(hipMemcpyAsync(grad, S.g, n * sizeof(float), hipMemcpyDeviceToHost, S.st));
+    SACHK(hipStreamSynchronize(S.st));
+    return 0;
+}
+
+extern "C" int dn_gauss_filt(int nx, int ny, float dh, float wd_damp, float fc_end, float c_vs, float *grad)
+{
+    if (nx < 1 || ny < 1 || !grad) return dn_fail(nullptr, "dn_gauss_filt: bad arguments");
+    if (!(fc_end > 0.0f) || !(c_vs > 0.0f)) return dn_fail(nullptr, "dn_gauss_filt: FC_END and C_vs must be > 0");
+    Standalone &S = standalone();
+    if (sa_init(S)) return -1;
+    GaussPlan P;
+    std::string err;
+    const int rc = make_gauss_fixed_plan(nx, ny, dh, wd_damp, fc_end, c_vs, P, err);
+    if (rc < 0) return dn_fail(nullptr, "%s", err.c_str());
+    if (rc > 0) return 0;
     const size_t n = (size_t)nx * ny;
     SACHK(grow(&S.g, S.cap_g, n));
     SACHK(upload_plan(P, S.gd, n, S.st));

@@ -210,6 +210,10 @@ const char *dn_last_error(const dn_ctx *ctx);
  * iteration only (denise.c:746-819). */
 int dn_set_model(dn_ctx *ctx, const float *vs, const float *rho, const float *qs);
 int dn_get_model(dn_ctx *ctx, float *vs, float *rho);
+/* The means C_vs, C_rho (the globals of denise.c:746-819 that the gradient
+ * normalisation and gauss_filt.c read); returns 1 while they are not taken
+ * yet (no gradient and no dn_fwi_start so far), 0 after. */
+int dn_get_means(const dn_ctx *ctx, float *c_vs, float *c_rho);
 
 /* Receivers: grid indices (1-based), duplicates must already be removed
  * (receiver.c:50-63 does this on the host). */
@@ -484,8 +488,12 @@ typedef struct dn_taper {
 /* smooth_grad.c:10-35 on the context's gradients (after dn_gradient):
  * SPATFILTER 4 = gauss_filt_var.c:10-215, the spatially variable Gaussian
  * whose half widths follow the local wavelength of the context's vs model
- * (WD_DAMP1 along x, WD_DAMP along y, times vs / FC_END); 0 = none.
- * SPATFILTER 1-3 are not available (error).  Bit-identical to the
+ * (WD_DAMP1 along x, WD_DAMP along y, times vs / FC_END); SPATFILTER 3 =
+ * gauss_filt.c:56-160, one Gaussian of FILT_SIZE_GRAD = round(WD_DAMP C_vs /
+ * FC_END / DH) cells (made odd; 0: nothing; sizes below 5 or negative are
+ * rejected: the reference's sigma is 0 or its kernel not normalised; its
+ * uninitialised weight sum is taken to start at zero); 0 = none.
+ * SPATFILTER 1-2 are not available (error).  Bit-identical to the
  * reference's float loop.  ms (may be NULL) receives the device time. */
 int dn_smooth_grad(dn_ctx *ctx, int spatfilter, float wd_damp, float wd_damp1, float fc_end, float *ms);
 /* taper_grad.c on the context's gradients: which 1 vs, 2 rho, 3 both;
@@ -497,6 +505,9 @@ int dn_taper_grad(dn_ctx *ctx, int which, const dn_taper *taper);
  * taper_out (may be NULL) receives the coefficients. */
 int dn_gauss_filt_var(int nx, int ny, float dh, float wd_damp, float wd_damp1, float fc_end, const float *vel,
                       float *grad);
+/* gauss_filt(waveconv) of fd.h:133 (gauss_filt.c:56-160, SPATFILTER 3) on a
+ * host array, with the globals DH, WD_DAMP, FC_END and C_vs as arguments. */
+int dn_gauss_filt(int nx, int ny, float dh, float wd_damp, float fc_end, float c_vs, float *grad);
 int dn_taper_grad_host(int nx, int ny, const dn_taper *taper, int nsrc, const int *src_i, const int *src_j, int nrec,
                        const int *rec_i, const int *rec_j, float *grad, float *taper_out);
 

@@ -284,6 +284,15 @@ class Oracle:
                                       C.c_double(f32(wd_damp1)), C.c_double(f32(fc_end)), self._f32(v), self._r(g))
         return g
 
+    def gauss_filt(self, grad, dh, wd_damp, fc_end, c_vs):
+        """gauss_filt.c (SPATFILTER 3); returns (filtered gradient, status 0 / 1 none / -1 rejected)."""
+        g = np.ascontiguousarray(grad, self.dt).copy()
+        ny, nx = g.shape
+        f32 = lambda x: float(np.float32(x))
+        rc = self.fn("orc_gauss_filt")(nx, ny, C.c_double(f32(dh)), C.c_double(f32(wd_damp)),
+                                       C.c_double(f32(fc_end)), C.c_double(f32(c_vs)), self._r(g))
+        return g, rc
+
     def taper_grad(self, grad, sws, dh, gradt=(0, 0, 0, 0), srtshape=1, srtradius=1.0, filtsize=1,
                    src_i=(), src_j=(), rec_i=(), rec_j=()):
         """Returns (tapered gradient, taper coefficients)."""

@@ -1593,6 +1593,63 @@ void F(orc_gauss_filt_var)(int nx, int ny, double dh_in, double wd_damp_in, doub
     free(tmp);
 }
 
+/* gauss_filt.c:56-160 (SPATFILTER 3): one Gaussian for the whole grid.
+ * lam = C_vs / FC_END; FILT_SIZE_GRAD = round(WD_DAMP lam / DH), made odd;
+ * 0 leaves the gradient as it is; hfs = |FILT_SIZE_GRAD| / 2, sigma = hfs / 2
+ * in integer arithmetic, s = 2 sigma^2; weights exp(-ii^2/s - jj^2/s) for
+ * ii (x, outer) and jj (y, inner) in [-hfs, hfs], summed in that order from
+ * zero (the reference leaves `sum` uninitialised: this is its result when it
+ * starts at zero) and divided by the sum; the gradient, continued with its
+ * edge values (clamped indices, gauss_filt.c:96-120), convolved with them in
+ * the same order.  Returns 0, 1 when FILT_SIZE_GRAD is 0 (no filtering), -1
+ * for a negative size (the reference then skips the normalisation) or
+ * hfs < 2 (sigma 0: the reference computes 0/0).
+ * Pins (tests/test_oracle_gauss_filt.py): a delta returns the weights, equal to the closed-form normalised Gaussian
+ * of the integer sigma; a constant is kept; scipy.ndimage.correlate with
+ * that kernel (mode "nearest") matches, and the kernel of a real-valued
+ * sigma = hfs / 2 does not. */
+int F(orc_gauss_filt)(int nx, int ny, double dh_in, double wd_damp_in, double fc_end_in, double c_vs_in, real *grad)
+{
+    const float DH = (float)dh_in, WD_DAMP = (float)wd_damp_in, FC_END = (float)fc_end_in, C_vs = (float)c_vs_in;
+    const float lam = C_vs / FC_END;
+    int fs = (int)round((WD_DAMP * lam) / DH);
+    if (fs == 0) return 1;
+    if (fs < 0) return -1;
+    if (!(fs % 2)) fs += 1;
+    const int hfs = abs(fs) / 2;
+    if (hfs < 2) return -1;
+    const float sigma = (float)(hfs / 2);
+    const float s = (float)(2.0 * sigma * sigma);
+    const int w = 2 * hfs + 1;
+    float *kern = (float *)malloc((size_t)w * w * sizeof(float));
+    float sum = 0.0f;
+    for (int ii = -hfs; ii <= hfs; ii++)
+        for (int jj = -hfs; jj <= hfs; jj++) {
+            const float k = (float)exp((double)(-((float)(ii * ii) / s) - ((float)(jj * jj) / s)));
+            kern[(ii + hfs) * w + (jj + hfs)] = k;
+            sum += k;
+        }
+    for (int t = 0; t < w * w; t++) kern[t] /= sum;
+    const size_t n = (size_t)nx * (size_t)ny;
+    real *tmp = (real *)malloc((n ? n : 1) * sizeof(real));
+    memcpy(tmp, grad, n * sizeof(real));
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            real conv = 0.0;
+            for (int ii = -hfs; ii <= hfs; ii++)
+                for (int jj = -hfs; jj <= hfs; jj++) {
+                    int y = j + jj, x = i + ii;
+                    y = y < 1 ? 1 : (y > ny ? ny : y);
+                    x = x < 1 ? 1 : (x > nx ? nx : x);
+                    conv += tmp[(size_t)(y - 1) * nx + (size_t)(x - 1)] * (real)kern[(ii + hfs) * w + (jj + hfs)];
+                }
+            grad[(size_t)(j - 1) * nx + (size_t)(i - 1)] = conv;
+        }
+    free(tmp);
+    free(kern);
+    return 0;
+}
+
 /* taper_grad.c:11-584 for sws 1 (vertical), 2 (horizontal) and 3 (taper
  * around the sources); the taper coefficients go to taper [NY][NX], the
  * gradient is multiplied by them and then zeroed where the reference zeroes

@@ -127,6 +127,62 @@ def test_context_smooth_and_taper_match_oracle():
     ctx.close()
 
 
+# (name, ny, nx, dh, WD_DAMP, FC_END, C_vs): FILT_SIZE_GRAD 7, 11, 41 and 101
+# (the last beyond the LDS window of a 64 x 4 tile), then 0 (nothing done)
+FIXED_CASES = [
+    ("size 7, ragged tiles", 61, 131, 0.25, 0.35, 40.0, 200.0),
+    ("size 11", 70, 150, 0.2, 0.55, 50.0, 200.0),
+    ("size 41", 50, 90, 0.25, 1.0, 20.0, 200.0),
+    ("size 101, unstaged", 50, 90, 0.25, 5.05, 40.0, 200.0),
+    ("size 0 leaves it", 20, 30, 0.25, 0.0, 40.0, 200.0),
+]
+
+
+@pytest.mark.parametrize("case", FIXED_CASES, ids=[c[0] for c in FIXED_CASES])
+def test_gauss_filt_bit_identical(case):
+    """gauss_filt (SPATFILTER 3) on a host array, bit-identical to the
+    oracle's float restatement."""
+    dn = product()
+    name, ny, nx, dh, wd, fc, cvs = case
+    say("case gauss_filt", name)
+    g = np.random.default_rng(8).standard_normal((ny, nx)).astype(np.float32)
+    got = dn.gauss_filt(g, dh, wd, fc, cvs)
+    want, rc = O32.gauss_filt(g, dh, wd, fc, cvs)
+    assert rc in (0, 1)
+    assert np.array_equal(bits(got), bits(want))
+    if rc == 1:
+        assert np.array_equal(got, g)
+    w64, _ = oracle("f64").gauss_filt(g.astype(np.float64), dh, wd, fc, cvs)
+    say("  rel L2 vs FP64 oracle %.2e" % rel_l2(got, w64))
+
+
+def test_gauss_filt_rejects_what_the_reference_cannot_filter():
+    dn = product()
+    g = np.ones((10, 12), np.float32)
+    for wd in (0.15, -0.25):          # FILT_SIZE_GRAD 3 (sigma 0) and a negative size
+        with pytest.raises(dn.DeniseError):
+            dn.gauss_filt(g, 0.25, wd, 40.0, 200.0)
+
+
+def test_context_smooth_grad_spatfilter_3_matches_oracle():
+    """SPATFILTER 3 on the context's gradients uses the context's C_vs (the
+    mean of the first gradient's model)."""
+    dn = product()
+    ctx, vs, rho, src_i, rec_i = _ctx_case(dn)
+    assert ctx.means() is None
+    ctx.gradient()
+    cvs, _ = ctx.means()
+    gvs, grho, _, _ = ctx.get_gradient()
+    ctx.smooth_grad(3, 0.35, 0.0, 40.0)
+    svs, srho, _, _ = ctx.get_gradient()
+    for g, s in ((gvs, svs), (grho, srho)):
+        w, rc = O32.gauss_filt(g, 0.25, 0.35, 40.0, cvs)
+        assert rc == 0
+        assert np.array_equal(bits(s), bits(w))
+    say("  C_vs %.4f" % cvs)
+    ctx.close()
+
+
 def test_fwi_iteration_with_smoothing_and_taper():
     """The FWI loop applies SPATFILTER 4 and the vertical taper to the summed
     gradient before the optimiser: the first model update is a multiple of
