@@ -26,7 +26,7 @@ EXPORTS = [
     "dn_set_exec", "dn_autotune", "dn_pseudo_hessian", "dn_get_energy_maps", "dn_hessian_probe",
     "dn_unet_create", "dn_unet_destroy", "dn_unet_num_params", "dn_unet_get_params", "dn_unet_set_params",
     "dn_unet_forward", "dn_unet_loss_grad", "dn_unet_train_step", "dn_fwi_set_learned", "dn_precond_features",
-    "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host", "dn_gauss_filt", "dn_get_means",
+    "dn_smooth_grad", "dn_taper_grad", "dn_gauss_filt_var", "dn_taper_grad_host", "dn_gauss_filt", "dn_get_means", "dn_smooth2",
     "dn_set_datacond", "dn_time_window", "dn_stf_shot", "dn_stf", "dn_fwi_write_files", "dn_set_wave_shots",
     "dn_set_group_layout", "dn_batch_shots", "dn_calc_res",
     "dn_calc_res_noalloc",
@@ -610,6 +610,17 @@ def timedomain_filt(data, dt, fc, order, method):
     if lib().dn_timedomain_filt(_f(d), d.shape[0], d.shape[1], dt, fc, order, method) != 0:
         raise DeniseError("dn_timedomain_filt failed")
     return d
+
+
+def smooth2(grad, wd_damp):
+    """smooth2.c (SPATFILTER 2) on a host array [ny][nx] (runs on the GPU)."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    ny, nx = g.shape
+    L = lib()
+    L.dn_smooth2.argtypes = [C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
+    if L.dn_smooth2(nx, ny, wd_damp, _f(g)) != 0:
+        raise DeniseError(L.dn_last_error(None).decode())
+    return g
 
 
 def gauss_filt(grad, dh, wd_damp, fc_end, c_vs):

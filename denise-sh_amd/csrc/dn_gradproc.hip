@@ -1,8 +1,18 @@
 // Gradient post-processing on the device: the spatially variable Gaussian
 // smoothing of smooth_grad.c (SPATFILTER 4, gauss_filt_var.c:10-215), its
 // fixed Gaussian (SPATFILTER 3, gauss_filt.c:56-160: one weight table for
-// every cell, normalised before the sum) and the gradient tapers of
+// every cell, normalised before the sum), its tridiagonal smoothing
+// (SPATFILTER 2, smooth2.c:28-165) and the gradient tapers of
 // taper_grad.c:11-584 (sws 1, 2, 3 and a user map).
+//
+// smooth2.  Every x system (one per grid row) and every z system (one per
+// column) has the same matrix, so the host factors the two matrices once
+// with tripd's float operations (util.c:638-671) and the device runs only
+// the substitutions: one thread per row for x, then one per column for z,
+// in tripd's order.  The reference's index ranges are kept: its gradient
+// sits at v[1..NX][1..NY] of a zeroed v[0..NX][0..NY] and it smooths over
+// 0..n-1, so the zero row and column take part and the last row and
+// column keep their values.
 //
 // Gaussian smoothing.  Each cell's half widths (hx, hz) follow its local
 // wavelength vs / FC_END; its kernel weights exp(-ii^2/sx - jj^2/sz) depend
@@ -102,6 +112,56 @@ __global__ __launch_bounds__(256) void k_gauss_filt_var(int nx, int ny, const fl
         }
     }
     out[t] = conv / P.sum;
+}
+
+// x substitutions of smooth2: thread j (1..ny-1) solves the system of grid
+// row j over v[0..nx-1][j], v[0][j] = 0 (the zero column), in place on
+// g[j-1][0..nx-2]; e, d: the factored x matrix (tripd's e[k-1] and d[k])
+__global__ void k_smooth2_x(int nx, int ny, float *__restrict__ g, const float *__restrict__ e,
+                            const float *__restrict__ d, float v0)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > ny - 1) return;
+    float *row = g + (size_t)(j - 1) * nx;   // v[ix][j] = row[ix - 1] for ix >= 1
+    float prev = v0;                         // b[0]
+    for (int k = 1; k < nx; ++k) {
+        const float b = row[k - 1] - e[k - 1] * prev;
+        row[k - 1] = b;
+        prev = b;
+    }
+    float bk = prev / d[nx - 1];             // b[n-1]
+    row[nx - 2] = bk;
+    for (int k = nx - 1; k >= 2; --k) {      // b[k-1] for k-1 >= 1; b[0] is not kept
+        const float b = row[k - 2] / d[k - 1] - e[k - 1] * bk;
+        row[k - 2] = b;
+        bk = b;
+    }
+}
+
+// z substitutions of smooth2: thread i (1..nx-1) solves the system of grid
+// column i over v[i][1..ny-1] (n = ny - 1), its first right-hand side plus
+// r1 w v[i][0] (v[i][0] = v0 = 0, the zero row after its x smoothing)
+__global__ void k_smooth2_z(int nx, int ny, float *__restrict__ g, const float *__restrict__ e,
+                            const float *__restrict__ d, float r1w, float v0)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (i > nx - 1) return;
+    float *col = g + (i - 1);                // v[i][iz + 1] = col[iz * nx]
+    const int n = ny - 1;
+    float prev = col[0] + (r1w * v0);
+    col[0] = prev;
+    for (int k = 1; k < n; ++k) {
+        const float b = col[(size_t)k * nx] - e[k - 1] * prev;
+        col[(size_t)k * nx] = b;
+        prev = b;
+    }
+    float bk = prev / d[n - 1];
+    col[(size_t)(n - 1) * nx] = bk;
+    for (int k = n - 1; k >= 1; --k) {
+        const float b = col[(size_t)(k - 1) * nx] / d[k - 1] - e[k - 1] * bk;
+        col[(size_t)(k - 1) * nx] = b;
+        bk = b;
+    }
 }
 
 __global__ void k_taper_apply(size_t n, float *__restrict__ g, const float *__restrict__ tap,
@@ -238,6 +298,57 @@ int make_gauss_fixed_plan(int nx, int ny, float DH, float WD_DAMP, float FC_END,
     P.tiles_y = (ny + kTileY - 1) / kTileY;
     P.tile_half.assign((size_t)P.tiles_x * P.tiles_y, hfs);
     return 0;
+}
+
+// smooth2.c:28-165: the x matrix (n = NX: diagonal 1 + 2 r2, 1 + r2 at both
+// ends, superdiagonal -r2) and the z matrix (n = NY - 1: 1 + 2 r1, 1 + r1 at
+// the end, -r1), built with the reference's float operands (w = 1: the window
+// is the whole grid) and factored as tripd does.  r1 = r2 = WD_DAMP^2 / 4.
+struct Smooth2Plan {
+    std::vector<float> ex, dx, ez, dz;
+    float r1 = 0.0f;
+};
+
+static void tripd_factor(std::vector<float> &d, std::vector<float> &e)
+{
+    const int n = (int)d.size();
+    for (int k = 1; k < n; ++k) {
+        const float temp = e[k - 1];
+        e[k - 1] = temp / d[k - 1];
+        d[k] -= temp * e[k - 1];
+    }
+}
+
+bool make_smooth2_plan(int nx, int ny, float WD_DAMP, Smooth2Plan &P, std::string &err)
+{
+    if (nx < 2 || ny < 3) {
+        err = "smooth_grad: SPATFILTER 2 needs NX >= 2 and NY >= 3";
+        return false;
+    }
+    float r1 = WD_DAMP, r2 = r1;
+    r1 = (float)(r1 * r1 * 0.25);
+    r2 = (float)(r2 * r2 * 0.25);
+    const float w = 1.0f;
+    P.dx.assign(nx, 0.0f);
+    P.ex.assign(nx, 0.0f);
+    for (int ix = 0; ix < nx - 1; ++ix) {
+        P.dx[ix] = (float)(1.0 + r2 * (w + w));
+        P.ex[ix] = -r2 * w;
+    }
+    P.dx[0] -= r2 * w;
+    P.dx[nx - 1] = (float)(1.0 + r2 * w);
+    tripd_factor(P.dx, P.ex);
+    const int n = ny - 1;
+    P.dz.assign(n, 0.0f);
+    P.ez.assign(n, 0.0f);
+    for (int iz = 0; iz < ny - 2; ++iz) {
+        P.dz[iz] = (float)(1.0 + r1 * (w + w));
+        P.ez[iz] = -r1 * w;
+    }
+    P.dz[n - 1] = (float)(1.0 + r1 * w);
+    tripd_factor(P.dz, P.ez);
+    P.r1 = r1 * w;
+    return true;
 }
 
 // device copies of a plan plus a scratch plane; grown on demand
@@ -451,13 +562,39 @@ hipError_t run_taper(const TaperMap &M, float *g, size_t n, float *d_tap, unsign
     return hipGetLastError();
 }
 
+// smooth2 on the device plane g [ny][nx] in place.  The factored matrices go
+// to the device on the first call with an empty host vector (ex, dx: nx
+// each; ez, dz: ny - 1 each); later calls with the same vector reuse them.
+// host must stay alive until the stream is synchronised.
+hipError_t run_smooth2(int nx, int ny, const Smooth2Plan &P, std::vector<float> &host, float **buf, size_t &cap,
+                       float *g, hipStream_t st)
+{
+    hipError_t e;
+    if (host.empty()) {
+        host.insert(host.end(), P.ex.begin(), P.ex.end());
+        host.insert(host.end(), P.dx.begin(), P.dx.end());
+        host.insert(host.end(), P.ez.begin(), P.ez.end());
+        host.insert(host.end(), P.dz.begin(), P.dz.end());
+        if ((e = grow(buf, cap, host.size())) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(*buf, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, st)) !=
+            hipSuccess)
+            return e;
+    }
+    const float *ex = *buf, *dx = ex + nx, *ez = dx + nx, *dz = ez + (ny - 1);
+    hipLaunchKernelGGL(k_smooth2_x, dim3((unsigned)((ny - 1 + 63) / 64)), dim3(64), 0, st, nx, ny, g, ex, dx, 0.0f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_smooth2_z, dim3((unsigned)((nx - 1 + 63) / 64)), dim3(64), 0, st, nx, ny, g, ez, dz, P.r1,
+                       0.0f);
+    return hipGetLastError();
+}
+
 // per-process device state of the context-free entry points
 struct Standalone {
     hipStream_t st = nullptr;
     GaussDev gd;
-    float *g = nullptr, *tap = nullptr;
+    float *g = nullptr, *tap = nullptr, *s2 = nullptr;
     unsigned char *zero = nullptr;
-    size_t cap_g = 0, cap_tap = 0, cap_zero = 0;
+    size_t cap_g = 0, cap_tap = 0, cap_zero = 0, cap_s2 = 0;
 };
 Standalone &standalone()
 {
@@ -472,9 +609,9 @@ Standalone &standalone()
 // ---------------------------------------------------------------------------
 struct dn_gradproc_state {
     GaussDev gd;
-    float *tap = nullptr;
+    float *tap = nullptr, *s2 = nullptr;
     unsigned char *zero = nullptr;
-    size_t cap_tap = 0, cap_zero = 0;
+    size_t cap_tap = 0, cap_zero = 0, cap_s2 = 0;
     float ms_smooth = 0.0f;
 };
 
@@ -483,6 +620,7 @@ void dn_gradproc_destroy(dn_ctx *c)
     if (!c || !c->gradproc) return;
     c->gradproc->gd.release();
     (void)hipFree(c->gradproc->tap);
+    (void)hipFree(c->gradproc->s2);
     (void)hipFree(c->gradproc->zero);
     delete c->gradproc;
     c->gradproc = nullptr;
@@ -508,11 +646,27 @@ int dn_smooth_grad_device(dn_ctx *c, int spatfilter, float wd_damp, float wd_dam
                           float *g_rho)
 {
     if (spatfilter == 0) return 0;
-    if (spatfilter != 3 && spatfilter != 4)
-        return dn_fail(c, "smooth_grad: SPATFILTER %d is not available (0 none, 3 gauss_filt, 4 gauss_filt_var)",
+    if (spatfilter < 2 || spatfilter > 4)
+        return dn_fail(c,
+                       "smooth_grad: SPATFILTER %d is not available (0 none, 2 smooth2, 3 gauss_filt, "
+                       "4 gauss_filt_var)",
                        spatfilter);
-    if (!(fc_end > 0.0f)) return dn_fail(c, "smooth_grad: FC_END must be > 0");
     const Geo &g = c->g;
+    if (spatfilter == 2) {
+        Smooth2Plan S2;
+        std::string err2;
+        if (!make_smooth2_plan(g.nx, g.ny, wd_damp, S2, err2)) return dn_fail(c, "%s", err2.c_str());
+        dn_gradproc_state *s = gp(c);
+        std::vector<float> host;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        if (g_vs) HIPCHK(c, run_smooth2(g.nx, g.ny, S2, host, &s->s2, s->cap_s2, g_vs, c->stream));
+        if (g_rho) HIPCHK(c, run_smooth2(g.nx, g.ny, S2, host, &s->s2, s->cap_s2, g_rho, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipEventElapsedTime(&s->ms_smooth, c->ev0, c->ev1));
+        return 0;
+    }
+    if (!(fc_end > 0.0f)) return dn_fail(c, "smooth_grad: FC_END must be > 0");
     GaussPlan P;
     std::string err;
     if (spatfilter == 3) {
@@ -609,6 +763,24 @@ extern "C" int dn_gauss_filt_var(int nx, int ny, float dh, float wd_damp, float 
     SACHK(upload_plan(P, S.gd, n, S.st));
     SACHK(hipMemcpyAsync(S.g, grad, n * sizeof(float), hipMemcpyHostToDevice, S.st));
     SACHK(run_gauss(nx, ny, P, S.gd, S.g, S.st));
+    SACHK(hipMemcpyAsync(grad, S.g, n * sizeof(float), hipMemcpyDeviceToHost, S.st));
+    SACHK(hipStreamSynchronize(S.st));
+    return 0;
+}
+
+extern "C" int dn_smooth2(int nx, int ny, float wd_damp, float *grad)
+{
+    if (nx < 1 || ny < 1 || !grad) return dn_fail(nullptr, "dn_smooth2: bad arguments");
+    Smooth2Plan P;
+    std::string err;
+    if (!make_smooth2_plan(nx, ny, wd_damp, P, err)) return dn_fail(nullptr, "%s", err.c_str());
+    Standalone &S = standalone();
+    if (sa_init(S)) return -1;
+    const size_t n = (size_t)nx * ny;
+    std::vector<float> host;
+    SACHK(grow(&S.g, S.cap_g, n));
+    SACHK(hipMemcpyAsync(S.g, grad, n * sizeof(float), hipMemcpyHostToDevice, S.st));
+    SACHK(run_smooth2(nx, ny, P, host, &S.s2, S.cap_s2, S.g, S.st));
     SACHK(hipMemcpyAsync(grad, S.g, n * sizeof(float), hipMemcpyDeviceToHost, S.st));
     SACHK(hipStreamSynchronize(S.st));
     return 0;

@@ -492,9 +492,12 @@ typedef struct dn_taper {
  * gauss_filt.c:56-160, one Gaussian of FILT_SIZE_GRAD = round(WD_DAMP C_vs /
  * FC_END / DH) cells (made odd; 0: nothing; sizes below 5 or negative are
  * rejected: the reference's sigma is 0 or its kernel not normalised; its
- * uninitialised weight sum is taken to start at zero); 0 = none.
- * SPATFILTER 1-2 are not available (error).  Bit-identical to the
- * reference's float loop.  ms (may be NULL) receives the device time. */
+ * uninitialised weight sum is taken to start at zero); SPATFILTER 2 =
+ * smooth2.c:28-165, tridiagonal smoothing with r = WD_DAMP^2 / 4 over the
+ * reference's index ranges (its zero row and column 0 take part, the last
+ * row and column are kept); 0 = none.  SPATFILTER 1 (FFTW wavenumber
+ * damping) is not available (error).  Bit-identical to the reference's
+ * float loops.  ms (may be NULL) receives the device time. */
 int dn_smooth_grad(dn_ctx *ctx, int spatfilter, float wd_damp, float wd_damp1, float fc_end, float *ms);
 /* taper_grad.c on the context's gradients: which 1 vs, 2 rho, 3 both;
  * sources and receivers are the context's. */
@@ -508,6 +511,9 @@ int dn_gauss_filt_var(int nx, int ny, float dh, float wd_damp, float wd_damp1, f
 /* gauss_filt(waveconv) of fd.h:133 (gauss_filt.c:56-160, SPATFILTER 3) on a
  * host array, with the globals DH, WD_DAMP, FC_END and C_vs as arguments. */
 int dn_gauss_filt(int nx, int ny, float dh, float wd_damp, float fc_end, float c_vs, float *grad);
+/* smooth2(grad) of fd.h:277 (smooth2.c:28-165, SPATFILTER 2) on a host
+ * array, with the global WD_DAMP as an argument (nx >= 2, ny >= 3). */
+int dn_smooth2(int nx, int ny, float wd_damp, float *grad);
 int dn_taper_grad_host(int nx, int ny, const dn_taper *taper, int nsrc, const int *src_i, const int *src_j, int nrec,
                        const int *rec_i, const int *rec_j, float *grad, float *taper_out);
 

@@ -284,6 +284,15 @@ class Oracle:
                                       C.c_double(f32(wd_damp1)), C.c_double(f32(fc_end)), self._f32(v), self._r(g))
         return g
 
+    def smooth2(self, grad, wd_damp):
+        """smooth2.c (SPATFILTER 2), float as the reference; returns the filtered [ny][nx] gradient."""
+        g = np.ascontiguousarray(grad, np.float32).copy()
+        ny, nx = g.shape
+        rc = self.fn("orc_smooth2")(nx, ny, C.c_double(float(np.float32(wd_damp))), self._f32(g))
+        if rc:
+            raise ValueError("orc_smooth2: grid too small")
+        return g
+
     def gauss_filt(self, grad, dh, wd_damp, fc_end, c_vs):
         """gauss_filt.c (SPATFILTER 3); returns (filtered gradient, status 0 / 1 none / -1 rejected)."""
         g = np.ascontiguousarray(grad, self.dt).copy()

@@ -1593,6 +1593,90 @@ void F(orc_gauss_filt_var)(int nx, int ny, double dh_in, double wd_damp_in, doub
     free(tmp);
 }
 
+/* tripd (util.c:638-671): symmetric tridiagonal solve, d diagonal, e
+ * superdiagonal, b right-hand side overwritten with the solution. */
+static void orc_tripd(float *d, float *e, float *b, int n)
+{
+    int k;
+    float temp;
+    for (k = 1; k < n; ++k) {
+        temp = e[k - 1];
+        e[k - 1] = temp / d[k - 1];
+        d[k] -= temp * e[k - 1];
+    }
+    for (k = 1; k < n; ++k) b[k] -= e[k - 1] * b[k - 1];
+    b[n - 1] /= d[n - 1];
+    for (k = n - 1; k > 0; --k) b[k - 1] = b[k - 1] / d[k - 1] - e[k - 1] * b[k];
+}
+
+/* smooth2.c:28-165 (SPATFILTER 2) on a gradient [ny][nx] (always float,
+ * as the reference), r1 = r2 = WD_DAMP^2 / 4.  The reference loads the
+ * gradient into v[1..n2][1..n1] of a zeroed matrix v[0..n2][0..n1] (n2 = NX,
+ * n1 = NY) and smooths over 0..n-1: the zero row and column 0 take part, and
+ * the last row and column (x = NX, y = NY) keep their values.  The window
+ * covers the whole grid, so its smoothing (rw = 0) is skipped.  Needs
+ * nx >= 2, ny >= 3 (returns -1 otherwise).
+ * Pins (tests/test_oracle_smooth2.py): the last row and column are kept; the
+ * two tridiagonal operators built from the stated structure, applied to the
+ * result in reverse order, give back the input (float rounding); the result
+ * is linear in the gradient. */
+int F(orc_smooth2)(int nx, int ny, double wd_damp_in, float *grad)
+{
+    const int n1 = ny, n2 = nx;
+    if (n2 < 2 || n1 < 3) return -1;
+    float r1 = (float)wd_damp_in, r2;
+    r2 = r1;
+    r1 = (float)(r1 * r1 * 0.25);
+    r2 = (float)(r2 * r2 * 0.25);
+    const int nmax = (n1 < n2) ? n2 : n1;
+    const size_t ld = (size_t)n1 + 1;
+    float *v = (float *)calloc(((size_t)n2 + 1) * ld, sizeof(float));
+    float *w = (float *)calloc(((size_t)n2 + 1) * ld, sizeof(float));
+    float *d = (float *)calloc((size_t)nmax + 1, sizeof(float));
+    float *e = (float *)calloc((size_t)nmax + 1, sizeof(float));
+    float *f = (float *)calloc((size_t)nmax + 1, sizeof(float));
+#define V(ix, iz) v[(size_t)(ix) * ld + (size_t)(iz)]
+#define W(ix, iz) w[(size_t)(ix) * ld + (size_t)(iz)]
+    for (int i = 1; i <= n2; i++)
+        for (int j = 1; j <= n1; j++) V(i, j) = grad[(size_t)(j - 1) * nx + (size_t)(i - 1)];
+    for (int ix = 0; ix < n2; ++ix)
+        for (int iz = 0; iz < n1; ++iz) W(ix, iz) = 1;
+    for (int iz = 0; iz < n1; ++iz) {
+        for (int ix = 0; ix < n2 - 1; ++ix) {
+            d[ix] = (float)(1.0 + r2 * (W(ix, iz) + W(ix + 1, iz)));
+            e[ix] = -r2 * W(ix + 1, iz);
+            f[ix] = V(ix, iz);
+        }
+        d[0] -= r2 * W(0, iz);
+        d[n2 - 1] = (float)(1.0 + r2 * W(n2 - 1, iz));
+        f[n2 - 1] = V(n2 - 1, iz);
+        orc_tripd(d, e, f, n2);
+        for (int ix = 0; ix < n2; ++ix) V(ix, iz) = f[ix];
+    }
+    for (int ix = 0; ix < n2; ++ix) {
+        for (int iz = 0; iz < n1 - 2; ++iz) {
+            d[iz] = (float)(1.0 + r1 * (W(ix, iz + 1) + W(ix, iz + 2)));
+            e[iz] = -r1 * W(ix, iz + 2);
+            f[iz] = V(ix, iz + 1);
+        }
+        f[0] += r1 * W(ix, 1) * V(ix, 0);
+        d[n1 - 2] = (float)(1.0 + r1 * W(ix, n1 - 1));
+        f[n1 - 2] = V(ix, n1 - 1);
+        orc_tripd(d, e, f, n1 - 1);
+        for (int iz = 0; iz < n1 - 1; ++iz) V(ix, iz + 1) = f[iz];
+    }
+    for (int i = 1; i <= n2; i++)
+        for (int j = 1; j <= n1; j++) grad[(size_t)(j - 1) * nx + (size_t)(i - 1)] = V(i, j);
+#undef V
+#undef W
+    free(v);
+    free(w);
+    free(d);
+    free(e);
+    free(f);
+    return 0;
+}
+
 /* gauss_filt.c:56-160 (SPATFILTER 3): one Gaussian for the whole grid.
  * lam = C_vs / FC_END; FILT_SIZE_GRAD = round(WD_DAMP lam / DH), made odd;
  * 0 leaves the gradient as it is; hfs = |FILT_SIZE_GRAD| / 2, sigma = hfs / 2

@@ -1,5 +1,6 @@
 """Gradient post-processing on the GPU against the oracle (denise.c:1818-1847):
-gauss_filt_var (SPATFILTER 4) and taper_grad (sws 1, 2, 3, 5), bit-identical
+gauss_filt_var (SPATFILTER 4), gauss_filt (3), smooth2 (2) and taper_grad
+(sws 1, 2, 3, 5), bit-identical
 to the oracle's float restatement, on host arrays and on the context's
 gradient, plus an FWI iteration with both switched on."""
 import sys
@@ -162,6 +163,48 @@ def test_gauss_filt_rejects_what_the_reference_cannot_filter():
     for wd in (0.15, -0.25):          # FILT_SIZE_GRAD 3 (sigma 0) and a negative size
         with pytest.raises(dn.DeniseError):
             dn.gauss_filt(g, 0.25, wd, 40.0, 200.0)
+
+
+# (name, ny, nx, WD_DAMP)
+SMOOTH2_CASES = [
+    ("61 x 131", 61, 131, 3.0),
+    ("smallest grid", 3, 2, 1.5),
+    ("300 x 200, strong", 200, 300, 5.0),
+    ("40 x 57, weak", 40, 57, 0.7),
+]
+
+
+@pytest.mark.parametrize("case", SMOOTH2_CASES, ids=[c[0] for c in SMOOTH2_CASES])
+def test_smooth2_bit_identical(case):
+    """smooth2 (SPATFILTER 2) on a host array, bit-identical to the oracle's
+    float restatement (the reference's index ranges: the last row and column
+    are kept)."""
+    dn = product()
+    name, ny, nx, wd = case
+    say("case smooth2", name)
+    g = np.random.default_rng(9).standard_normal((ny, nx)).astype(np.float32)
+    got = dn.smooth2(g, wd)
+    want = O32.smooth2(g, wd)
+    assert np.array_equal(bits(got), bits(want))
+    assert np.array_equal(got[-1], g[-1]) and np.array_equal(got[:, -1], g[:, -1])
+
+
+def test_smooth2_rejects_grids_below_three_rows():
+    dn = product()
+    with pytest.raises(dn.DeniseError):
+        dn.smooth2(np.ones((2, 10), np.float32), 1.0)
+
+
+def test_context_smooth_grad_spatfilter_2_matches_oracle():
+    dn = product()
+    ctx, vs, rho, src_i, rec_i = _ctx_case(dn)
+    ctx.gradient()
+    gvs, grho, _, _ = ctx.get_gradient()
+    ctx.smooth_grad(2, 2.5, 0.0, 40.0)
+    svs, srho, _, _ = ctx.get_gradient()
+    for g, s in ((gvs, svs), (grho, srho)):
+        assert np.array_equal(bits(s), bits(O32.smooth2(g, 2.5)))
+    ctx.close()
 
 
 def test_context_smooth_grad_spatfilter_3_matches_oracle():
